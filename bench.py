@@ -1,0 +1,211 @@
+#!/usr/bin/env python
+"""Benchmark of the run-time stepping physics on one MI355X.
+
+Headline workload (one "step" = one pass of the hot path over one batch):
+1M fresh tracks, e-:e+:gamma = 4:1:5 by track id, kinetic energy log-uniform in
+[100 keV, 10 GeV], isotropic directions, couples uniform over
+{G4_PbWO4, G4_Pb, G4_lAr, G4_Si} at a 0.7 mm production cut, safety
+log-uniform in [1 um, 10 mm], one Philox stream per (seed, track id); one
+HowFar + AlongStep + PostStep Perform of every track in FP64 (hepem_step:
+particle-type compaction + fused physics kernel). Inputs are resident in HBM
+before the timed region; every step starts from the same input batch and writes
+a separate output batch.
+
+Also reported: the FP32-table form of the same step, config-1 style lookups
+(100k and 10M e- queries over G4_Pb/G4_lAr) and the oracle's CPU throughput on
+the same workload on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SET1 = ["G4_Pb", "G4_lAr"]
+SET2 = ["G4_PbWO4", "G4_Pb", "G4_lAr", "G4_Si"]
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(ptype: np.ndarray, out_of_place: bool = True) -> int:
+    """HBM bytes the step kernel must move per launch: the per-track state it
+    reads and writes, the queue entry, the step results and both secondary
+    slots (SoA field sizes of include/g4hepem_amd.h)."""
+    from g4hepem_amd._native import OUT_FIELDS
+    sz = lambda dt, c: np.dtype(dt).itemsize * c  # noqa: E731
+    common_r = sz("int8", 1) + sz("int32", 1) * 2 + sz("float64", 1) + sz("float64", 3) + sz("float64", 1) + 1 \
+        + sz("float64", 4) + 4 + 8 + 1  # ptype id imc ekin dir safety onb nial rng(ctr gauss flag)
+    msc = 3 * 8 + 1
+    queue = 4
+    state_w = 8 + 24 + 32 + 4 + 8 + 1  # ekin dir nial rng
+    ident_w = (1 + 4 + 4 + 8 + 1) if out_of_place else 0
+    res_w = sum(sz(dt, c) for dt, c in OUT_FIELDS.values())
+    e_bytes = common_r + msc + queue + state_w + msc + ident_w + res_w
+    g_bytes = common_r + queue + state_w + ident_w + res_w
+    ne = int(np.count_nonzero(ptype != 2))
+    return ne * e_bytes + (len(ptype) - ne) * g_bytes
+
+
+def time_steps(ctx, tin, tout, res, work, steps, warmup, seed):
+    import torch
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        ctx.step_prepare(tin, work)
+        ctx.step_run(seed, tin, tout, res, work)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for k in range(steps):
+        ctx.step_prepare(tin, work)
+        ev[k][0].record(stream)
+        ctx.step_run(seed, tin, tout, res, work)
+        ev[k][1].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    run_ms = [a.elapsed_time(b) for a, b in ev]
+    return total_ms / steps, float(np.mean(run_ms)), wall
+
+
+def time_lookups(ctx, n, reps, seed=2026):
+    import torch
+    from g4hepem_amd.tracks import lookup_queries
+    imc, e, le = lookup_queries(n, seed, len(SET1))
+    d = torch.device("cuda")
+    timc, te, tle = (torch.from_numpy(np.ascontiguousarray(x)).to(d) for x in (imc, e, le))
+    out = torch.empty(5 * n, dtype=torch.float64, device=d)
+    for _ in range(3):
+        ctx.lookup(0, timc, te, tle, out)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        ctx.lookup(0, timc, te, tle, out)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    return n / (ms * 1e-3), ms
+
+
+def cpu_baseline(tables, tracks: dict, min_seconds: float, seed: int):
+    """The oracle (C restatement) on the host cores: repeated passes over a
+    bounded prefix of the same workload until min_seconds of CPU work."""
+    from oracle.oracle_py import Oracle
+    o = Oracle()
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(int(cores), 64))
+    n_sample = min(len(tracks["ekin"]), 250_000)
+    sample = {k: v[: n_sample * (len(v) // len(tracks["ekin"]))] if len(v) != len(tracks["ekin"]) else v[:n_sample]
+              for k, v in tracks.items()}
+    # planar fields must be re-sliced component by component
+    n_full = len(tracks["ekin"])
+    for k, v in tracks.items():
+        c = len(v) // n_full
+        if c > 1:
+            sample[k] = np.concatenate([v[i * n_full: i * n_full + n_sample] for i in range(c)])
+    t0 = time.perf_counter()
+    done = 0
+    while True:
+        o.step(tables, seed, sample, nthreads=cores)
+        done += n_sample
+        if time.perf_counter() - t0 >= min_seconds or done >= 40 * n_sample:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "track-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{done // n_sample} passes over the first {n_sample} tracks of the same batch "
+                      f"({done} track-steps, {dt:.1f} s, {cores} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=0x5EED2026)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline FP64 step only (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by track batch with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from g4hepem_amd import HEPEM_FP32T, HEPEM_FP64, Context, Tables
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_tracks
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    dev = torch.device("cuda:0")
+    tables2 = Tables(SET2, 0.7)
+    n = args.n
+    tracks = step_tracks(n, args.seed, len(SET2))
+    tin = TrackSoA.from_numpy(tracks, "torch", dev)
+    tout = TrackSoA(n, "torch", dev)
+    res = StepOutSoA(n, "torch", dev)
+    work = Context.work_buffer(n, dev)
+
+    ctx64 = Context(tables2, HEPEM_FP64)
+    ms64, run64, wall64 = time_steps(ctx64, tin, tout, res, work, args.steps, args.warmup, args.seed)
+    value = n / (ms64 * 1e-3)
+    abytes = algorithmic_bytes(tracks["ptype"])
+    achieved = abytes / (run64 * 1e-3) / 1e9
+    line = {
+        "metric": "track-steps/s on 1 MI355X, FP64 vs FP32-table form, + HBM GB/s vs peak",
+        "value": value,
+        "unit": "track-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Philox inputs; tables from the project's offline builder)",
+        "config": {
+            "workload": "config2: one HowFar+Perform step of 1M fresh tracks, e-:e+:gamma=4:1:5, "
+                        "E log-U[100 keV,10 GeV], couples U{G4_PbWO4,G4_Pb,G4_lAr,G4_Si} @0.7 mm, FP64",
+            "tracks": n, "wall_s": round(wall64, 4),
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_step<double>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel_ms": run64, "algorithmic_bytes_per_launch": abytes,
+        },
+    }
+    if not args.no_extras:
+        ctx32 = Context(tables2, HEPEM_FP32T)
+        ms32, run32, _ = time_steps(ctx32, tin, tout, res, work, args.steps, args.warmup, args.seed)
+        line["fp32_table_form"] = {"value": n / (ms32 * 1e-3), "unit": "track-steps/s", "ms_per_step": ms32,
+                                   "kernel_ms": run32, "speedup_vs_fp64": ms64 / ms32}
+        tables1 = Tables(SET1, 0.7)
+        ctx1 = Context(tables1, HEPEM_FP64)
+        q100k, ms100k = time_lookups(ctx1, 100_000, 200)
+        q10m, ms10m = time_lookups(ctx1, 10_000_000, 20)
+        line["config1_lookups"] = {"queries_per_s_100k": q100k, "ms_100k": ms100k,
+                                   "queries_per_s_10M": q10m, "ms_10M": ms10m,
+                                   "achieved_GBs_10M": 60.0 * 10_000_000 / (ms10m * 1e-3) / 1e9}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(tables2, tracks, args.cpu_seconds, args.seed)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+"""ctypes binding of the C-ABI in include/g4hepem_amd.h and
+include/g4hepem_amd_tables.h.
+
+The library is built in-tree (``python -m g4hepem_amd.build``); importing this
+module without it raises, so no code path silently falls back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libg4hepem_amd.so"
+
+HEPEM_FP64 = 0
+HEPEM_FP32T = 1
+ELECTRON, POSITRON, GAMMA = 0, 1, 2
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "capacity exceeded",
+          -4: "allocation failed", -5: "no usable GPU device", -6: "unsupported option"}
+
+
+class HepEmError(RuntimeError):
+    def __init__(self, where: str, status: int):
+        super().__init__(f"{where} failed: {STATUS.get(status, status)} ({status})")
+        self.status = status
+
+
+P = C.c_void_p
+
+
+class Tracks(C.Structure):
+    """hepem_tracks: per-track state (device or host pointers)."""
+    _fields_ = [("n", C.c_int32), ("ptype", P), ("id", P), ("imc", P), ("ekin", P), ("dir", P),
+                ("safety", P), ("on_boundary", P), ("num_ia_left", P), ("msc_init_range", P),
+                ("msc_dyn_range_factor", P), ("msc_tlimit_min", P), ("msc_first_step", P),
+                ("rng_ctr", P), ("rng_gauss", P), ("rng_has_gauss", P)]
+
+
+class StepOut(C.Structure):
+    """hepem_step_out: per-step results."""
+    _fields_ = [("true_step", P), ("geom_step", P), ("edep", P), ("displacement", P), ("winner", P),
+                ("stopped", P), ("nsec", P), ("sec_ptype", P), ("sec_ekin", P), ("sec_dir", P)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("inp", Tracks), ("out", Tracks), ("res", StepOut)]
+
+
+# field name -> (dtype name, components)
+TRACK_FIELDS = {
+    "ptype": ("int8", 1), "id": ("int32", 1), "imc": ("int32", 1), "ekin": ("float64", 1),
+    "dir": ("float64", 3), "safety": ("float64", 1), "on_boundary": ("uint8", 1),
+    "num_ia_left": ("float64", 4), "msc_init_range": ("float64", 1),
+    "msc_dyn_range_factor": ("float64", 1), "msc_tlimit_min": ("float64", 1),
+    "msc_first_step": ("uint8", 1), "rng_ctr": ("uint32", 1), "rng_gauss": ("float64", 1),
+    "rng_has_gauss": ("uint8", 1),
+}
+OUT_FIELDS = {
+    "true_step": ("float64", 1), "geom_step": ("float64", 1), "edep": ("float64", 1),
+    "displacement": ("float64", 3), "winner": ("int32", 1), "stopped": ("uint8", 1),
+    "nsec": ("int32", 1), "sec_ptype": ("int8", 2), "sec_ekin": ("float64", 2), "sec_dir": ("float64", 6),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m g4hepem_amd.build native`")
+    L = C.CDLL(str(LIB_PATH))
+    i32, u64, vp, dp, ip = C.c_int32, C.c_uint64, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    L.hepem_tables_build.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.c_double, C.POINTER(vp), C.POINTER(vp)]
+    L.hepem_tables_free.argtypes = [vp, vp]
+    L.hepem_tables_serialize.argtypes = [vp, vp, vp, C.POINTER(u64)]
+    L.hepem_ctx_create.argtypes = [vp, vp, C.c_int, C.POINTER(vp)]
+    L.hepem_ctx_destroy.argtypes = [vp]
+    L.hepem_ctx_table_bytes.argtypes = [vp, C.POINTER(u64)]
+    L.hepem_status_string.restype = C.c_char_p
+    L.hepem_lookup_eloss_xsec.argtypes = [vp, C.c_int, i32, vp, vp, vp, vp, vp]
+    L.hepem_step.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(Tracks), C.POINTER(StepOut), vp, vp]
+    L.hepem_step_work_bytes.argtypes = [i32]
+    L.hepem_step_work_bytes.restype = u64
+    L.hepem_step_prepare.argtypes = [vp, C.POINTER(Tracks), vp, vp]
+    L.hepem_step_run.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(Tracks), C.POINTER(StepOut), vp, vp]
+    L.hepem_step_rejected.argtypes = [vp, C.POINTER(C.c_uint32), vp]
+    L.hepem_step_batched.argtypes = [vp, u64, C.POINTER(BatchDesc), i32, vp]
+    L.hepem_tracks_reset_state.argtypes = [C.POINTER(Tracks), vp]
+    for name in ("hepem_tables_build", "hepem_tables_free", "hepem_tables_serialize", "hepem_ctx_create",
+                 "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_step",
+                 "hepem_step_prepare", "hepem_step_run", "hepem_step_rejected",
+                 "hepem_step_batched", "hepem_tracks_reset_state"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(where: str, rc: int) -> None:
+    if rc != 0:
+        raise HepEmError(where, rc)
+
+
+class Tables:
+    """G4HepEmData + G4HepEmParameters built by the offline table builder."""
+
+    def __init__(self, materials, prod_cut_mm: float = 0.7):
+        self.materials = list(materials)
+        self.prod_cut_mm = float(prod_cut_mm)
+        names = (C.c_char_p * len(self.materials))(*[m.encode() for m in self.materials])
+        d, p = C.c_void_p(), C.c_void_p()
+        check("hepem_tables_build", lib().hepem_tables_build(names, len(self.materials), self.prod_cut_mm,
+                                                              C.byref(d), C.byref(p)))
+        self.data, self.pars = d, p
+
+    def serialize(self) -> bytes:
+        size = C.c_uint64()
+        check("hepem_tables_serialize", lib().hepem_tables_serialize(self.data, self.pars, None, C.byref(size)))
+        buf = (C.c_uint8 * size.value)()
+        check("hepem_tables_serialize", lib().hepem_tables_serialize(self.data, self.pars, buf, C.byref(size)))
+        return bytes(buf)
+
+    def digest(self) -> str:
+        import hashlib
+        return hashlib.sha256(self.serialize()).hexdigest()
+
+    def close(self):
+        if self.data:
+            lib().hepem_tables_free(self.data, self.pars)
+            self.data = self.pars = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,83 @@
+"""Host-side handle of a device context: the Python mirror of the C-ABI.
+
+Device memory and streams come from torch (plumbing only); every computation
+runs in the HIP kernels of libg4hepem_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _native as N
+from .tracks import StepOutSoA, TrackSoA
+
+
+def _stream_handle(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+class Context:
+    """hepem_ctx: device image of the tables in FP64 or FP32-table form."""
+
+    def __init__(self, tables: N.Tables, precision: int = N.HEPEM_FP64):
+        self.tables = tables
+        self.precision = precision
+        h = C.c_void_p()
+        N.check("hepem_ctx_create", N.lib().hepem_ctx_create(tables.data, tables.pars, precision, C.byref(h)))
+        self.h = h
+
+    def table_bytes(self) -> int:
+        b = C.c_uint64()
+        N.check("hepem_ctx_table_bytes", N.lib().hepem_ctx_table_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def lookup(self, ptype: int, imc, ekin, lekin, out=None, stream=None):
+        """Config-1 lookups on device tensors; returns a [5, n] float64 tensor
+        (dE/dx, range, inverse range, ioni and brem macroscopic cross sections)."""
+        import torch
+        n = imc.numel()
+        if out is None:
+            out = torch.empty(5 * n, dtype=torch.float64, device=imc.device)
+        N.check("hepem_lookup_eloss_xsec",
+                N.lib().hepem_lookup_eloss_xsec(self.h, ptype, n, imc.data_ptr(), ekin.data_ptr(), lekin.data_ptr(),
+                                                out.data_ptr(), _stream_handle(stream)))
+        return out.view(5, n)
+
+    def step(self, seed: int, tin: TrackSoA, tout: TrackSoA, res: StepOutSoA, work=None, stream=None) -> None:
+        si, so, rr = tin.struct(), tout.struct(), res.struct()
+        wp = work.data_ptr() if work is not None else None
+        N.check("hepem_step", N.lib().hepem_step(self.h, C.c_uint64(seed), C.byref(si), C.byref(so), C.byref(rr), wp,
+                                                  _stream_handle(stream)))
+
+    def step_prepare(self, tin: TrackSoA, work, stream=None) -> None:
+        si = tin.struct()
+        N.check("hepem_step_prepare", N.lib().hepem_step_prepare(self.h, C.byref(si), work.data_ptr(),
+                                                                  _stream_handle(stream)))
+
+    def step_run(self, seed: int, tin: TrackSoA, tout: TrackSoA, res: StepOutSoA, work, stream=None) -> None:
+        si, so, rr = tin.struct(), tout.struct(), res.struct()
+        N.check("hepem_step_run", N.lib().hepem_step_run(self.h, C.c_uint64(seed), C.byref(si), C.byref(so),
+                                                          C.byref(rr), work.data_ptr(), _stream_handle(stream)))
+
+    @staticmethod
+    def rejected(work, stream=None) -> int:
+        v = C.c_uint32()
+        N.check("hepem_step_rejected", N.lib().hepem_step_rejected(work.data_ptr(), C.byref(v), _stream_handle(stream)))
+        return v.value
+
+    @staticmethod
+    def work_buffer(n: int, device="cuda"):
+        import torch
+        return torch.empty(int(N.lib().hepem_step_work_bytes(n)), dtype=torch.uint8, device=device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().hepem_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,416 @@
+// hepem_capi.hip -- the C-ABI of the library (declared in include/g4hepem_amd.h):
+// context creation (one device image of every table, FP64 and FP32 copies),
+// and the launch wrappers of the lookup and stepping kernels.
+#include "hepem_kernels.hh"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <utility>
+#include <vector>
+
+using namespace hepem;
+
+#define HIP_OK(call)                                    \
+  do {                                                  \
+    if ((call) != hipSuccess) return HEPEM_ERR_HIP;     \
+  } while (0)
+
+namespace {
+
+// Host staging of the device image: arrays appended at 256-byte alignment;
+// pointer fields of the DevTables structs are patched after the upload.
+struct Packer {
+  std::vector<uint8_t> buf;
+  std::vector<std::pair<const void**, size_t>> fix;
+  template <typename T> void add(const T*& field, const T* src, size_t n) {
+    size_t off = (buf.size() + 255) & ~size_t(255);
+    buf.resize(off + (n ? n : 1) * sizeof(T), 0);
+    if (n && src) std::memcpy(buf.data() + off, src, n * sizeof(T));
+    fix.emplace_back(reinterpret_cast<const void**>(&field), off);
+  }
+  template <typename T> void add(const T*& field, const std::vector<T>& v) { add(field, v.data(), v.size()); }
+  // table values converted to the form's precision R
+  template <typename R> void addR(const R*& field, const double* src, size_t n) {
+    std::vector<R> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = (R)src[i];
+    add(field, v);
+  }
+};
+
+template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronData* e) {
+  const int N = e->fELossEnergyGridSize;
+  E.N = N;
+  E.lossLogMin = (R)e->fELossLogMinEkin;
+  E.lossILD = (R)e->fELossEILDelta;
+  P.addR(E.lossGrid, e->fELossEnergyGrid, N);
+  P.addR(E.lossData, e->fELossData, (size_t)5 * N * e->fNumMatCuts);
+  P.add(E.xsStart, e->fResMacXSecStartIndexPerMatCut, e->fNumMatCuts);
+  P.addR(E.xsData, e->fResMacXSecData, e->fResMacXSecNumData);
+  E.nucN = e->fENucEnergyGridSize;
+  E.nucLogMin = (R)e->fENucLogMinEkin;
+  E.nucILD = (R)e->fENucEILDelta;
+  P.addR(E.nucGrid, e->fENucEnergyGrid, e->fENucEnergyGridSize);
+  P.addR(E.nucData, e->fENucMacXsecData, (size_t)2 * e->fENucEnergyGridSize * e->fNumMaterials);
+  P.addR(E.tr1, e->fTr1MacXSecData, (size_t)2 * N * e->fNumMaterials);
+  P.add(E.selSBStart, e->fElemSelectorBremSBStartIndexPerMatCut, e->fNumMatCuts);
+  P.addR(E.selSB, e->fElemSelectorBremSBData, e->fElemSelectorBremSBNumData);
+  P.add(E.selRBStart, e->fElemSelectorBremRBStartIndexPerMatCut, e->fNumMatCuts);
+  P.addR(E.selRB, e->fElemSelectorBremRBData, e->fElemSelectorBremRBNumData);
+}
+
+template <typename R> void PackTables(Packer& P, DevTables<R>& T, const G4HepEmData* d, const G4HepEmParameters* p) {
+  PackEl(P, T.el[0], d->fTheElectronData);
+  PackEl(P, T.el[1], d->fThePositronData);
+  const G4HepEmMatCutData* mc = d->fTheMatCutData;
+  const int nmc = mc->fNumMatCutData;
+  T.nmc = nmc;
+  std::vector<double> eCut(nmc), gCut(nmc), lgCut(nmc);
+  std::vector<int> mMat(nmc), mReg(nmc);
+  for (int i = 0; i < nmc; ++i) {
+    eCut[i] = mc->fMatCutData[i].fSecElProdCutE;
+    gCut[i] = mc->fMatCutData[i].fSecGamProdCutE;
+    lgCut[i] = mc->fMatCutData[i].fLogSecGamCutE;
+    mMat[i] = mc->fMatCutData[i].fHepEmMatIndex;
+    mReg[i] = mc->fMatCutData[i].fG4RegionIndex;
+  }
+  P.add(T.mcElCut, eCut);
+  P.add(T.mcGamCut, gCut);
+  P.add(T.mcLogGamCut, lgCut);
+  P.add(T.mcMat, mMat);
+  P.add(T.mcReg, mReg);
+  const G4HepEmMaterialData* md = d->fTheMaterialData;
+  const int nmat = md->fNumMaterialData;
+  T.nmat = nmat;
+  std::vector<int> nEl(nmat), elStart(nmat), elZ, sStart(nmat), sN(nmat);
+  std::vector<double> nat, densCor(nmat), elDens(nmat), radLen(nmat), meanExc(nmat), zeff(nmat), zeff23(nmat), zeffSqrt(nmat),
+      umscPar(nmat), stepMin(2 * nmat), tail(4 * nmat), theta(2 * nmat), sE, sC;
+  for (int i = 0; i < nmat; ++i) {
+    const G4HepEmMatData& m = md->fMaterialData[i];
+    nEl[i] = m.fNumOfElement;
+    elStart[i] = (int)elZ.size();
+    for (int k = 0; k < m.fNumOfElement; ++k) {
+      elZ.push_back(m.fElementVect[k]);
+      nat.push_back(m.fNumOfAtomsPerVolumeVect[k]);
+    }
+    densCor[i] = m.fDensityCorFactor;
+    elDens[i] = m.fElectronDensity;
+    radLen[i] = m.fRadiationLength;
+    meanExc[i] = m.fMeanExEnergy;
+    zeff[i] = m.fZeff;
+    zeff23[i] = m.fZeff23;
+    zeffSqrt[i] = m.fZeffSqrt;
+    umscPar[i] = m.fUMSCPar;
+    for (int k = 0; k < 2; ++k) stepMin[2 * i + k] = m.fUMSCStepMinPars[k];
+    for (int k = 0; k < 4; ++k) tail[4 * i + k] = m.fUMSCTailCoeff[k];
+    for (int k = 0; k < 2; ++k) theta[2 * i + k] = m.fUMSCThetaCoeff[k];
+    sStart[i] = (int)sE.size();
+    sN[i] = m.fNumOfSandiaIntervals;
+    for (int k = 0; k < m.fNumOfSandiaIntervals; ++k) sE.push_back(m.fSandiaEnergies[k]);
+    for (int k = 0; k < 4 * m.fNumOfSandiaIntervals; ++k) sC.push_back(m.fSandiaCoefficients[k]);
+  }
+  P.add(T.matNElem, nEl);
+  P.add(T.matElemStart, elStart);
+  P.add(T.matElemZ, elZ);
+  P.add(T.matElemNat, nat);
+  P.add(T.matDensCor, densCor);
+  P.add(T.matElDens, elDens);
+  P.add(T.matRadLen, radLen);
+  P.add(T.matMeanExc, meanExc);
+  P.add(T.matZeff, zeff);
+  P.add(T.matZeff23, zeff23);
+  P.add(T.matZeffSqrt, zeffSqrt);
+  P.add(T.matUMSCPar, umscPar);
+  P.add(T.matStepMin, stepMin);
+  P.add(T.matTail, tail);
+  P.add(T.matTheta, theta);
+  P.add(T.matSandiaStart, sStart);
+  P.add(T.matSandiaN, sN);
+  P.add(T.matSandiaE, sE);
+  P.add(T.matSandiaC, sC);
+  const G4HepEmElementData* ed = d->fTheElementData;
+  const int nz = ed->fMaxZet + 1;
+  T.maxZ = ed->fMaxZet;
+  std::vector<double> z13(nz), z23(nz), coul(nz), logZ(nz), zf1(nz), dLow(nz), dHigh(nz), ils1(nz), ils1c(nz), kb(nz), eE, eC;
+  std::vector<int> eS(nz), eN(nz);
+  for (int z = 0; z < nz; ++z) {
+    const G4HepEmElemData& e = ed->fElementData[z];
+    z13[z] = e.fZet13;
+    z23[z] = e.fZet23;
+    coul[z] = e.fCoulomb;
+    logZ[z] = e.fLogZ;
+    zf1[z] = e.fZFactor1;
+    dLow[z] = e.fDeltaMaxLow;
+    dHigh[z] = e.fDeltaMaxHigh;
+    ils1[z] = e.fILVarS1;
+    ils1c[z] = e.fILVarS1Cond;
+    kb[z] = e.fKShellBindingEnergy;
+    eS[z] = (int)eE.size();
+    const int ni = e.fZet >= 0.0 ? e.fNumOfSandiaIntervals : 0;
+    eN[z] = ni;
+    for (int k = 0; k < ni; ++k) eE.push_back(e.fSandiaEnergies[k]);
+    for (int k = 0; k < 4 * ni; ++k) eC.push_back(e.fSandiaCoefficients[k]);
+  }
+  P.add(T.elZ13, z13);
+  P.add(T.elZ23, z23);
+  P.add(T.elCoulomb, coul);
+  P.add(T.elLogZ, logZ);
+  P.add(T.elZF1, zf1);
+  P.add(T.elDMaxLow, dLow);
+  P.add(T.elDMaxHigh, dHigh);
+  P.add(T.elILS1, ils1);
+  P.add(T.elILS1Cond, ils1c);
+  P.add(T.elKBind, kb);
+  P.add(T.elSandiaStart, eS);
+  P.add(T.elSandiaN, eN);
+  P.add(T.elSandiaE, eE);
+  P.add(T.elSandiaC, eC);
+  const G4HepEmSBTableData* sb = d->fTheSBTableData;
+  P.addR(T.sbData, sb->fSBTableData, sb->fNumSBTableData);
+  P.add(T.sbStartZ, sb->fSBTablesStartPerZ, 121);
+  P.add(T.sbGCutStart, sb->fGammaCutIndxStartIndexPerMC, sb->fNumHepEmMatCuts);
+  P.add(T.sbGCutIdx, sb->fGammaCutIndices, sb->fNumElemsInMatCuts);
+  P.add(T.sbElE, sb->fElEnergyVect, 65);
+  P.add(T.sbLElE, sb->fLElEnergyVect, 65);
+  P.addR(T.sbLKappa, sb->fLKappaVect, 54);
+  T.sbLogMinE = sb->fLogMinElEnergy;
+  T.sbILDE = sb->fILDeltaElEnergy;
+  T.sbNumKappa = sb->fNumKappa;
+  const G4HepEmGammaData* g = d->fTheGammaData;
+  P.addR(T.gmData, g->fMacXsecData, (size_t)g->fNumMaterials * g->fDataPerMat);
+  T.gmPerMat = g->fDataPerMat;
+  T.gmOff1 = g->fNumData0;
+  T.gmOff2 = g->fNumData0 + g->fNumData1;
+  T.gmN0 = g->fEGridSize0;
+  T.gmN1 = g->fEGridSize1;
+  T.gmN2 = g->fEGridSize2;
+  T.gEMax0 = g->fEMax0;
+  T.gEMax1 = g->fEMax1;
+  T.gLogEMin0 = (R)g->fLogEMin0;
+  T.gILD0 = (R)g->fEILDelta0;
+  T.gLogEMin1 = (R)g->fLogEMin1;
+  T.gILD1 = (R)g->fEILDelta1;
+  T.gLogEMin2 = (R)g->fLogEMin2;
+  T.gILD2 = (R)g->fEILDelta2;
+  P.add(T.convSelStart, g->fElemSelectorConvStartIndexPerMat, g->fNumMaterials);
+  P.addR(T.convSelData, g->fElemSelectorConvData, g->fElemSelectorConvNumData);
+  P.addR(T.convSelGrid, g->fElemSelectorConvEgrid, g->fElemSelectorConvEgridSize);
+  T.convSelN = g->fElemSelectorConvEgridSize;
+  T.convSelLogMin = (R)g->fElemSelectorConvLogMinEkin;
+  T.convSelILD = (R)g->fElemSelectorConvEILDelta;
+  T.elTrackCut = p->fElectronTrackingCut;
+  T.gamTrackCut = p->fGammaTrackingCut;
+  T.minLossE = p->fMinLossTableEnergy;
+  T.bremModelLim = p->fElectronBremModelLim;
+  T.posCor = p->fIsMSCPositronCor ? 1 : 0;
+  T.displace = p->fIsMSCDisplacement ? 1 : 0;
+  const int nreg = p->fNumRegions;
+  std::vector<double> fr(nreg), dr(nreg), ll(nreg), rf(nreg), sf(nreg);
+  std::vector<int> mn(nreg), fl(nreg);
+  for (int r = 0; r < nreg; ++r) {
+    const G4HepEmRegionParmeters& rp = p->fParametersPerRegion[r];
+    fr[r] = rp.fFinalRange;
+    dr[r] = rp.fDRoverRange;
+    ll[r] = rp.fLinELossLimit;
+    rf[r] = rp.fMSCRangeFactor;
+    sf[r] = rp.fMSCSafetyFactor;
+    mn[r] = rp.fIsMSCMinimalStepLimit ? 1 : 0;
+    fl[r] = rp.fIsELossFluctuation ? 1 : 0;
+  }
+  P.add(T.regFinalRange, fr);
+  P.add(T.regDRover, dr);
+  P.add(T.regLinLim, ll);
+  P.add(T.regRangeFac, rf);
+  P.add(T.regSafetyFac, sf);
+  P.add(T.regMinimal, mn);
+  P.add(T.regFluct, fl);
+}
+
+bool DataComplete(const G4HepEmData* d, const G4HepEmParameters* p) {
+  if (!d || !p || !d->fTheMatCutData || !d->fTheMaterialData || !d->fTheElementData || !d->fTheElectronData ||
+      !d->fThePositronData || !d->fTheSBTableData || !d->fTheGammaData || p->fNumRegions < 1 || !p->fParametersPerRegion)
+    return false;
+  const int nmc = d->fTheMatCutData->fNumMatCutData;
+  if (nmc < 1 || !d->fTheMatCutData->fMatCutData) return false;
+  for (int i = 0; i < nmc; ++i) {
+    const G4HepEmMCCData& c = d->fTheMatCutData->fMatCutData[i];
+    if (c.fHepEmMatIndex < 0 || c.fHepEmMatIndex >= d->fTheMaterialData->fNumMaterialData) return false;
+    if (c.fG4RegionIndex < 0 || c.fG4RegionIndex >= p->fNumRegions) return false;
+  }
+  return d->fTheElementData->fMaxZet >= 1 && d->fTheElementData->fMaxZet <= 120;
+}
+
+}  // namespace
+
+struct hepem_ctx {
+  int device = 0;
+  int precision = HEPEM_FP64;
+  void* image = nullptr;
+  uint64_t imageBytes = 0;
+  DevTables<double> t64;
+  DevTables<float> t32;
+  void* work = nullptr;       // internal work buffer for callers passing work == NULL
+  uint64_t workBytes = 0;
+};
+
+extern "C" const char* hepem_status_string(int s) {
+  switch (s) {
+    case HEPEM_OK: return "ok";
+    case HEPEM_ERR_INVALID: return "invalid argument";
+    case HEPEM_ERR_HIP: return "HIP runtime error";
+    case HEPEM_ERR_CAPACITY: return "capacity exceeded";
+    case HEPEM_ERR_ALLOC: return "allocation failed";
+    case HEPEM_ERR_NO_DEVICE: return "no usable GPU device";
+    case HEPEM_ERR_UNSUPPORTED: return "unsupported option";
+  }
+  return "unknown status";
+}
+
+extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p, int precision, hepem_ctx** out) {
+  if (!out) return HEPEM_ERR_INVALID;
+  *out = nullptr;
+  if (!DataComplete(d, p) || (precision != HEPEM_FP64 && precision != HEPEM_FP32T)) return HEPEM_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return HEPEM_ERR_NO_DEVICE;
+  hepem_ctx* c = new (std::nothrow) hepem_ctx();
+  if (!c) return HEPEM_ERR_ALLOC;
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HEPEM_ERR_HIP; }
+  c->precision = precision;
+  Packer P;
+  PackTables(P, c->t64, d, p);
+  PackTables(P, c->t32, d, p);
+  c->imageBytes = P.buf.size();
+  if (hipMalloc(&c->image, c->imageBytes) != hipSuccess) { delete c; return HEPEM_ERR_ALLOC; }
+  if (hipMemcpy(c->image, P.buf.data(), c->imageBytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(c->image);
+    delete c;
+    return HEPEM_ERR_HIP;
+  }
+  for (auto& f : P.fix) *f.first = static_cast<const uint8_t*>(c->image) + f.second;
+  *out = c;
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_ctx_destroy(hepem_ctx* c) {
+  if (!c) return HEPEM_ERR_INVALID;
+  int rc = HEPEM_OK;
+  if (c->image && hipFree(c->image) != hipSuccess) rc = HEPEM_ERR_HIP;
+  if (c->work && hipFree(c->work) != hipSuccess) rc = HEPEM_ERR_HIP;
+  delete c;
+  return rc;
+}
+
+extern "C" int hepem_ctx_table_bytes(const hepem_ctx* c, uint64_t* bytes) {
+  if (!c || !bytes) return HEPEM_ERR_INVALID;
+  *bytes = c->imageBytes;
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_lookup_eloss_xsec(const hepem_ctx* c, int ptype, int32_t n, const int32_t* imc, const double* ekin,
+                                       const double* lekin, double* out, void* stream) {
+  if (!c || (ptype != HEPEM_ELECTRON && ptype != HEPEM_POSITRON) || n < 0) return HEPEM_ERR_INVALID;
+  if (n == 0) return HEPEM_OK;
+  if (!imc || !ekin || !lekin || !out) return HEPEM_ERR_INVALID;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int grid = (n + kBlock - 1) / kBlock;
+  if (c->precision == HEPEM_FP64)
+    hipLaunchKernelGGL(k_lookup<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, ptype, n, imc, ekin, lekin, out);
+  else
+    hipLaunchKernelGGL(k_lookup<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, ptype, n, imc, ekin, lekin, out);
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+
+namespace {
+bool TracksOk(const hepem_tracks* t) {
+  return t && t->n >= 0 && t->ptype && t->id && t->imc && t->ekin && t->dir && t->safety && t->on_boundary && t->num_ia_left &&
+         t->msc_init_range && t->msc_dyn_range_factor && t->msc_tlimit_min && t->msc_first_step && t->rng_ctr && t->rng_gauss &&
+         t->rng_has_gauss;
+}
+bool ResOk(const hepem_step_out* r) {
+  return r && r->true_step && r->geom_step && r->edep && r->displacement && r->winner && r->stopped && r->nsec && r->sec_ptype &&
+         r->sec_ekin && r->sec_dir;
+}
+}  // namespace
+
+extern "C" uint64_t hepem_step_work_bytes(int32_t n) { return 256 + 4ull * (uint64_t)(n > 0 ? n : 0); }
+
+extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
+  if (!c || !TracksOk(in) || !work) return HEPEM_ERR_INVALID;
+  const int n = in->n;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t* counters = static_cast<uint32_t*>(work);
+  uint32_t* queue = counters + 64;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters, 64, 0u);
+  if (n > 0) {
+    const int grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kBlock), 0, s, n, in->ptype, queue, counters);
+  }
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
+                              hepem_step_out* res, void* work, void* stream) {
+  if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res) || out->n != in->n || !work) return HEPEM_ERR_INVALID;
+  const int n = in->n;
+  if (n == 0) return HEPEM_OK;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t* counters = static_cast<uint32_t*>(work);
+  uint32_t* queue = counters + 64;
+  const int grid = (n + kBlock - 1) / kBlock;
+  if (c->precision == HEPEM_FP64)
+    hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, queue, counters);
+  else
+    hipLaunchKernelGGL(k_step<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, queue, counters);
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_step(const hepem_ctx* cc, uint64_t seed, const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res,
+                          void* work, void* stream) {
+  hepem_ctx* c = const_cast<hepem_ctx*>(cc);
+  if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res) || out->n != in->n) return HEPEM_ERR_INVALID;
+  const int n = in->n;
+  if (n == 0) return HEPEM_OK;
+  if (!work) {
+    const uint64_t need = hepem_step_work_bytes(n);
+    if (c->workBytes < need) {
+      if (c->work) HIP_OK(hipFree(c->work));
+      c->work = nullptr;
+      c->workBytes = 0;
+      if (hipMalloc(&c->work, need) != hipSuccess) return HEPEM_ERR_ALLOC;
+      c->workBytes = need;
+    }
+    work = c->work;
+  }
+  const int rc = hepem_step_prepare(c, in, work, stream);
+  if (rc != HEPEM_OK) return rc;
+  return hepem_step_run(c, seed, in, out, res, work, stream);
+}
+
+extern "C" int hepem_step_rejected(const void* work, uint32_t* nrej, void* stream) {
+  if (!work || !nrej) return HEPEM_ERR_INVALID;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_OK(hipMemcpyAsync(nrej, static_cast<const uint32_t*>(work) + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_step_batched(const hepem_ctx* c, uint64_t seed, const hepem_batch_desc* descs, int32_t nbatch, void* stream) {
+  if (!c || nbatch < 0 || (nbatch > 0 && !descs)) return HEPEM_ERR_INVALID;
+  (void)seed;
+  (void)stream;
+  return HEPEM_ERR_UNSUPPORTED;
+}
+
+extern "C" int hepem_tracks_reset_state(const hepem_tracks* t, void* stream) {
+  if (!TracksOk(t)) return HEPEM_ERR_INVALID;
+  if (t->n == 0) return HEPEM_OK;
+  const int grid = (t->n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_reset_state, dim3(grid), dim3(kBlock), 0, static_cast<hipStream_t>(stream), *t);
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}

@@ -1,0 +1,308 @@
+// hepem_kernels.hh -- kernels of the stepping library (one translation unit,
+// included by hepem_capi.hip).
+//
+// Stepping is split in two launches:
+//   k_classify  wave-level compaction of track slots by particle type into one
+//               queue: e-/e+ slots fill it from the front, gamma slots from the
+//               back (ballot + popcount per wave, one atomic per wave and kind);
+//   k_step      one thread per queue entry; the e-/e+ and gamma paths are
+//               therefore taken by whole waves (only the one wave that straddles
+//               the boundary diverges).
+// Every track writes only its own slot, so results do not depend on the queue
+// order and are bit-identical to an unsorted pass.
+#pragma once
+
+#include "device/dev_physics.hh"
+#include "g4hepem_amd.h"
+
+namespace hepem {
+
+constexpr int kBlock = 256;
+
+// counters[0]: e-/e+ tracks, counters[1]: gammas, counters[2]: rejected tracks
+__global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_classify(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
+                                                     uint32_t* __restrict__ counters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  const int8_t pt = valid ? ptype[i] : (int8_t)-1;
+  const bool isE = valid && (pt == 0 || pt == 1);
+  const bool isG = valid && !isE;
+  const uint64_t mE = __ballot(isE);
+  const uint64_t mG = __ballot(isG);
+  const int lane = __lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t baseE = 0, baseG = 0;
+  if (lane == 0) {
+    if (mE) baseE = atomicAdd(&counters[0], (uint32_t)__popcll(mE));
+    if (mG) baseG = atomicAdd(&counters[1], (uint32_t)__popcll(mG));
+  }
+  baseE = __shfl(baseE, 0);
+  baseG = __shfl(baseG, 0);
+  if (isE) queue[baseE + __popcll(mE & below)] = (uint32_t)i;
+  if (isG) queue[(uint32_t)n - 1u - (baseG + __popcll(mG & below))] = (uint32_t)i;
+}
+
+template <typename R>
+HD void WriteSecondaries(const hepem_step_out& res, int n, int i, const SecList<R>& sl) {
+  int ns = 0;
+  const Sec<R>* list[4];
+  for (int k = 0; k < sl.nel; ++k) list[ns++] = &sl.el[k];
+  for (int k = 0; k < sl.ngm; ++k) list[ns++] = &sl.gm[k];
+  res.nsec[i] = ns;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    if (s < ns) {
+      res.sec_ptype[s * n + i] = (int8_t)list[s]->ptype;
+      res.sec_ekin[s * n + i] = list[s]->ekin;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = (double)list[s]->dir[c];
+    } else {
+      res.sec_ptype[s * n + i] = -1;
+      res.sec_ekin[s * n + i] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = 0.0;
+    }
+  }
+}
+
+// A track is stepped only with a valid couple index and a finite, positive
+// kinetic energy; otherwise it is passed through unchanged, flagged with
+// winner -99 and counted in counters[2].
+template <typename R>
+HD bool RejectTrack(const DevTables<R>& T, const hepem_tracks& in, hepem_tracks& out, const hepem_step_out& res, int n, int i,
+                    uint32_t* counters) {
+  const int imc = in.imc[i];
+  const double e = in.ekin[i];
+  if (imc >= 0 && imc < T.nmc && e > 0.0 && e < 1.0e300) return false;
+  atomicAdd(&counters[2], 1u);
+  res.true_step[i] = 0.0;
+  res.geom_step[i] = 0.0;
+  res.edep[i] = 0.0;
+  for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = 0.0;
+  res.winner[i] = -99;
+  res.stopped[i] = 1;
+  SecList<R> sl;
+  sl.nel = sl.ngm = 0;
+  WriteSecondaries(res, n, i, sl);
+  if (out.ekin != in.ekin) {
+    out.ptype[i] = in.ptype[i];
+    out.id[i] = in.id[i];
+    out.imc[i] = imc;
+    out.ekin[i] = e;
+    for (int c = 0; c < 3; ++c) out.dir[c * n + i] = in.dir[c * n + i];
+    out.safety[i] = in.safety[i];
+    out.on_boundary[i] = in.on_boundary[i];
+    for (int k = 0; k < 4; ++k) out.num_ia_left[k * n + i] = in.num_ia_left[k * n + i];
+    out.rng_ctr[i] = in.rng_ctr[i];
+    out.rng_gauss[i] = in.rng_gauss[i];
+    out.rng_has_gauss[i] = in.rng_has_gauss[i];
+    if (in.ptype[i] != 2) {
+      out.msc_init_range[i] = in.msc_init_range[i];
+      out.msc_dyn_range_factor[i] = in.msc_dyn_range_factor[i];
+      out.msc_tlimit_min[i] = in.msc_tlimit_min[i];
+      out.msc_first_step[i] = in.msc_first_step[i];
+    }
+  }
+  return true;
+}
+
+template <typename R>
+HD void StepElectron(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out, const hepem_step_out& res,
+                     int n, int i) {
+  ElState<R> el;
+  Trk<R>& t = el.t;
+  t.isEl = in.ptype[i] == 0;
+  t.setE(in.ekin[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c] = (R)in.dir[c * n + i];
+  t.imc = in.imc[i];
+  t.safety = (R)in.safety[i];
+  t.onBoundary = in.on_boundary[i] != 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    t.nial[k] = (R)in.num_ia_left[k * n + i];
+    t.mfp[k] = (R)-1.0;
+  }
+  t.winner = -1;
+  t.edep = 0.0;
+  t.gstep = (R)0.0;
+  Msc<R>& m = el.msc;
+  m.lambtr1 = m.trueStep = m.zpath = (R)0.;
+  m.displ[0] = m.displ[1] = m.displ[2] = (R)0.;
+  m.dir[0] = m.dir[1] = (R)0.;
+  m.dir[2] = (R)1.;
+  m.initRange = (R)in.msc_init_range[i];
+  m.dynRangeFactor = (R)in.msc_dyn_range_factor[i];
+  m.tlimitMin = (R)in.msc_tlimit_min[i];
+  m.par1 = (R)-1.;
+  m.par2 = m.par3 = (R)0.;
+  m.noScat = m.isDisplace = m.active = false;
+  m.firstStep = in.msc_first_step[i] != 0;
+  el.range = el.pstep = (R)0.;
+  Rng rng;
+  rng.init(seed, (uint32_t)in.id[i], in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
+  SecList<R> sl;
+  sl.nel = sl.ngm = 0;
+
+  ElectronHowFar(T, el, rng);
+  ElectronPerform(T, el, sl, rng);
+
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out.num_ia_left[k * n + i] = (double)t.nial[k];
+  out.msc_init_range[i] = (double)m.initRange;
+  out.msc_dyn_range_factor[i] = (double)m.dynRangeFactor;
+  out.msc_tlimit_min[i] = (double)m.tlimitMin;
+  out.msc_first_step[i] = m.firstStep ? 1 : 0;
+  out.rng_ctr[i] = rng.ctr;
+  out.rng_gauss[i] = rng.gauss;
+  out.rng_has_gauss[i] = rng.has_gauss ? 1 : 0;
+  res.true_step[i] = (double)el.pstep;
+  res.geom_step[i] = (double)t.gstep;
+  res.edep[i] = t.edep;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = (double)m.displ[c];
+  res.winner[i] = t.winner;
+  res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+  WriteSecondaries(res, n, i, sl);
+}
+
+template <typename R>
+HD void StepGamma(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out, const hepem_step_out& res,
+                  int n, int i) {
+  Trk<R> t;
+  t.isEl = false;
+  t.setE(in.ekin[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c] = (R)in.dir[c * n + i];
+  t.imc = in.imc[i];
+  t.safety = (R)in.safety[i];
+  t.onBoundary = in.on_boundary[i] != 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    t.nial[k] = (R)in.num_ia_left[k * n + i];
+    t.mfp[k] = (R)-1.0;
+  }
+  t.winner = -1;
+  t.edep = 0.0;
+  Rng rng;
+  rng.init(seed, (uint32_t)in.id[i], in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
+  SecList<R> sl;
+  sl.nel = sl.ngm = 0;
+  R pe = (R)0.0;
+  // G4HepEmGammaManager::HowFar (G4HepEmGammaManager.icc:27-48)
+  if (t.nial[0] <= (R)0.0) t.nial[0] = -vlog((R)rng.flat());
+  const R tot = GetTotalMacXSec(T, t, pe);
+  const R mfp = (tot > (R)0.) ? (R)1. / tot : (R)kALargeValue;
+  t.mfp[0] = mfp;
+  t.gstep = mfp * t.nial[0];
+  if (t.onBoundary) {
+    t.nial[0] -= t.gstep / t.mfp[0];
+  } else {
+    // SelectInteraction + Perform (G4HepEmGammaManager.icc:54-94,173-219)
+    SampleInteraction(T, t, pe, (R)rng.flat());
+    if (t.winner != 3) {
+      t.nial[0] -= t.gstep / t.mfp[0];
+      t.nial[t.winner] = (R)-1.0;
+      switch (t.winner) {
+        case 0: ConvPerform(T, t, sl, rng); break;
+        case 1: ComptonPerform(t, sl, rng); break;
+        case 2: PEPerform(T, t, pe, sl, rng); break;
+        default: break;
+      }
+      const double finalEkin = t.ekin;
+      if (finalEkin > 0.0 && finalEkin <= T.gamTrackCut) {
+        t.setE(0.0);
+        t.edep += finalEkin;
+      }
+    }
+  }
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out.num_ia_left[k * n + i] = (double)t.nial[k];
+  out.rng_ctr[i] = rng.ctr;
+  out.rng_gauss[i] = rng.gauss;
+  out.rng_has_gauss[i] = rng.has_gauss ? 1 : 0;
+  res.true_step[i] = (double)t.gstep;
+  res.geom_step[i] = (double)t.gstep;
+  res.edep[i] = t.edep;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = 0.0;
+  res.winner[i] = t.winner;
+  res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+  WriteSecondaries(res, n, i, sl);
+}
+
+template <typename R>
+HD void StepSlot(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out, const hepem_step_out& res,
+                 int n, int i, bool isE, uint32_t* counters) {
+  if (RejectTrack(T, in, out, res, n, i, counters)) return;
+  if (out.ekin != in.ekin) {
+    out.ptype[i] = in.ptype[i];
+    out.id[i] = in.id[i];
+    out.imc[i] = in.imc[i];
+    out.safety[i] = in.safety[i];
+    out.on_boundary[i] = in.on_boundary[i];
+  }
+  if (isE) StepElectron(T, seed, in, out, res, n, i);
+  else StepGamma(T, seed, in, out, res, n, i);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
+                                                 hepem_step_out res, const uint32_t* __restrict__ queue,
+                                                 uint32_t* __restrict__ counters) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = in.n;
+  if (j >= n) return;
+  const uint32_t nE = counters[0];
+  const int i = (int)queue[j];
+  StepSlot(T, seed, in, out, res, n, i, (uint32_t)j < nE, counters);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, int n, const int32_t* __restrict__ imc,
+                                                   const double* __restrict__ ekin, const double* __restrict__ lekin,
+                                                   double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevEl<R>& e = T.el[ptype];
+  const int mc = imc[i];
+  if (mc < 0 || mc >= T.nmc) {
+    for (int k = 0; k < 5; ++k) out[k * n + i] = __longlong_as_double(0x7FF8000000000000LL);
+    return;
+  }
+  const R E = (R)ekin[i], lE = (R)lekin[i];
+  const R r = GetRestRange(e, mc, E, lE);
+  out[i] = (double)GetRestDEDX(e, mc, E, lE);
+  out[n + i] = (double)r;
+  out[2 * n + i] = GetInvRange(e, mc, (double)r);
+  out[3 * n + i] = (double)GetRestMacXSec(e, mc, E, lE, true);
+  out[4 * n + i] = (double)GetRestMacXSec(e, mc, E, lE, false);
+}
+
+__global__ void k_reset_state(hepem_tracks t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = t.n;
+  if (i >= n) return;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) t.num_ia_left[k * n + i] = -1.0;
+  t.msc_init_range[i] = 1.0e+21;
+  t.msc_dyn_range_factor[i] = 0.04;
+  t.msc_tlimit_min[i] = 1.0E-7;
+  t.msc_first_step[i] = 1;
+  t.rng_ctr[i] = 0u;
+  t.rng_gauss[i] = 0.0;
+  t.rng_has_gauss[i] = 0;
+}
+
+}  // namespace hepem

@@ -1,0 +1,171 @@
+/*
+ * g4hepem_amd.h -- C-ABI of the MI355X (gfx950) run-time stepping-physics
+ * library. Plain pointers and sizes only; every entry point returns an int
+ * status (0 = ok, negative = hepem_status) and never exits the process.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *
+ *   hepem_ctx_create / hepem_ctx_destroy
+ *       replace CopyG4HepEmDataToGPU / FreeG4HepEmDataOnGPU
+ *       (G4HepEm/G4HepEmData/src/G4HepEmData.cc:78-102) and
+ *       CopyG4HepEmParametersToGPU (G4HepEm/G4HepEmData/src/G4HepEmParameters.cc:28-36).
+ *       The context deep-copies the tables into one device image (FP64 and
+ *       optional FP32 copies); it is immutable afterwards.
+ *
+ *   hepem_lookup_eloss_xsec
+ *       the batched form of G4HepEmElectronManager::GetRestDEDX / GetRestRange /
+ *       GetInvRange / GetRestMacXSec
+ *       (G4HepEm/G4HepEmRun/include/G4HepEmElectronManager.hh:90-150,
+ *        .icc:486-532), as driven by the reference test kernels
+ *       TestElossDataKernel (testing/ElectronEnergyLoss/src/ELossData.cu:18-26)
+ *       and TestResMacXSecDataKernel (testing/ElectronXSections/src/ResMacXSecs.cu:16-23).
+ *
+ *   hepem_step
+ *       one HowFar + Perform step of every track of a batch:
+ *       G4HepEmElectronManager::HowFar(data, pars, tlData) + Perform(data, pars, tlData)
+ *       (G4HepEmElectronManager.hh:71,222; .icc:35-45,461-483) for e-/e+ and
+ *       G4HepEmGammaManager::HowFar + SelectInteraction + Perform
+ *       (G4HepEmGammaManager.hh:32-55; .icc:27-35,54-94,173-178) for gammas,
+ *       with no geometry between HowFar and Perform (geometric step = physics
+ *       step, not on a boundary).
+ *
+ *   hepem_step_batched
+ *       many independent track batches of uneven size in one launch (no
+ *       reference counterpart: the reference steps one track per thread,
+ *       G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:346,895).
+ *
+ * Random numbers: every track owns a counter-based Philox4x32-10 stream keyed
+ * by (seed, track id); its draw counter and the cached second Gaussian
+ * deviate of G4HepEmRandomEngine::Gauss (G4HepEmRandomEngine.hh:50-70) are
+ * part of the track state, so a batch can be re-run or resumed bit-exactly.
+ *
+ * All track arrays are device pointers (HBM), structure-of-arrays, indexed by
+ * track slot i in [0, n). Multi-component fields are stored as planes:
+ * component c of track i is at [c*n + i].
+ */
+#ifndef G4HEPEM_AMD_H
+#define G4HEPEM_AMD_H
+
+#include <stdint.h>
+#include "g4hepem_amd_data.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum hepem_status {
+  HEPEM_OK              =  0,
+  HEPEM_ERR_INVALID     = -1,  /* null/invalid argument or context */
+  HEPEM_ERR_HIP         = -2,  /* a HIP runtime call failed          */
+  HEPEM_ERR_CAPACITY    = -3,  /* a size exceeds a fixed capacity     */
+  HEPEM_ERR_ALLOC       = -4,  /* device or host allocation failed    */
+  HEPEM_ERR_NO_DEVICE   = -5,  /* no usable gfx950 device             */
+  HEPEM_ERR_UNSUPPORTED = -6   /* option not available in this build  */
+};
+
+enum hepem_precision {
+  HEPEM_FP64  = 0,   /* tables and arithmetic in double (bit-exact form)   */
+  HEPEM_FP32T = 1    /* FP32 tables/arithmetic with FP64 islands            */
+};
+
+enum hepem_particle { HEPEM_ELECTRON = 0, HEPEM_POSITRON = 1, HEPEM_GAMMA = 2 };
+
+typedef struct hepem_ctx hepem_ctx;
+
+/* Per-track state, read and written by the stepping entry points. */
+typedef struct hepem_tracks {
+  int32_t   n;                 /* number of tracks                              */
+  int8_t*   ptype;             /* hepem_particle                                */
+  int32_t*  id;                /* track id: selects the Philox stream           */
+  int32_t*  imc;               /* material-cuts couple index                    */
+  double*   ekin;              /* kinetic energy [MeV]                          */
+  double*   dir;               /* [3][n] unit direction                         */
+  double*   safety;            /* isotropic safety [mm]                         */
+  uint8_t*  on_boundary;       /* pre-step point on a volume boundary           */
+  double*   num_ia_left;       /* [4][n] number of interaction lengths left     */
+  double*   msc_init_range;    /* Urban MSC persistent state (e-/e+ only)       */
+  double*   msc_dyn_range_factor;
+  double*   msc_tlimit_min;
+  uint8_t*  msc_first_step;
+  uint32_t* rng_ctr;           /* Philox draw counter                           */
+  double*   rng_gauss;         /* cached second Gaussian deviate                */
+  uint8_t*  rng_has_gauss;
+} hepem_tracks;
+
+/* Per-step results (write only). Up to 2 secondaries per track and step. */
+typedef struct hepem_step_out {
+  double*   true_step;         /* physical (true) step length [mm]               */
+  double*   geom_step;         /* geometrical step length [mm]                   */
+  double*   edep;              /* energy deposit in the step [MeV]               */
+  double*   displacement;      /* [3][n] MSC lateral displacement (lab frame)     */
+  int32_t*  winner;            /* winner process index (reference convention)    */
+  uint8_t*  stopped;           /* 1 if the track stopped/was killed in this step  */
+  int32_t*  nsec;              /* number of secondaries created (0..2)           */
+  int8_t*   sec_ptype;         /* [2][n] hepem_particle of secondary s            */
+  double*   sec_ekin;          /* [2][n]                                          */
+  double*   sec_dir;           /* [3][2][n]: component c of slot s at (c*2+s)*n+i */
+} hepem_step_out;
+
+/* One batch of the batched launch. */
+typedef struct hepem_batch_desc {
+  hepem_tracks   in;
+  hepem_tracks   out;          /* may alias `in` (in-place update)               */
+  hepem_step_out res;
+} hepem_batch_desc;
+
+/* ---- context ------------------------------------------------------------ */
+int  hepem_ctx_create(const struct G4HepEmData* data, const struct G4HepEmParameters* pars,
+                      int precision, hepem_ctx** out_ctx);
+int  hepem_ctx_destroy(hepem_ctx* ctx);
+const char* hepem_status_string(int status);
+/* Device bytes held by the context's table image. */
+int  hepem_ctx_table_bytes(const hepem_ctx* ctx, uint64_t* bytes);
+
+/* ---- config-1 style lookups ---------------------------------------------
+ * For n queries (imc, E, lnE) of particle `ptype` (e- or e+) writes
+ * out[0*n+i] restricted dE/dx, out[1*n+i] range, out[2*n+i] inverse range of
+ * that range, out[3*n+i] ioni and out[4*n+i] brem restricted macroscopic
+ * cross sections. `out` holds 5*n doubles. All pointers are device pointers. */
+int  hepem_lookup_eloss_xsec(const hepem_ctx* ctx, int ptype, int32_t n,
+                             const int32_t* imc, const double* ekin, const double* lekin,
+                             double* out, void* hip_stream);
+
+/* ---- stepping ------------------------------------------------------------
+ * One step of every track of the batch. `out` may alias `in`. `work` must be
+ * at least hepem_step_work_bytes(n) bytes of device memory (queues and
+ * counters); pass NULL to let the context use its own growable buffer
+ * (then calls on one context must not overlap). */
+int  hepem_step(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
+                hepem_tracks* out, hepem_step_out* res, void* work, void* hip_stream);
+uint64_t hepem_step_work_bytes(int32_t n);
+
+/* The two phases of hepem_step, for callers that time or reuse them:
+ * hepem_step_prepare compacts the track slots of `in` into the per-particle
+ * queue held in `work` (wave-level sort by particle type); hepem_step_run
+ * performs the physics step of every queued slot. `work` is required here.
+ * hepem_step == prepare + run on the same stream. */
+int  hepem_step_prepare(const hepem_ctx* ctx, const hepem_tracks* in, void* work, void* hip_stream);
+int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
+                    hepem_step_out* res, void* work, void* hip_stream);
+/* Number of tracks the last step on `work` rejected (invalid couple index or
+ * non-positive/non-finite energy; such tracks get winner -99 and are left
+ * unchanged). Synchronises `hip_stream`. */
+int  hepem_step_rejected(const void* work, uint32_t* nrejected, void* hip_stream);
+
+/* Many independent batches in one persistent launch. `descs` is a host array.
+ * Results are bit-identical to calling hepem_step on each batch. */
+int  hepem_step_batched(const hepem_ctx* ctx, uint64_t seed, const hepem_batch_desc* descs,
+                        int32_t nbatch, void* hip_stream);
+
+/* ---- track-state helpers -------------------------------------------------
+ * Reset the persistent physics state of n tracks to the reference's start-of-
+ * track values (G4HepEmTrack::ReSet, G4HepEmMSCTrackData::ReSet,
+ * G4HepEmRandomEngine::DiscardGauss): num_ia_left = -1, MSC first-step
+ * defaults, rng_ctr = 0, no cached Gaussian. */
+int  hepem_tracks_reset_state(const hepem_tracks* t, void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G4HEPEM_AMD_H */

@@ -1,0 +1,46 @@
+/*
+ * g4hepem_amd_tables.h -- offline, deterministic builder of G4HepEmData /
+ * G4HepEmParameters in the reference layout.
+ *
+ * The reference fills these tables from Geant4 models in G4HepEmInit
+ * (InitG4HepEmState, G4HepEm/G4HepEmInit/src/G4HepEmStateInit.cc:12-28), which
+ * needs Geant4 and G4LEDATA. This builder produces the same array shapes,
+ * grids and index conventions from analytic models (Berger-Seltzer restricted
+ * stopping power, Moller/Bhabha, Tsai complete-screening bremsstrahlung with
+ * dielectric suppression, Klein-Nishina, Bethe-Heitler pair production with
+ * Coulomb correction, a K/L-edge photoabsorption model, screened-Rutherford
+ * transport cross sections, RITA-parametrised bremsstrahlung sampling
+ * tables). Table physics is approximate; run-time parity is defined on
+ * identical tables.
+ */
+#ifndef G4HEPEM_AMD_TABLES_H
+#define G4HEPEM_AMD_TABLES_H
+
+#include <stdint.h>
+#include "g4hepem_amd_data.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Build tables for `nmat` materials (one material-cuts couple each, in the
+ * given order, all in detector region 0) with a common production cut given
+ * as a range in mm. Known material names: G4_Pb, G4_lAr, G4_Si, G4_PbWO4,
+ * G4_W, G4_Cu, G4_Fe, G4_WATER. Returns 0 or a negative hepem_status. */
+int hepem_tables_build(const char* const* material_names, int nmat, double prod_cut_mm,
+                       struct G4HepEmData** out_data, struct G4HepEmParameters** out_pars);
+
+/* Free everything hepem_tables_build allocated. */
+int hepem_tables_free(struct G4HepEmData* data, struct G4HepEmParameters* pars);
+
+/* Serialise every table value (and the parameters) into a flat byte stream in
+ * a fixed order. With buf == NULL only *size is set. Used for the digest of
+ * the table content. */
+int hepem_tables_serialize(const struct G4HepEmData* data, const struct G4HepEmParameters* pars,
+                           uint8_t* buf, uint64_t* size);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* G4HEPEM_AMD_TABLES_H */

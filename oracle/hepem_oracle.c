@@ -1,0 +1,1982 @@
+/*
+ * hepem_oracle.c -- TEST INFRASTRUCTURE ONLY (see hepem_oracle.h).
+ *
+ * Operation-by-operation C restatement of the reference's G4HepEmRun
+ * run-time physics (paths relative to the reference tree, directory
+ * G4HepEm/G4HepEmRun/include unless given). Compiled with
+ * -ffp-contract=off so that every product and sum is rounded as written.
+ *
+ * The only deliberate departures from the reference are:
+ *  - the random engine: G4HepEmRandomEngine::flat/flatArray are consumer
+ *    supplied (G4HepEmRandomEngine.hh:22-47); here they draw consecutive
+ *    doubles from a per-track Philox4x32-10 stream;
+ *  - gamma SampleInteraction (G4HepEmGammaManager.icc:203-208) reads a fifth
+ *    "process" beyond the stored four when the photon falls into the
+ *    gamma-nuclear bin; that value is never used (the interaction is not
+ *    handled), so it is not read here.
+ */
+#include "hepem_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- constants: G4HepEmConstants.hh:7-29 ------------------------------------ */
+static const double kPi = 3.1415926535897931e+00;
+static const double k2Pi = 2.0 * 3.1415926535897931e+00;
+static const double kElectronMassC2 = 5.1099890999999997e-01;
+static const double kInvElectronMassC2 = 1.0 / 5.1099890999999997e-01;
+static const double kAlpha = 7.2973525653052150e-03;
+static const double kPir02 = 2.4946724123674787e-23;
+static const double kMigdalConst = 5.2804955733859579e-30;
+static const double kLPMconstant = 7.6843819381368661e+05;
+static const double kALargeValue = 1.0E+20;
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
+/* ---- VDT/Cephes exp and log: G4HepEmExp.hh:182-223, G4HepEmLog.hh:228-263 ----
+ * The polynomial coefficients are the published Cephes ones (S. Moshier). */
+static inline uint64_t d2u(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static inline double u2d(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+static inline uint32_t f2u(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+
+double oracle_vdt_exp(double initial_x) {
+  double x = initial_x;
+  /* floor of LOG2E*x+0.5 as VDT computes it: truncation, minus the sign bit of
+     the value rounded to float (G4HepEmExp.hh:110-114) */
+  const double t = 1.4426950408889634073599 * x + 0.5;
+  int32_t fl = (int32_t)t;
+  fl -= (int32_t)(f2u((float)t) >> 31);
+  double px = (double)fl;
+  const int32_t n = (int32_t)px;
+  x -= px * 6.93145751953125E-1;
+  x -= px * 1.42860682030941723212E-6;
+  const double xx = x * x;
+  px = 1.26177193074810590878E-4;
+  px *= xx;
+  px += 3.02994407707441961300E-2;
+  px *= xx;
+  px += 9.99999999999999999910E-1;
+  px *= x;
+  double qx = 3.00198505138664455042E-6;
+  qx *= xx;
+  qx += 2.52448340349684104192E-3;
+  qx *= xx;
+  qx += 2.27265548208155028766E-1;
+  qx *= xx;
+  qx += 2.00000000000000000009E0;
+  x = px / (qx - px);
+  x = 1.0 + 2.0 * x;
+  x *= u2d((((uint64_t)(int64_t)n) + 1023) << 52);
+  if (initial_x > 708.0) x = INFINITY;
+  if (initial_x < -708.0) x = 0.;
+  return x;
+}
+
+double oracle_vdt_log(double x) {
+  const double original_x = x;
+  uint64_t n = d2u(x);
+  const int32_t e = (int32_t)(n >> 52);
+  double fe = e - 1023;
+  n &= 0x800FFFFFFFFFFFFFULL;
+  n |= 0x3FE0000000000000ULL;
+  x = u2d(n);
+  if (x > 0.70710678118654752440) fe += 1.; else x += x;
+  x -= 1.0;
+  double px = 1.01875663804580931796E-4;
+  px *= x; px += 4.97494994976747001425E-1;
+  px *= x; px += 4.70579119878881725854E0;
+  px *= x; px += 1.44989225341610930846E1;
+  px *= x; px += 1.79368678507819816313E1;
+  px *= x; px += 7.70838733755885391666E0;
+  const double x2 = x * x;
+  px *= x;
+  px *= x2;
+  double qx = x;
+  qx += 1.12873587189167450590E1;
+  qx *= x; qx += 4.52279145837532221105E1;
+  qx *= x; qx += 8.29875266912776603211E1;
+  qx *= x; qx += 7.11544750618563894466E1;
+  qx *= x; qx += 2.31251620126765340583E1;
+  double res = px / qx;
+  res -= fe * 2.121944400546905827679e-4;
+  res -= 0.5 * x2;
+  res = x + res;
+  res += fe * 0.693359375;
+  if (original_x > 1e307) res = INFINITY;
+  if (original_x < 0) res = -NAN;
+  return res;
+}
+
+#define HLog(x) oracle_vdt_log(x)
+#define HExp(x) oracle_vdt_exp(x)
+static inline double HPow(double x, double a) { return oracle_vdt_exp(a * oracle_vdt_log(x)); }
+
+/* ---- RNG: Philox4x32-10 stream + G4HepEmRandomEngine.hh:50-99 --------------- */
+void oracle_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+typedef struct orng {
+  uint32_t key[2];
+  uint32_t id;
+  uint32_t ctr;       /* draw index */
+  int      has_gauss; /* G4HepEmRandomEngine::fIsGauss */
+  double   gauss;     /* G4HepEmRandomEngine::fGauss   */
+} orng;
+
+/* draw d of stream id: Philox block d/2, counter {d/2, id, 0, 0}; double j of a
+   block from words (2j, 2j+1): 52 high bits k -> (k + 0.5) 2^-52 in (0,1) */
+static inline double rng_draw(const uint32_t key[2], uint32_t id, uint32_t d) {
+  const uint32_t c[4] = {d >> 1, id, 0u, 0u};
+  uint32_t o[4];
+  oracle_philox4x32_10(c, key, o);
+  const int j = (int)(d & 1u);
+  const uint64_t w = ((uint64_t)o[2 * j + 1] << 32) | (uint64_t)o[2 * j];
+  return ((double)(w >> 12) + 0.5) * 2.220446049250313080847e-16;
+}
+
+void oracle_rng_uniforms(uint64_t seed, uint32_t id, uint32_t first, int n, double* out) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int i = 0; i < n; ++i) out[i] = rng_draw(key, id, first + (uint32_t)i);
+}
+
+static inline double flat(orng* r) { return rng_draw(r->key, r->id, r->ctr++); }
+static inline void flatArray(orng* r, int n, double* v) { for (int i = 0; i < n; ++i) v[i] = flat(r); }
+
+static double Gauss(orng* r, double mean, double stDev) {
+  if (r->has_gauss) {
+    r->has_gauss = 0;
+    return r->gauss * stDev + mean;
+  }
+  double rnd[2], rr, v1, v2;
+  do {
+    flatArray(r, 2, rnd);
+    v1 = 2. * rnd[0] - 1.;
+    v2 = 2. * rnd[1] - 1.;
+    rr = v1 * v1 + v2 * v2;
+  } while (rr > 1.);
+  const double fac = sqrt(-2. * HLog(rr) / rr);
+  r->gauss = v1 * fac;
+  r->has_gauss = 1;
+  return v2 * fac * stDev + mean;
+}
+
+static int Poisson(orng* r, double mean) {
+  const int border = 16;
+  const double limit = 2.E+9;
+  int number = 0;
+  if (mean <= border) {
+    const double position = flat(r);
+    double poissonValue = HExp(-mean);
+    double poissonSum = poissonValue;
+    /* bounded at 1000 terms: for mean <= 16 the sum reaches 1 within double
+       rounding long before, so the bound only acts where the reference's
+       unbounded loop (G4HepEmRandomEngine.hh:83) would never end */
+    while (poissonSum <= position && number < 1000) {
+      ++number;
+      poissonValue *= mean / number;
+      poissonSum += poissonValue;
+    }
+    return number;
+  }
+  double rnd[2];
+  flatArray(r, 2, rnd);
+  const double t = sqrt(-2. * HLog(rnd[0])) * cos(k2Pi * rnd[1]);
+  const double value = mean + t * sqrt(mean) + 0.5;
+  return value < 0. ? 0 : value >= limit ? (int)limit : (int)value;
+}
+
+/* ---- G4HepEmRunUtils.icc ------------------------------------------------------ */
+static void RotateToReferenceFrame(double* dir, const double* refDir) { /* :32-46 */
+  double up = refDir[0] * refDir[0] + refDir[1] * refDir[1];
+  if (up > 0.) {
+    up = sqrt(up);
+    const double px = dir[0], py = dir[1], pz = dir[2];
+    dir[0] = (refDir[0] * refDir[2] * px - refDir[1] * py) / up + refDir[0] * pz;
+    dir[1] = (refDir[1] * refDir[2] * px + refDir[0] * py) / up + refDir[1] * pz;
+    dir[2] = -up * px + refDir[2] * pz;
+  } else if (refDir[2] < 0.) {
+    dir[0] = -dir[0];
+    dir[2] = -dir[2];
+  }
+}
+
+static inline double GetSpline(double x1, double x2, double y1, double y2, double sd1, double sd2, double x) { /* :49-61 */
+  const double dl = x2 - x1;
+  const double b = dmax(0., dmin(1., (x - x1) / dl));
+  const double os = 0.166666666667;
+  const double c0 = (2.0 - b) * sd1;
+  const double c1 = (1.0 + b) * sd2;
+  return y1 + b * (y2 - y1) + (b * (b - 1.0)) * (c0 + c1) * (dl * dl * os);
+}
+static inline double GetLinear(double x1, double x2, double y1, double y2, double x) { /* :63-71 */
+  const double dl = x2 - x1;
+  const double b = dmax(0., dmin(1., (x - x1) / dl));
+  return y1 + b * (y2 - y1);
+}
+static inline int LogBin(double logx, double logxmin, double invLD, int ndata) {
+  return (int)dmax(0., dmin((logx - logxmin) * invLD, ndata - 2.));
+}
+/* x grid separate, [y,y''] interleaved: :84-91 */
+static inline double GetSplineLog2(int ndata, const double* xdata, const double* ydata, double x, double logx,
+                                   double logxmin, double invLD) {
+  const double xv = dmax(xdata[0], dmin(xdata[ndata - 1], x));
+  const int idx = LogBin(logx, logxmin, invLD, ndata);
+  const int idx2 = 2 * idx;
+  return GetSpline(xdata[idx], xdata[idx + 1], ydata[idx2], ydata[idx2 + 2], ydata[idx2 + 1], ydata[idx2 + 3], xv);
+}
+/* [x,y,y''] interleaved: :95-102 */
+static inline double GetSplineLog3(int ndata, const double* data, double x, double logx, double logxmin, double invLD) {
+  const double xv = dmax(data[0], dmin(data[3 * (ndata - 1)], x));
+  const int idx = LogBin(logx, logxmin, invLD, ndata);
+  const int idx3 = 3 * idx;
+  return GetSpline(data[idx3], data[idx3 + 3], data[idx3 + 1], data[idx3 + 4], data[idx3 + 2], data[idx3 + 5], xv);
+}
+static inline double GetLinearLog(int ndata, const double* data, double x, double logx, double logxmin, double invLD) { /* :125-133 */
+  const int idx = LogBin(logx, logxmin, invLD, ndata);
+  const int i0 = 2 * idx, i1 = i0 + 2;
+  return GetLinear(data[i0], data[i1], data[i0 + 1], data[i1 + 1], x);
+}
+static inline void GetLinearLog2All(int ndata, const double* data, double x, double logx, double logxmin, double invLD,
+                                    double res[2]) { /* :146-161 */
+  const int idx = LogBin(logx, logxmin, invLD, ndata);
+  const int i0 = 3 * idx, i1 = i0 + 3;
+  const double dl = data[i1] - data[i0];
+  const double b = dmax(0., dmin(1., (x - data[i0]) / dl));
+  for (int i = 1; i < 3; ++i) {
+    const double y1 = data[i0 + i], y2 = data[i1 + i];
+    res[i - 1] = dmax(0.0, y1 + b * (y2 - y1));
+  }
+}
+static inline double GetSplineLog4(int ndata, const double* data, double x, double logx, double logxmin, double invLD,
+                                   int iwhich) { /* :178-187 */
+  const int idx = LogBin(logx, logxmin, invLD, ndata);
+  const int i0 = 9 * idx, i1 = i0 + 9;
+  iwhich = (iwhich - 1) * 2 + 1;
+  return GetSpline(data[i0], data[i1], data[i0 + iwhich], data[i1 + iwhich], data[i0 + iwhich + 1], data[i1 + iwhich + 1], x);
+}
+static inline int FindLowerBinIndex(const double* xdata, int num, double x, int step) { /* :227-236 */
+  int ml = -1, mu = num - 1;
+  while (abs(mu - ml) > 1) {
+    const int mav = (int)(0.5 * (ml + mu));
+    if (x < xdata[step * mav]) mu = mav; else ml = mav;
+  }
+  return mu - 1;
+}
+
+/* ---- track state (G4HepEmTrack.hh, G4HepEmElectronTrack.hh, G4HepEmGammaTrack.hh,
+        G4HepEmMSCTrackData.hh) ------------------------------------------------ */
+typedef struct otrack {
+  double dir[3], ekin, lekin, charge, edep, gstep, mfp[4], nial[4], safety;
+  int imc, winner, on_boundary;
+} otrack;
+typedef struct omsc {
+  double lambtr1, true_step, zpath, displ[3], dir[3], init_range, dyn_range_factor, tlimit_min, par1, par2, par3;
+  int no_scat, is_displace, first_step, active;
+} omsc;
+typedef struct oel { otrack t; omsc msc; double range, pstep, pre_ekin, pre_lekin; } oel;
+typedef struct ogm { otrack t; double pe_mxsec; } ogm;
+typedef struct osec { int ptype; double ekin, dir[3]; } osec;
+typedef struct otl {
+  oel el; ogm gm; orng* rng;
+  int nsec_el, nsec_gm;
+  osec sec_el[2], sec_gm[2];
+} otl;
+
+static inline void SetEKin(otrack* t, double e) { t->ekin = e; t->lekin = 100.0; }
+static inline double GetLogEKin(otrack* t) { /* G4HepEmTrack.hh:95-100 */
+  if (t->lekin > 99.0) t->lekin = (t->ekin > 0.) ? HLog(t->ekin) : -30;
+  return t->lekin;
+}
+static inline void TrackReset(otrack* t) {
+  memset(t, 0, sizeof(*t));
+  t->lekin = 100.0;
+  for (int i = 0; i < 4; ++i) { t->mfp[i] = -1.0; t->nial[i] = -1.0; }
+  t->imc = -1; t->winner = -1; t->on_boundary = 0;
+}
+static inline void MscReset(omsc* m) { /* G4HepEmMSCTrackData.hh ReSet */
+  memset(m, 0, sizeof(*m));
+  m->dir[2] = 1.;
+  m->init_range = 1.0e+21;
+  m->dyn_range_factor = 0.04;
+  m->tlimit_min = 1.0E-7;
+  m->par1 = -1.;
+  m->first_step = 1;
+}
+static inline osec* AddSecEl(otl* tl, int ptype) { osec* s = &tl->sec_el[tl->nsec_el++]; s->ptype = ptype; return s; }
+static inline osec* AddSecGm(otl* tl) { osec* s = &tl->sec_gm[tl->nsec_gm++]; s->ptype = HEPEM_GAMMA; return s; }
+
+/* ---- G4HepEmElectronManager.icc: table accessors ----------------------------- */
+static inline const struct G4HepEmElectronData* ElData(const struct G4HepEmData* d, int isEl) {
+  return isEl ? d->fTheElectronData : d->fThePositronData;
+}
+static double GetRestRange(const struct G4HepEmElectronData* e, int imc, double ekin, double lekin) { /* :486-493 */
+  const int N = e->fELossEnergyGridSize;
+  const double r = GetSplineLog2(N, e->fELossEnergyGrid, &e->fELossData[5 * N * imc], ekin, lekin, e->fELossLogMinEkin, e->fELossEILDelta);
+  return dmax(0.0, r);
+}
+static double GetRestDEDX(const struct G4HepEmElectronData* e, int imc, double ekin, double lekin) { /* :495-502 */
+  const int N = e->fELossEnergyGridSize;
+  const double r = GetSplineLog2(N, e->fELossEnergyGrid, &e->fELossData[N * (5 * imc + 2)], ekin, lekin, e->fELossLogMinEkin, e->fELossEILDelta);
+  return dmax(0.0, r);
+}
+static double GetInvRange(const struct G4HepEmElectronData* e, int imc, double range) { /* :504-520 */
+  const int N = e->fELossEnergyGridSize;
+  const int iR = 5 * N * imc;
+  const double minRange = e->fELossData[iR];
+  if (range < minRange) {
+    const double dum = range / minRange;
+    return dmax(0.0, e->fELossEnergyGrid[0] * dum * dum);
+  }
+  const int i = FindLowerBinIndex(&e->fELossData[iR], N, range, 2);
+  const double* xd = &e->fELossData[iR];
+  const double* sd = &e->fELossData[iR + 4 * N];
+  const double en = GetSpline(xd[2 * i], xd[2 * (i + 1)], e->fELossEnergyGrid[i], e->fELossEnergyGrid[i + 1], sd[i], sd[i + 1], range);
+  return dmax(0.0, en);
+}
+static double GetRestMacXSec(const struct G4HepEmElectronData* e, int imc, double ekin, double lekin, int isioni) { /* :522-532 */
+  const int iIoni = e->fResMacXSecStartIndexPerMatCut[imc];
+  const int nIoni = (int)e->fResMacXSecData[iIoni];
+  const int iStart = isioni ? iIoni : iIoni + 3 * nIoni + 5;
+  const int nData = (int)e->fResMacXSecData[iStart];
+  const double minE = e->fResMacXSecData[iStart + 5];
+  if (ekin < minE) return 0.0;
+  const double m = GetSplineLog3(nData, &e->fResMacXSecData[iStart + 5], ekin, lekin, e->fResMacXSecData[iStart + 3], e->fResMacXSecData[iStart + 4]);
+  return dmax(0.0, m);
+}
+static double GetMacXSecNuclear(const struct G4HepEmElectronData* e, int imat, double ekin, double lekin) { /* :534-541 */
+  if (ekin < e->fENucEnergyGrid[0]) return 0.0;
+  const int n = e->fENucEnergyGridSize;
+  const double m = GetSplineLog2(n, e->fENucEnergyGrid, &e->fENucMacXsecData[imat * 2 * n], ekin, lekin, e->fENucLogMinEkin, e->fENucEILDelta);
+  return dmax(0.0, m);
+}
+static double GetRestMacXSecForStepping(const struct G4HepEmElectronData* e, int imc, double ekin, double lekin, int isioni) { /* :544-568 */
+  const double log08 = -0.22314355131420971;
+  const int iIoni = e->fResMacXSecStartIndexPerMatCut[imc];
+  const int nIoni = (int)e->fResMacXSecData[iIoni];
+  const int iStart = isioni ? iIoni : iIoni + 3 * nIoni + 5;
+  const int nData = (int)e->fResMacXSecData[iStart];
+  const double minE = e->fResMacXSecData[iStart + 5];
+  const double maxE = e->fResMacXSecData[iStart + 1];
+  const double maxV = e->fResMacXSecData[iStart + 2];
+  if (ekin > maxE) {
+    const double ekinReduced = 0.8 * ekin;
+    if (ekinReduced < maxE) return dmax(0.0, maxV);
+    ekin = ekinReduced;
+    lekin += log08;
+  }
+  if (ekin < minE) return 0.0;
+  const double m = GetSplineLog3(nData, &e->fResMacXSecData[iStart + 5], ekin, lekin, e->fResMacXSecData[iStart + 3], e->fResMacXSecData[iStart + 4]);
+  return dmax(0.0, m);
+}
+static double GetTransportMFP(const struct G4HepEmElectronData* e, int im, double ekin, double lekin) { /* :576-583 */
+  const int N = e->fELossEnergyGridSize;
+  const double tr = dmax(0.0, GetSplineLog2(N, e->fELossEnergyGrid, &e->fTr1MacXSecData[2 * N * im], ekin, lekin, e->fELossLogMinEkin, e->fELossEILDelta));
+  return tr > 0. ? 1. / tr : kALargeValue;
+}
+static double ComputeMacXsecAnnihilation(double ekin, double eDens) { /* :585-593 */
+  const double tau = ekin * kInvElectronMassC2;
+  const double gam = tau + 1.0;
+  const double gam2 = gam * gam;
+  const double bg2 = tau * (tau + 2.0);
+  const double bg = sqrt(bg2);
+  return eDens * kPir02 * ((gam2 + 4. * gam + 1.) * HLog(gam + bg) - (gam + 3.) * bg) / (bg2 * (gam + 1.));
+}
+static double ComputeMacXsecAnnihilationForStepping(double ekin, double eDens) { return ComputeMacXsecAnnihilation(0.8 * ekin, eDens); }
+
+/* ---- G4HepEmElectronInteractionUMSC.icc -------------------------------------- */
+static void UMSC_StepLimit(const struct G4HepEmData* d, const struct G4HepEmParameters* p, omsc* m, double ekin, int imat,
+                           int ireg, double range, double presafety, int onBoundary, int isEl, orng* rng) { /* :19-126 */
+  m->no_scat = 0;
+  m->is_displace = 1;
+  const double kTLimitMinfix = 1.0E-8;
+  const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[imat];
+  if (m->true_step < kTLimitMinfix || range * md->fUMSCPar < presafety) {
+    m->is_displace = 0;
+    return;
+  }
+  const struct G4HepEmRegionParmeters* rp = &p->fParametersPerRegion[ireg];
+  const double rangeFactor = rp->fMSCRangeFactor, safetyFactor = rp->fMSCSafetyFactor;
+  const int useSafety = !rp->fIsMSCMinimalStepLimit;
+  double tlimit = 0.0;
+  if (useSafety) {
+    if (m->first_step || onBoundary) {
+      const double lambdaTr1 = m->lambtr1;
+      m->init_range = dmax(range, lambdaTr1);
+      m->dyn_range_factor = lambdaTr1 > 1.0 ? rangeFactor * (0.75 + 0.25 * lambdaTr1) : rangeFactor;
+      const double stepMin = lambdaTr1 * 1.0E-3 / (2.0E-3 + ekin * (md->fUMSCStepMinPars[0] + ekin * md->fUMSCStepMinPars[1]));
+      const double dum0 = isEl ? 0.87 * md->fZeff23 : 0.70 * md->fZeffSqrt;
+      const double dum1 = ekin > 5.0E-3 ? dum0 * stepMin : dum0 * stepMin * 0.5 * (1.0 + ekin * 200.0);
+      m->tlimit_min = dmax(dum1, kTLimitMinfix);
+      m->first_step = 0;
+    }
+    const double tlimitmin = m->tlimit_min;
+    tlimit = range > presafety ? dmax(dmax(m->init_range * m->dyn_range_factor, safetyFactor * presafety), tlimitmin)
+                               : dmax(range, tlimitmin);
+  } else {
+    if (onBoundary) {
+      const double lambdaTr1 = m->lambtr1;
+      const double tmp = range > lambdaTr1 ? rangeFactor * range : rangeFactor * lambdaTr1;
+      m->init_range = dmax(tmp, 10 * kTLimitMinfix);
+    }
+    tlimit = m->init_range;
+  }
+  const double tlimitmin = m->tlimit_min;
+  if (tlimit < m->true_step) {
+    const double dum0 = tlimit > tlimitmin ? dmax(Gauss(rng, tlimit, 0.1 * (tlimit - tlimitmin)), tlimitmin) : tlimitmin;
+    m->true_step = dmin(dum0, m->true_step);
+  }
+}
+
+static double UMSC_SimpleScattering(double xmeanth, double x2meanth, orng* rng) { /* :280-293 */
+  const double dum0 = 3. * x2meanth - 1;
+  const double dum1 = 2. * xmeanth - dum0;
+  const double a = 1. + 4. * dum0 / dum1;
+  const double prob = (2. + a) * xmeanth / a;
+  double r[2];
+  flatArray(rng, 2, r);
+  return (r[0] < prob) ? -1. + 2. * HPow(r[1], 1. / (1. + a)) : -1. + 2. * r[1];
+}
+
+static double UMSC_Theta0PositronCorrection(double eekin, double zeff) { /* :313-338 */
+  const double ff = 1. + zeff * (1.84035E-4 * zeff - 1.86427E-2) + 0.41125;
+  const double a = 0.994 - 4.08E-3 * zeff;
+  const double b = 7.16 + (52.6 + 365. / zeff) / zeff;
+  const double tu = sqrt(eekin) * kInvElectronMassC2;
+  const double x = sqrt(tu * (tu + 2.) / ((tu + 1.) * (tu + 1.)));
+  const double xl = 0.6;
+  if (x < xl) return ff * a * (1. - HExp(-b * x));
+  const double c = 1.00 - 4.47E-3 * zeff;
+  const double dd = 1.21E-3 * zeff;
+  const double e = 113.0;
+  const double xh = 0.9;
+  if (x > xh) return ff * (c + dd * HExp(e * (x - 1.)));
+  const double yl = a * (1. - HExp(-b * xl));
+  const double yh = c + dd * HExp(e * (xh - 1.));
+  const double y0 = (yh - yl) / (xh - xl);
+  const double y1 = yl - y0 * xl;
+  return ff * (y0 * x + y1);
+}
+
+static double UMSC_ComputeTheta0(double stepInRadLength, double postEkin, double preEkin, double zeff, const double* thetaCoeff,
+                                 int isEl, int isPosCor) { /* :297-309 */
+  const double kHighland = 13.6;
+  const double postInvBetaPc = (postEkin + kElectronMassC2) / (postEkin * (postEkin + 2. * kElectronMassC2));
+  const double invBetaPc = preEkin != postEkin
+      ? sqrt(postInvBetaPc * (preEkin + kElectronMassC2) / (preEkin * (preEkin + 2. * kElectronMassC2)))
+      : postInvBetaPc;
+  const double y = (isEl || !isPosCor) ? stepInRadLength : stepInRadLength * UMSC_Theta0PositronCorrection(preEkin * postEkin, zeff);
+  return kHighland * sqrt(y) * invBetaPc * (thetaCoeff[0] + thetaCoeff[1] * HLog(y));
+}
+
+static double UMSC_SampleCosineTheta(double pStep, double preEkin, double preTr1mfp, double postEkin, double postTr1mfp,
+                                     double tlimitMin, double radLength, double zeff, const double* tailCoeff,
+                                     const double* thetaCoeff, int isEl, int isPosCor, orng* rng) { /* :152-277 */
+  const double iPreTr1mfp = 1.0 / preTr1mfp;
+  const double deltaR1mfp = preTr1mfp - postTr1mfp;
+  const double tau = fabs(deltaR1mfp) > 0.01 * preTr1mfp ? pStep * HLog(preTr1mfp / postTr1mfp) / deltaR1mfp : pStep * iPreTr1mfp;
+  const double kTauBig = 8.0;
+  if (tau > kTauBig) return 2.0 * flat(rng) - 1.0;
+  const double kTauSmall = 1.0E-16;
+  if (tau < kTauSmall) return 1.0;
+  double xmeanth, x2meanth;
+  if (tau < 0.01) {
+    xmeanth = 1.0 - tau * (1.0 - 0.5 * tau);
+    x2meanth = 1.0 - tau * (5.0 - 6.25 * tau) * 0.333333;
+  } else {
+    xmeanth = HExp(-tau);
+    x2meanth = (1.0 + 2.0 * HExp(-2.5 * tau)) * 0.333333;
+  }
+  if (postEkin < 0.5 * preEkin) return UMSC_SimpleScattering(xmeanth, x2meanth, rng);
+  const double tsmall = dmin(tlimitMin, 1.0);
+  const int notSmall = pStep > tsmall;
+  const double theta0 = notSmall
+      ? UMSC_ComputeTheta0(pStep / radLength, postEkin, preEkin, zeff, thetaCoeff, isEl, isPosCor)
+      : UMSC_ComputeTheta0(tsmall / radLength, postEkin, preEkin, zeff, thetaCoeff, isEl, isPosCor) * sqrt(pStep / tsmall);
+  if (theta0 > kPi * 0.166666) return UMSC_SimpleScattering(xmeanth, x2meanth, rng);
+  const double theta2 = theta0 * theta0;
+  if (theta2 < kTauSmall) return 1.0;
+  const double dumtau = notSmall ? tau : tsmall * iPreTr1mfp;
+  const double parU = HPow(dumtau, 0.1666666);
+  const double dumxsi = tailCoeff[0] + parU * (tailCoeff[1] + parU * tailCoeff[2]) + tailCoeff[3] * HLog(pStep / (tau * radLength));
+  const double parXsi = dmax(dumxsi, 1.9);
+  const double parC = fabs(parXsi - 3.) < 0.001 ? 3.001 : fabs(parXsi - 2.) < 0.001 ? 2.001 : parXsi;
+  const double dumC1 = parC - 1.;
+  const double dumEa = HExp(-parXsi);
+  const double dumEaa = 1. / (1. - dumEa);
+  double thex = theta2 * (1.0 - theta2 * 0.0833333);
+  if (theta2 > 0.01) {
+    const double dum = 2.0 * sin(0.5 * theta0);
+    thex = dum * dum;
+  }
+  const double xmean1 = 1. - (1. - (1. + parXsi) * dumEa) * thex * dumEaa;
+  if (xmean1 <= 0.999 * xmeanth) return UMSC_SimpleScattering(xmeanth, x2meanth, rng);
+  const double x0 = 1. - parXsi * thex;
+  const double bx = parC * thex;
+  const double b = bx + x0;
+  const double b1 = b + 1.;
+  const double eb1 = HPow(b1, dumC1);
+  const double ebx = HPow(bx, dumC1);
+  const double dd = ebx / eb1;
+  const double xmean2 = (x0 + dd - (bx - b1 * dd) / (parC - 2.)) / (1. - dd);
+  const double f1x0 = dumEa * dumEaa;
+  const double f2x0 = dumC1 / (parC * (1. - dd));
+  const double prob = f2x0 / (f1x0 + f2x0);
+  const double qprb = xmeanth / (prob * xmean1 + (1. - prob) * xmean2);
+  double r[3];
+  flatArray(rng, 3, r);
+  if (r[0] < qprb) {
+    if (r[1] < prob) return 1. + HLog(dumEa + r[2] / dumEaa) * thex;
+    const double var0 = (1.0 - dd) * r[2];
+    if (var0 < 0.01 * dd) {
+      const double var = var0 / (dd * dumC1);
+      return -1.0 + var * (1.0 - var * 0.5 * parC) * b1;
+    }
+    return 1.0 + thex * (parC - parXsi - parC * HPow(var0 + dd, -1. / dumC1));
+  }
+  return 2.0 * r[1] - 1.0;
+}
+
+static void UMSC_SampleDisplacement(double pStep, double thePhi, omsc* m, orng* rng) { /* :342-357 */
+  const double r = 0.73 * sqrt((pStep - m->zpath) * (pStep + m->zpath));
+  const double cbeta = 2.16;
+  const double cbeta1 = 1. - HExp(-cbeta * kPi);
+  double rr[2];
+  flatArray(rng, 2, rr);
+  const double psi = -HLog(1. - rr[0] * cbeta1) / cbeta;
+  const double phi = (rr[1] < 0.5) ? thePhi + psi : thePhi - psi;
+  m->displ[0] = r * cos(phi);
+  m->displ[1] = r * sin(phi);
+  m->displ[2] = 0.0;
+}
+
+static void UMSC_SampleScattering(const struct G4HepEmData* d, omsc* m, double pStep, double preEkin, double preTr1mfp,
+                                  double postEkin, double postTr1mfp, int imat, int isEl, int isPosCor, int isDisplace,
+                                  orng* rng) { /* :129-149 */
+  const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[imat];
+  const double cost = UMSC_SampleCosineTheta(pStep, preEkin, preTr1mfp, postEkin, postTr1mfp, m->tlimit_min, md->fRadiationLength,
+                                             md->fZeff, md->fUMSCTailCoeff, md->fUMSCThetaCoeff, isEl, isPosCor, rng);
+  if (fabs(cost) >= 1.0) {
+    m->no_scat = 1;
+    return;
+  }
+  const double sth = sqrt((1.0 - cost) * (1.0 + cost));
+  const double phi = k2Pi * flat(rng);
+  m->dir[0] = sth * cos(phi);
+  m->dir[1] = sth * sin(phi);
+  m->dir[2] = cost;
+  m->is_displace = m->is_displace && isDisplace;
+  if (m->is_displace && pStep > m->zpath) UMSC_SampleDisplacement(pStep, phi, m, rng);
+}
+
+/* ---- G4HepEmElectronEnergyLossFluctuation.icc -------------------------------- */
+static double SampleGaussianLoss(double meane, double sig2e, orng* rng) { /* :100-111 */
+  const double twom = 2. * meane;
+  if (meane * meane < 0.0625 * sig2e) return twom * flat(rng);
+  const double sig = sqrt(sig2e);
+  double eloss;
+  do {
+    eloss = Gauss(rng, meane, sig);
+  } while (eloss < 0. || eloss > twom);
+  return eloss;
+}
+
+static double SampleEnergyLossFluctuation(double tcut, double excEner, double meanELoss, orng* rng) { /* :11-97 */
+  const double scaling = dmin(1. + 5.E-4 / tcut, 1.5);
+  const double meanLoss = meanELoss / scaling;
+  const double kRate = 0.56, kE0 = 1.E-5, kNMaxCont = 8.;
+  const double w1 = tcut / kE0;
+  double a3 = meanLoss * (tcut - kE0) / (kE0 * tcut * HLog(w1));
+  double a1 = 0., e1 = excEner, eloss = 0.0;
+  if (tcut > excEner) {
+    const double a1Tmp = meanLoss * (1. - kRate) / excEner;
+    const double kA0 = 42., kFw = 4.;
+    const double dum0 = a1Tmp < kA0 ? .1 + (kFw - .1) * sqrt(a1Tmp / kA0) : kFw;
+    a1 = a1Tmp / dum0;
+    e1 *= dum0;
+    a3 *= kRate;
+    if (a1 > kNMaxCont) {
+      const double emean = a1 * e1;
+      const double sig2e = emean * e1;
+      eloss = SampleGaussianLoss(emean, sig2e, rng);
+    } else {
+      const int pp = Poisson(rng, a1);
+      eloss = pp > 0 ? ((pp + 1) - 2. * flat(rng)) * e1 : 0.;
+    }
+  }
+  if (a3 > 0.) {
+    double p3 = a3, alfa = 1.;
+    if (a3 > kNMaxCont) {
+      alfa = w1 * (kNMaxCont + a3) / (w1 * kNMaxCont + a3);
+      const double alfa1 = alfa * HLog(alfa) / (alfa - 1.);
+      const double namean = a3 * w1 * (alfa - 1.) / ((w1 - 1.) * alfa);
+      const double emean = namean * kE0 * alfa1;
+      const double sig2e = kE0 * kE0 * namean * (alfa - alfa1 * alfa1);
+      eloss += SampleGaussianLoss(emean, sig2e, rng);
+      p3 = a3 - namean;
+    }
+    const double w3 = alfa * kE0;
+    if (tcut > w3) {
+      const double w = (tcut - w3) / tcut;
+      const int nnb = Poisson(rng, p3);
+      if (nnb > 0) {
+        const int kBlock = 8;
+        const int nBlocks = nnb / kBlock;
+        double rndm[8];
+        for (int ib = 0; ib < nBlocks; ++ib) {
+          flatArray(rng, kBlock, rndm);
+          for (int i = 0; i < kBlock; ++i) eloss += w3 / (1. - w * rndm[i]);
+        }
+        const int nTail = nnb - nBlocks * kBlock;
+        flatArray(rng, nTail, rndm);
+        for (int i = 0; i < nTail; ++i) eloss += w3 / (1. - w * rndm[i]);
+      }
+    }
+  }
+  return eloss * scaling;
+}
+
+/* ---- G4HepEmInteractionUtils.{hh,icc} ----------------------------------------- */
+/* LPM G(s), Phi(s) table on s in [0,2] step 0.05 (Stanev et al. approximation
+   tabulated in G4HepEmInteractionUtils.hh:21-36 and in Geant4's G4eBremsstrahlungRelModel) */
+static const double kFuncLPM[] = {
+  0.0000E+00, 0.0000E+00, 6.9163E-02, 2.5747E-01, 2.0597E-01, 4.4573E-01,
+  3.5098E-01, 5.8373E-01, 4.8095E-01, 6.8530E-01, 5.8926E-01, 7.6040E-01,
+  6.7626E-01, 8.1626E-01, 7.4479E-01, 8.5805E-01, 7.9826E-01, 8.8952E-01,
+  8.4003E-01, 9.1338E-01, 8.7258E-01, 9.3159E-01, 8.9794E-01, 9.4558E-01,
+  9.1776E-01, 9.5640E-01, 9.3332E-01, 9.6483E-01, 9.4560E-01, 9.7143E-01,
+  9.5535E-01, 9.7664E-01, 9.6313E-01, 9.8078E-01, 9.6939E-01, 9.8408E-01,
+  9.7444E-01, 9.8673E-01, 9.7855E-01, 9.8888E-01, 9.8191E-01, 9.9062E-01,
+  9.8467E-01, 9.9204E-01, 9.8695E-01, 9.9321E-01, 9.8884E-01, 9.9417E-01,
+  9.9042E-01, 9.9497E-01, 9.9174E-01, 9.9564E-01, 9.9285E-01, 9.9619E-01,
+  9.9379E-01, 9.9666E-01, 9.9458E-01, 9.9706E-01, 9.9526E-01, 9.9739E-01,
+  9.9583E-01, 9.9768E-01, 9.9632E-01, 9.9794E-01, 9.9674E-01, 9.9818E-01,
+  9.9710E-01, 9.9839E-01, 9.9741E-01, 9.9857E-01, 9.9767E-01, 9.9873E-01,
+  9.9790E-01, 9.9887E-01, 9.9809E-01, 9.9898E-01, 9.9826E-01, 9.9909E-01,
+  9.9840E-01, 9.9918E-01, 9.9856E-01, 9.9926E-01};
+
+static double SampleCostModifiedTsai(double primEkin, orng* rng) { /* .icc:11-25 */
+  const double uMax = 2.0 * (1.0 + primEkin * kInvElectronMassC2);
+  double r[3], u;
+  do {
+    flatArray(rng, 3, r);
+    const double uu = -HLog(r[0] * r[1]);
+    u = (0.25 > r[2]) ? uu * 1.6 : uu * 0.533333333;
+  } while (u > uMax);
+  return 1.0 - 2.0 * u * u / (uMax * uMax);
+}
+
+static void EvaluateLPMFunctions(double* funcXiS, double* funcGS, double* funcPhiS, double egamma, double etotal, double elpm,
+                                 double z23, double ilVarS1, double ilVarS1Cond, double densityCor, double times) { /* .icc:28-76 */
+  const double sqrt2 = 1.414213562373095;
+  const double redegamma = egamma / etotal;
+  const double varSprime = sqrt(0.125 * redegamma * elpm / (times * (1.0 - redegamma) * etotal));
+  const double varS1 = z23 / (184.15 * 184.15);
+  const double condition = sqrt2 * varS1;
+  double funcXiSprime = 2.0;
+  if (varSprime > 1.0) {
+    funcXiSprime = 1.0;
+  } else if (varSprime > condition) {
+    const double funcHSprime = HLog(varSprime) * ilVarS1Cond;
+    funcXiSprime = 1.0 + funcHSprime - 0.08 * (1.0 - funcHSprime) * funcHSprime * (2.0 - funcHSprime) * ilVarS1Cond;
+  }
+  *funcXiS = funcXiSprime;
+  const double varS = varSprime / sqrt(funcXiSprime);
+  double varShat = varS;
+  if (densityCor != 0.0) {
+    varShat *= (1.0 + densityCor / (egamma * egamma));
+    *funcXiS = 2.0;
+    if (varShat > 1.0) {
+      *funcXiS = 1.0;
+    } else if (varShat > varS1) {
+      *funcXiS = 1.0 + HLog(varShat) * ilVarS1;
+    }
+  }
+  const double lpmSLimit = 2.0, lpmISDelt = 20.0;
+  if (varShat < lpmSLimit) {
+    double val = varShat * lpmISDelt;
+    int ilow = (int)val;
+    val -= ilow;
+    ilow *= 2;
+    *funcGS = (kFuncLPM[ilow + 2] - kFuncLPM[ilow]) * val + kFuncLPM[ilow];
+    *funcPhiS = (kFuncLPM[ilow + 3] - kFuncLPM[ilow + 1]) * val + kFuncLPM[ilow + 1];
+  } else {
+    double ss = 1.0 / (varShat * varShat);
+    ss *= ss;
+    *funcGS = 1.0 - 0.0230655 * ss;
+    *funcPhiS = 1.0 - 0.01190476 * ss;
+  }
+  if ((*funcXiS) * (*funcPhiS) > 1.0 || varShat > 0.57) *funcXiS = 1.0 / (*funcPhiS);
+}
+
+/* ---- G4HepEmElectronInteractionIoni.icc ------------------------------------------ */
+static double SampleETransferMoller(double elCut, double primEkin, orng* rng) { /* :50-74 */
+  const double tmin = elCut, tmax = 0.5 * primEkin;
+  const double xmin = tmin / primEkin, xmax = tmax / primEkin;
+  const double gamma = primEkin * kInvElectronMassC2 + 1.0;
+  const double gamma2 = gamma * gamma;
+  const double xminmax = xmin * xmax;
+  const double gg = (2.0 * gamma - 1.0) / gamma2;
+  const double y = 1. - xmax;
+  const double gf = 1.0 - gg * xmax + xmax * xmax * (1.0 - gg + (1.0 - gg * y) / (y * y));
+  double dum, r[2], delta = 0.;
+  do {
+    flatArray(rng, 2, r);
+    delta = xminmax / (xmin * (1.0 - r[0]) + xmax * r[0]);
+    const double xx = 1.0 - delta;
+    dum = 1.0 - gg * delta + delta * delta * (1.0 - gg + (1.0 - gg * xx) / (xx * xx));
+  } while (gf * r[1] > dum);
+  return delta * primEkin;
+}
+static double SampleETransferBhabha(double elCut, double primEkin, orng* rng) { /* :76-108 */
+  const double tmin = elCut, tmax = primEkin;
+  const double xmin = tmin / primEkin, xmax = tmax / primEkin;
+  const double gamma = primEkin * kInvElectronMassC2 + 1.0;
+  const double gamma2 = gamma * gamma;
+  const double beta2 = 1. - 1. / gamma2;
+  const double xminmax = xmin * xmax;
+  const double y = 1.0 / (1.0 + gamma), y2 = y * y, y12 = 1.0 - 2.0 * y;
+  const double b1 = 2.0 - y2, b2 = y12 * (3.0 + y2), y122 = y12 * y12, b4 = y122 * y12, b3 = b4 + y122;
+  const double xmax2 = xmax * xmax;
+  const double gf = 1.0 + (xmax2 * b4 - xmin * xmin * xmin * b3 + xmax2 * b2 - xmin * b1) * beta2;
+  double dum, r[2], delta = 0.;
+  do {
+    flatArray(rng, 2, r);
+    delta = xminmax / (xmin * (1.0 - r[0]) + xmax * r[0]);
+    const double xx = delta * delta;
+    dum = 1.0 + (xx * xx * b4 - delta * xx * b3 + xx * b2 - delta * b1) * beta2;
+  } while (gf * r[1] > dum);
+  return delta * primEkin;
+}
+static void IoniSampleDirections(double primEkin, double deltaEkin, double* secDir, double* primDir, orng* rng) { /* :111-138 */
+  const double elInitETot = primEkin + kElectronMassC2;
+  const double elInitPTot = sqrt(primEkin * (elInitETot + kElectronMassC2));
+  const double deltaPTot = sqrt(deltaEkin * (deltaEkin + 2.0 * kElectronMassC2));
+  const double cost = deltaEkin * (elInitETot + kElectronMassC2) / (deltaPTot * elInitPTot);
+  const double cosTheta = dmax(-1.0, dmin(cost, 1.0));
+  const double sinTheta = sqrt((1.0 - cosTheta) * (1.0 + cosTheta));
+  const double phi = k2Pi * flat(rng);
+  secDir[0] = sinTheta * cos(phi);
+  secDir[1] = sinTheta * sin(phi);
+  secDir[2] = cosTheta;
+  RotateToReferenceFrame(secDir, primDir);
+  primDir[0] = elInitPTot * primDir[0] - deltaPTot * secDir[0];
+  primDir[1] = elInitPTot * primDir[1] - deltaPTot * secDir[1];
+  primDir[2] = elInitPTot * primDir[2] - deltaPTot * secDir[2];
+  const double norm = 1.0 / sqrt(primDir[0] * primDir[0] + primDir[1] * primDir[1] + primDir[2] * primDir[2]);
+  primDir[0] *= norm;
+  primDir[1] *= norm;
+  primDir[2] *= norm;
+}
+static void IoniPerform(otl* tl, const struct G4HepEmData* d, int isEl) { /* :19-47 */
+  otrack* t = &tl->el.t;
+  const double primEkin = t->ekin;
+  const double elCut = d->fTheMatCutData->fMatCutData[t->imc].fSecElProdCutE;
+  const double maxET = isEl ? 0.5 * primEkin : primEkin;
+  if (maxET <= elCut) return;
+  const double delta = isEl ? SampleETransferMoller(elCut, primEkin, tl->rng) : SampleETransferBhabha(elCut, primEkin, tl->rng);
+  osec* s = AddSecEl(tl, HEPEM_ELECTRON);
+  IoniSampleDirections(primEkin, delta, s->dir, t->dir, tl->rng);
+  SetEKin(t, primEkin - delta);
+  s->ekin = delta;
+}
+
+/* ---- G4HepEmElectronInteractionBrem.icc ------------------------------------------ */
+static int BremSelectTargetAtom(const struct G4HepEmElectronData* e, int imc, double ekin, double lekin, double urndn,
+                                int isSB) { /* :266-297 */
+  const int start = isSB ? e->fElemSelectorBremSBStartIndexPerMatCut[imc] : e->fElemSelectorBremRBStartIndexPerMatCut[imc];
+  const double* data = isSB ? &e->fElemSelectorBremSBData[start] : &e->fElemSelectorBremRBData[start];
+  const int numData = (int)data[0];
+  const int numElem = (int)data[1];
+  const double logE0 = data[2], invLD = data[3];
+  const double* xdata = &data[4];
+  const double xv = dmax(xdata[0], dmin(xdata[numElem * (numData - 1)], ekin));
+  const int idxEkin = (int)dmax(0.0, dmin((lekin - logE0) * invLD, numData - 2.0));
+  int i0 = idxEkin * numElem, i1 = i0 + numElem;
+  const double x1 = xdata[i0++], x2 = xdata[i1++];
+  const double dl = x2 - x1;
+  const double b = dmax(0., dmin(1., (xv - x1) / dl));
+  int ie = 0;
+  while (ie < numElem - 1 && urndn > xdata[i0 + ie] + b * (xdata[i1 + ie] - xdata[i0 + ie])) ++ie;
+  return ie;
+}
+static int BremLinSearch(const double* v, int size, double val) { /* :328-344 */
+  int i = 0;
+  const int size3 = 3 * size;
+  while (i + 9 < size3) {
+    if (v[i + 0] > val) return i + 0;
+    if (v[i + 3] > val) return i + 3;
+    if (v[i + 6] > val) return i + 6;
+    if (v[i + 9] > val) return i + 9;
+    i += 12;
+  }
+  while (i < size3) {
+    if (v[i] > val) break;
+    i += 3;
+  }
+  return i;
+}
+static double SampleETransferSB(const struct G4HepEmData* d, double primEkin, double lEkin, int imc, orng* rng, int isEl) { /* :73-182 */
+  const struct G4HepEmMCCData* mc = &d->fTheMatCutData->fMatCutData[imc];
+  const double gamCut = mc->fSecGamProdCutE, logGamCut = mc->fLogSecGamCutE;
+  const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[mc->fHepEmMatIndex];
+  const struct G4HepEmElectronData* e = ElData(d, isEl);
+  const int elemIndx = (md->fNumOfElement > 1) ? BremSelectTargetAtom(e, imc, primEkin, lEkin, flat(rng), 1) : 0;
+  const int iZet = md->fElementVect[elemIndx];
+  const double dZet = (double)iZet;
+  const struct G4HepEmSBTableData* sb = d->fTheSBTableData;
+  const int iStart = sb->fSBTablesStartPerZ[iZet];
+  const int iGamCut = sb->fGammaCutIndices[sb->fGammaCutIndxStartIndexPerMC[imc] + elemIndx];
+  int isCorner = 0, isSimply = 0;
+  int eIdx = (int)(sb->fSBTableData[iStart + 2]);
+  if (primEkin < sb->fElEnergyVect[eIdx]) {
+    const double val = (lEkin - sb->fLogMinElEnergy) * sb->fILDeltaElEnergy;
+    eIdx = (int)val;
+    double pIndxH = val - eIdx;
+    if (sb->fElEnergyVect[eIdx] <= gamCut) {
+      pIndxH = (lEkin - logGamCut) / (sb->fLElEnergyVect[eIdx + 1] - logGamCut);
+      isCorner = 1;
+    }
+    if (flat(rng) < pIndxH) ++eIdx;
+    else if (isCorner) isSimply = 1;
+  }
+  const int minEIndx = (int)(sb->fSBTableData[iStart + 1]);
+  const int numGamCuts = (int)(sb->fSBTableData[iStart + 3]);
+  const int sizeOneE = (int)(numGamCuts + 3 * sb->fNumKappa);
+  const int iSTStart = iStart + 4 + (eIdx - minEIndx) * sizeOneE;
+  const double minV = sb->fSBTableData[iSTStart + iGamCut];
+  const double* stData = &sb->fSBTableData[iSTStart + numGamCuts];
+  const double lKTrans = (logGamCut - lEkin) / (logGamCut - sb->fLElEnergyVect[eIdx]);
+  const double primETot = primEkin + kElectronMassC2;
+  const double dielSupConst = md->fDensityCorFactor * primETot * primETot;
+  double suppression = 1.0, r[2], eGamma = 0.0;
+  do {
+    flatArray(rng, 2, r);
+    double kappa = 1.0;
+    if (!isSimply) {
+      const double cumRV = r[0] * (1.0 - minV) + minV;
+      const int cumLIndx3 = BremLinSearch(stData, sb->fNumKappa, cumRV) - 3;
+      const int cumLIndx = cumLIndx3 / 3;
+      const double cumL = stData[cumLIndx3];
+      const double pA = stData[cumLIndx3 + 1];
+      const double pB = stData[cumLIndx3 + 2];
+      const double cumH = stData[cumLIndx3 + 3];
+      const double lKL = sb->fLKappaVect[cumLIndx];
+      const double lKH = sb->fLKappaVect[cumLIndx + 1];
+      const double dm1 = (cumRV - cumL) / (cumH - cumL);
+      const double dm2 = (1.0 + pA + pB) * dm1;
+      const double dm3 = 1.0 + dm1 * (pA + pB * dm1);
+      const double lKappa = lKL + dm2 / dm3 * (lKH - lKL);
+      kappa = HExp(lKappa * lKTrans);
+    } else {
+      kappa = 1.0 - r[0] * (1.0 - gamCut / primEkin);
+    }
+    eGamma = kappa * primEkin;
+    const double invEGamma = 1.0 / eGamma;
+    suppression = 1.0 / (1.0 + dielSupConst * invEGamma * invEGamma);
+    if (!isEl) {
+      const double e1 = primEkin - gamCut;
+      const double iBeta1 = (e1 + kElectronMassC2) / sqrt(e1 * (e1 + 2.0 * kElectronMassC2));
+      const double e2 = primEkin - eGamma;
+      const double iBeta2 = (e2 + kElectronMassC2) / sqrt(e2 * (e2 + 2.0 * kElectronMassC2));
+      const double dum = kAlpha * k2Pi * dZet * (iBeta1 - iBeta2);
+      suppression = (dum > -12.) ? suppression * HExp(dum) : 0.;
+    }
+  } while (r[1] > suppression);
+  return eGamma;
+}
+static double SampleETransferRB(const struct G4HepEmData* d, double primEkin, double lEkin, int imc, orng* rng, int isEl) { /* :184-263 */
+  const struct G4HepEmMCCData* mc = &d->fTheMatCutData->fMatCutData[imc];
+  const double gamCut = mc->fSecGamProdCutE;
+  const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[mc->fHepEmMatIndex];
+  const struct G4HepEmElectronData* e = ElData(d, isEl);
+  const int elemIndx = (md->fNumOfElement > 1) ? BremSelectTargetAtom(e, imc, primEkin, lEkin, flat(rng), 0) : 0;
+  const int iZet = md->fElementVect[elemIndx];
+  const double dZet = (double)iZet;
+  const int zcl = iZet < d->fTheElementData->fMaxZet ? iZet : d->fTheElementData->fMaxZet;
+  const struct G4HepEmElemData* ed = &d->fTheElementData->fElementData[zcl];
+  const double densityFactor = kMigdalConst * md->fElectronDensity;
+  const double lpmEnergy = kLPMconstant * md->fRadiationLength;
+  const double lpmEnergyLim = sqrt(densityFactor) * lpmEnergy;
+  const double primTotalE = primEkin + kElectronMassC2;
+  const double densityCorr = densityFactor * primTotalE * primTotalE;
+  const int isLPMActive = (primTotalE > lpmEnergyLim);
+  const double zFactor1 = ed->fZFactor1;
+  const double zFactor2 = (1. + 1. / dZet) / 12.;
+  const double rejFuncMax = zFactor1 + zFactor2;
+  const double xmin = HLog(gamCut * gamCut + densityCorr);
+  const double xrange = HLog(primEkin * primEkin + densityCorr) - xmin;
+  double r[2], eGamma, funcVal;
+  do {
+    flatArray(rng, 2, r);
+    eGamma = sqrt(dmax(HExp(xmin + r[0] * xrange) - densityCorr, 0.0));
+    const double y = eGamma / primTotalE;
+    const double onemy = 1. - y;
+    const double dum0 = 0.25 * y * y;
+    if (isLPMActive) {
+      double funcGS, funcPhiS, funcXiS;
+      EvaluateLPMFunctions(&funcXiS, &funcGS, &funcPhiS, eGamma, primTotalE, lpmEnergy, ed->fZet23, ed->fILVarS1, ed->fILVarS1Cond,
+                           densityCorr, 1.0);
+      const double term1 = funcXiS * (dum0 * funcGS + (onemy + 2.0 * dum0) * funcPhiS);
+      funcVal = term1 * zFactor1 + onemy * zFactor2;
+    } else {
+      const double dum1 = onemy + 3. * dum0;
+      if (iZet < 5) {
+        funcVal = dum1 * zFactor1 + onemy * zFactor2;
+      } else {
+        const double dum2 = y / (primTotalE - eGamma);
+        const double gam = dum2 * 100. * kElectronMassC2 / ed->fZet13;
+        const double eps = gam / ed->fZet13;
+        const double gam2 = gam * gam;
+        const double phi1 = 16.863 - 2.0 * HLog(1.0 + 0.311877 * gam2) + 2.4 * HExp(-0.9 * gam) + 1.6 * HExp(-1.5 * gam);
+        const double phi2 = 2.0 / (3.0 + 19.5 * gam + 18.0 * gam2);
+        const double eps2 = eps * eps;
+        const double psi1 = 24.34 - 2.0 * HLog(1.0 + 13.111641 * eps2) + 2.8 * HExp(-8.0 * eps) + 1.2 * HExp(-29.2 * eps);
+        const double psi2 = 2.0 / (3.0 + 120.0 * eps + 1200.0 * eps2);
+        const double logZ = ed->fLogZ;
+        const double Fz = logZ / 3. + ed->fCoulomb;
+        const double invZ = 1. / dZet;
+        funcVal = dum1 * ((0.25 * phi1 - Fz) + (0.25 * psi1 - 2. * logZ / 3.) * invZ) + 0.125 * onemy * (phi2 + psi2 * invZ);
+      }
+    }
+    funcVal = dmax(0.0, funcVal);
+  } while (funcVal < rejFuncMax * r[1]);
+  return eGamma;
+}
+static void BremSampleDirections(double primEkin, double secEkin, double* secDir, double* primDir, orng* rng) { /* :299-322 */
+  const double cost = SampleCostModifiedTsai(primEkin, rng);
+  const double sint = sqrt((1.0 - cost) * (1.0 + cost));
+  const double phi = k2Pi * flat(rng);
+  secDir[0] = sint * cos(phi);
+  secDir[1] = sint * sin(phi);
+  secDir[2] = cost;
+  RotateToReferenceFrame(secDir, primDir);
+  const double primETot = primEkin + kElectronMassC2;
+  const double primPTot = sqrt(primEkin * (primETot + kElectronMassC2));
+  primDir[0] = primPTot * primDir[0] - secEkin * secDir[0];
+  primDir[1] = primPTot * primDir[1] - secEkin * secDir[1];
+  primDir[2] = primPTot * primDir[2] - secEkin * secDir[2];
+  const double norm = 1.0 / sqrt(primDir[0] * primDir[0] + primDir[1] * primDir[1] + primDir[2] * primDir[2]);
+  primDir[0] *= norm;
+  primDir[1] *= norm;
+  primDir[2] *= norm;
+}
+static void BremPerform(otl* tl, const struct G4HepEmData* d, int isEl, int isSB) { /* :35-70 */
+  otrack* t = &tl->el.t;
+  const double primEkin = t->ekin;
+  const double lEkin = GetLogEKin(t);
+  const double gamCut = d->fTheMatCutData->fMatCutData[t->imc].fSecGamProdCutE;
+  if (primEkin <= gamCut) return;
+  const double eGamma = isSB ? SampleETransferSB(d, primEkin, lEkin, t->imc, tl->rng, isEl)
+                             : SampleETransferRB(d, primEkin, lEkin, t->imc, tl->rng, isEl);
+  osec* s = AddSecGm(tl);
+  BremSampleDirections(primEkin, eGamma, s->dir, t->dir, tl->rng);
+  SetEKin(t, primEkin - eGamma);
+  s->ekin = eGamma;
+}
+
+/* ---- G4HepEmPositronInteractionAnnihilation.icc ------------------------------- */
+static void AnnihilateAtRest(otl* tl) { /* :23-49 */
+  const double cost = 2. * flat(tl->rng) - 1.;
+  const double sint = sqrt((1. - cost) * (1. + cost));
+  const double phi = k2Pi * flat(tl->rng);
+  osec* g1 = AddSecGm(tl);
+  osec* g2 = AddSecGm(tl);
+  g1->dir[0] = sint * cos(phi);
+  g1->dir[1] = sint * sin(phi);
+  g1->dir[2] = cost;
+  g2->dir[0] = -g1->dir[0];
+  g2->dir[1] = -g1->dir[1];
+  g2->dir[2] = -g1->dir[2];
+  g1->ekin = kElectronMassC2;
+  g2->ekin = kElectronMassC2;
+}
+static void AnnihilateInFlight(otl* tl) { /* :51-119 */
+  otrack* t = &tl->el.t;
+  const double primEkin = t->ekin;
+  const double* primDir = t->dir;
+  osec* g1 = AddSecGm(tl);
+  osec* g2 = AddSecGm(tl);
+  const double tau = primEkin * kInvElectronMassC2;
+  const double gam = tau + 1.0;
+  const double tau2 = tau + 2.0;
+  const double sqgrate = sqrt(tau / tau2) * 0.5;
+  const double epsmin = 0.5 - sqgrate, epsmax = 0.5 + sqgrate;
+  const double epsqot = epsmax / epsmin;
+  const double tau4 = tau2 * tau2;
+  double eps = 0.0, rfunc = 0.0, r[2];
+  do {
+    flatArray(tl->rng, 2, r);
+    eps = epsmin * HExp(HLog(epsqot) * r[0]);
+    rfunc = 1. - eps + (2. * gam * eps - 1.) / (eps * tau4);
+  } while (rfunc < r[1]);
+  const double sqg2m1 = sqrt(tau * tau2);
+  const double cost = dmin(1., dmax(-1., (eps * tau2 - 1.) / (eps * sqg2m1)));
+  const double sint = sqrt((1. + cost) * (1. - cost));
+  const double phi = k2Pi * flat(tl->rng);
+  const double initEt = primEkin + 2. * kElectronMassC2;
+  const double ekinG1 = eps * initEt;
+  g1->ekin = ekinG1;
+  g1->dir[0] = sint * cos(phi);
+  g1->dir[1] = sint * sin(phi);
+  g1->dir[2] = cost;
+  RotateToReferenceFrame(g1->dir, primDir);
+  g2->ekin = initEt - ekinG1;
+  const double initPt = sqrt(primEkin * (primEkin + 2 * kElectronMassC2));
+  const double px = initPt * primDir[0] - g1->dir[0] * ekinG1;
+  const double py = initPt * primDir[1] - g1->dir[1] * ekinG1;
+  const double pz = initPt * primDir[2] - g1->dir[2] * ekinG1;
+  const double norm = 1.0 / sqrt(px * px + py * py + pz * pz);
+  g2->dir[0] = px * norm;
+  g2->dir[1] = py * norm;
+  g2->dir[2] = pz * norm;
+  SetEKin(t, 0.0);
+}
+
+/* ---- G4HepEmElectronManager.icc: stepping ------------------------------------ */
+static void ConvertTrueToGeometricLength(const struct G4HepEmData* d, omsc* m, double ekin, double range, int imc, int isEl) { /* :602-650 */
+  m->par1 = -1.;
+  m->par2 = 0.;
+  m->par3 = 0.;
+  m->true_step = dmin(m->true_step, range);
+  m->zpath = m->true_step;
+  const double kTlimitMinfix2 = 1.0E-6;
+  if (m->true_step < kTlimitMinfix2) return;
+  const double kTauSmall = 1.0e-16, kDtrl = 0.05;
+  const double tau = m->true_step / m->lambtr1;
+  if (tau < kTauSmall) {
+    m->zpath = dmin(m->true_step, m->lambtr1);
+  } else if (m->true_step < range * kDtrl) {
+    const double kTauLim = 1.0e-6;
+    m->zpath = (tau < kTauLim) ? m->true_step * (1. - 0.5 * tau) : m->lambtr1 * (1. - HExp(-tau));
+  } else if (ekin < kElectronMassC2 || m->true_step == range) {
+    m->par1 = 1. / range;
+    m->par2 = 1. / (m->par1 * m->lambtr1);
+    m->par3 = 1. + m->par2;
+    m->zpath = 1. / (m->par1 * m->par3);
+    if (m->true_step < range) m->zpath *= (1. - HPow(1. - m->true_step / range, m->par3));
+  } else {
+    const double rfin = dmax(range - m->true_step, 0.01 * range);
+    const struct G4HepEmElectronData* e = ElData(d, isEl);
+    const double t1 = GetInvRange(e, imc, rfin);
+    const int imat = d->fTheMatCutData->fMatCutData[imc].fHepEmMatIndex;
+    const double lambda1 = GetTransportMFP(e, imat, t1, HLog(t1));
+    m->par1 = (m->lambtr1 - lambda1) / (m->lambtr1 * m->true_step);
+    m->par2 = 1. / (m->par1 * m->lambtr1);
+    m->par3 = 1. + m->par2;
+    m->zpath = (1. - HPow(lambda1 / m->lambtr1, m->par3)) / (m->par1 * m->par3);
+  }
+  m->zpath = dmin(m->zpath, m->lambtr1);
+}
+static void ConvertGeometricToTrueLength(omsc* m, double range, double gStep) { /* :653-690 */
+  m->zpath = gStep;
+  const double kTLimitMinfix2 = 1.0E-6;
+  if (gStep < kTLimitMinfix2) {
+    m->true_step = gStep;
+    return;
+  }
+  const double kTauSmall = 1.0e-16;
+  double tlength = gStep;
+  if (gStep > m->lambtr1 * kTauSmall) {
+    if (m->par1 < 0.) {
+      tlength = -m->lambtr1 * HLog(1. - gStep / m->lambtr1);
+    } else {
+      const double dum = m->par1 * m->par3 * gStep;
+      tlength = (dum < 1.) ? (1. - HPow(1. - dum, 1. / m->par3)) / m->par1 : range;
+    }
+    if (tlength < gStep) tlength = gStep;
+  }
+  m->true_step = tlength;
+}
+
+static void HowFarToDiscreteInteraction(const struct G4HepEmData* d, const struct G4HepEmParameters* p, oel* el) { /* :48-101 */
+  int winner = -1;
+  otrack* t = &el->t;
+  const double ekin = t->ekin;
+  const double lekin = GetLogEKin(t);
+  const int imc = t->imc;
+  const int isEl = t->charge < 0.0;
+  const struct G4HepEmElectronData* e = ElData(d, isEl);
+  const double range = GetRestRange(e, imc, ekin, lekin);
+  el->range = range;
+  const int ireg = d->fTheMatCutData->fMatCutData[imc].fG4RegionIndex;
+  const double frange = p->fParametersPerRegion[ireg].fFinalRange;
+  const double drange = p->fParametersPerRegion[ireg].fDRoverRange;
+  double pStep = (range > frange) ? range * drange + frange * (1.0 - drange) * (2.0 - frange / range) : range;
+  const int imat = d->fTheMatCutData->fMatCutData[imc].fHepEmMatIndex;
+  double mx[4];
+  mx[0] = GetRestMacXSecForStepping(e, imc, ekin, lekin, 1);
+  mx[1] = GetRestMacXSecForStepping(e, imc, ekin, lekin, 0);
+  mx[2] = isEl ? 0.0 : ComputeMacXsecAnnihilationForStepping(ekin, d->fTheMaterialData->fMaterialData[imat].fElectronDensity);
+  mx[3] = GetMacXSecNuclear(e, imat, ekin, lekin);
+  for (int ip = 0; ip < 4; ++ip) {
+    const double mfp = (mx[ip] > 0.) ? 1. / mx[ip] : kALargeValue;
+    t->mfp[ip] = mfp;
+    const double dStep = mfp * t->nial[ip];
+    if (dStep < pStep) {
+      pStep = dStep;
+      winner = ip;
+    }
+  }
+  el->pstep = pStep;
+  t->winner = winner;
+  t->gstep = pStep;
+}
+
+static void HowFarToMSC(const struct G4HepEmData* d, const struct G4HepEmParameters* p, oel* el, orng* rng) { /* :103-164 */
+  double pStep = el->pstep;
+  const double range = el->range;
+  otrack* t = &el->t;
+  const double ekin = t->ekin;
+  const double lekin = GetLogEKin(t);
+  const int imc = t->imc;
+  const int isEl = t->charge < 0.0;
+  const struct G4HepEmElectronData* e = ElData(d, isEl);
+  const struct G4HepEmMCCData* mc = &d->fTheMatCutData->fMatCutData[imc];
+  const int imat = mc->fHepEmMatIndex, ireg = mc->fG4RegionIndex;
+  omsc* m = &el->msc;
+  m->true_step = pStep;
+  m->zpath = pStep;
+  m->active = 0;
+  m->displ[0] = m->displ[1] = m->displ[2] = 0.;
+  m->dir[0] = 0.; m->dir[1] = 0.; m->dir[2] = 1.;
+  const double kGeomMinLength = 5.E-8;
+  if (pStep > kGeomMinLength && ekin > 1.0E-3) {
+    m->active = 1;
+    m->lambtr1 = GetTransportMFP(e, imat, ekin, lekin);
+    UMSC_StepLimit(d, p, m, ekin, imat, ireg, range, t->safety, t->on_boundary, isEl, rng);
+    ConvertTrueToGeometricLength(d, m, ekin, range, imc, isEl);
+    const double mscTrue = m->true_step;
+    if (mscTrue < pStep) {
+      t->winner = -2;
+      pStep = mscTrue;
+      el->pstep = pStep;
+    }
+    t->gstep = dmin(m->zpath, pStep);
+  }
+}
+
+static void UpdatePStepLength(oel* el) { /* :171-203 */
+  otrack* t = &el->t;
+  const double gStep = t->gstep;
+  double pStep = gStep;
+  const double range = el->range;
+  omsc* m = &el->msc;
+  if (m->active) {
+    pStep = m->true_step;
+    if (gStep < m->zpath) {
+      ConvertGeometricToTrueLength(m, range, gStep);
+      pStep = dmin(pStep, m->true_step);
+      m->true_step = pStep;
+    }
+    const double kGeomMinLength = 5.E-8;
+    if (pStep <= kGeomMinLength || range <= pStep) m->active = 0;
+  }
+  el->pstep = pStep;
+}
+
+static void UpdateNumIALeft(oel* el) { /* :205-214 */
+  const double pStep = el->pstep;
+  otrack* t = &el->t;
+  t->nial[0] -= pStep / t->mfp[0];
+  t->nial[1] -= pStep / t->mfp[1];
+  t->nial[2] -= pStep / t->mfp[2];
+  t->nial[3] -= pStep / t->mfp[3];
+}
+
+static int ApplyMeanEnergyLoss(const struct G4HepEmData* d, const struct G4HepEmParameters* p, oel* el) { /* :216-259 */
+  const double pStep = el->pstep;
+  otrack* t = &el->t;
+  const int isEl = t->charge < 0.0;
+  const double ekin = t->ekin;
+  const double range = el->range;
+  if (pStep >= range || ekin <= p->fMinLossTableEnergy) {
+    t->edep = ekin;
+    SetEKin(t, 0.0);
+    return 1;
+  }
+  const struct G4HepEmElectronData* e = ElData(d, isEl);
+  const int imc = t->imc;
+  const double lekin = GetLogEKin(t);
+  double eloss = pStep * GetRestDEDX(e, imc, ekin, lekin);
+  const int ireg = d->fTheMatCutData->fMatCutData[imc].fG4RegionIndex;
+  const double linLim = p->fParametersPerRegion[ireg].fLinELossLimit;
+  if (eloss > ekin * linLim) {
+    const double postRange = range - pStep;
+    eloss = ekin - GetInvRange(e, imc, postRange);
+  }
+  eloss = dmax(eloss, 0.0);
+  if (eloss >= ekin) {
+    eloss = ekin;
+    SetEKin(t, 0);
+    t->edep = eloss;
+    return 1;
+  }
+  SetEKin(t, ekin - eloss);
+  t->edep = eloss;
+  return 0;
+}
+
+static void SampleMSC(const struct G4HepEmData* d, const struct G4HepEmParameters* p, oel* el, orng* rng) { /* :261-322 */
+  const double pStep = el->pstep;
+  otrack* t = &el->t;
+  const int isEl = t->charge < 0.0;
+  const int imc = t->imc;
+  const double preEkin = el->pre_ekin;
+  const double range = el->range;
+  const struct G4HepEmElectronData* e = ElData(d, isEl);
+  const double kTLimitMinfix = 1.0E-8, kTauSmall = 1.0e-16;
+  omsc* m = &el->msc;
+  if (m->active && (pStep > dmax(kTLimitMinfix, kTauSmall * m->lambtr1))) {
+    double postEkin = preEkin;
+    double postLEkin = el->pre_lekin;
+    if (pStep > range * 0.01) {  /* G4VERSION_NUM >= 1100 */
+      postEkin = t->ekin;
+      postLEkin = GetLogEKin(t);
+    }
+    const int imat = d->fTheMatCutData->fMatCutData[imc].fHepEmMatIndex;
+    const double postTr1mfp = GetTransportMFP(e, imat, postEkin, postLEkin);
+    UMSC_SampleScattering(d, m, pStep, preEkin, m->lambtr1, postEkin, postTr1mfp, imat, isEl, p->fIsMSCPositronCor,
+                          p->fIsMSCDisplacement, rng);
+    if (!m->no_scat) {
+      RotateToReferenceFrame(m->dir, t->dir);
+      if (m->is_displace) RotateToReferenceFrame(m->displ, t->dir);
+      t->dir[0] = m->dir[0];
+      t->dir[1] = m->dir[1];
+      t->dir[2] = m->dir[2];
+    }
+  }
+}
+
+static int SampleLossFluctuations(const struct G4HepEmData* d, const struct G4HepEmParameters* p, oel* el, orng* rng) { /* :324-368 */
+  const double pStep = el->pstep;
+  otrack* t = &el->t;
+  const int isEl = t->charge < 0.0;
+  const int imc = t->imc;
+  const double preEkin = el->pre_ekin;
+  double finalEkin = t->ekin;
+  double eloss = t->edep;
+  (void)pStep;
+  const int ireg = d->fTheMatCutData->fMatCutData[imc].fG4RegionIndex;
+  const int isFluct = p->fParametersPerRegion[ireg].fIsELossFluctuation;
+  const double kFluctParMinEnergy = 1.E-5;
+  if (isFluct && eloss > kFluctParMinEnergy) {
+    const struct G4HepEmMCCData* mc = &d->fTheMatCutData->fMatCutData[imc];
+    const double elCut = mc->fSecElProdCutE;
+    const double meanExE = d->fTheMaterialData->fMaterialData[mc->fHepEmMatIndex].fMeanExEnergy;
+    const double tmax = isEl ? 0.5 * preEkin : preEkin;
+    const double tcut = dmin(elCut, tmax);
+    eloss = SampleEnergyLossFluctuation(tcut, meanExE, eloss, rng);
+    eloss = dmax(eloss, 0.0);
+    finalEkin = preEkin - eloss;
+  }
+  if (finalEkin <= p->fElectronTrackingCut) {
+    eloss = preEkin;
+    finalEkin = 0.0;
+    SetEKin(t, finalEkin);
+    t->edep = eloss;
+    return 1;
+  }
+  SetEKin(t, finalEkin);
+  t->edep = eloss;
+  return 0;
+}
+
+static int PerformContinuous(const struct G4HepEmData* d, const struct G4HepEmParameters* p, oel* el, orng* rng) { /* :375-405 */
+  el->pre_ekin = el->t.ekin;
+  el->pre_lekin = GetLogEKin(&el->t);
+  UpdatePStepLength(el);
+  const double pStep = el->pstep;
+  if (pStep <= 0.0) return 0;
+  UpdateNumIALeft(el);
+  if (ApplyMeanEnergyLoss(d, p, el)) return 1;
+  SampleMSC(d, p, el, rng);
+  return SampleLossFluctuations(d, p, el, rng);
+}
+
+static int CheckDelta(const struct G4HepEmData* d, otrack* t, double rand) { /* :408-423 */
+  const int isEl = t->charge < 0.0;
+  const struct G4HepEmElectronData* e = ElData(d, isEl);
+  const int iDProc = t->winner;
+  const int imc = t->imc;
+  const int imat = d->fTheMatCutData->fMatCutData[imc].fHepEmMatIndex;
+  const double ekin = t->ekin;
+  const double lekin = GetLogEKin(t);
+  const double mxsec = iDProc < 2 ? GetRestMacXSec(e, imc, ekin, lekin, iDProc == 0)
+                     : (iDProc < 3 ? ComputeMacXsecAnnihilation(ekin, d->fTheMaterialData->fMaterialData[imat].fElectronDensity)
+                                   : GetMacXSecNuclear(e, imat, ekin, lekin));
+  return mxsec <= 0.0 || rand > mxsec * t->mfp[iDProc];
+}
+
+static void PerformDiscrete(const struct G4HepEmData* d, const struct G4HepEmParameters* p, otl* tl) { /* :425-459 */
+  otrack* t = &tl->el.t;
+  const int isEl = t->charge < 0.0;
+  const int iDProc = t->winner;
+  if (iDProc < 0 || t->on_boundary) return;
+  t->nial[iDProc] = -1.0;
+  if (CheckDelta(d, t, flat(tl->rng))) return;
+  const double ekin = t->ekin;
+  switch (iDProc) {
+    case 0: IoniPerform(tl, d, isEl); break;
+    case 1: BremPerform(tl, d, isEl, ekin < p->fElectronBremModelLim); break;
+    case 2: AnnihilateInFlight(tl); break;
+    default: break;
+  }
+}
+
+static void ElectronHowFar(const struct G4HepEmData* d, const struct G4HepEmParameters* p, otl* tl) { /* :35-45,166-169 */
+  otrack* t = &tl->el.t;
+  for (int ip = 0; ip < 4; ++ip)
+    if (t->nial[ip] <= 0.) t->nial[ip] = -HLog(flat(tl->rng));
+  HowFarToDiscreteInteraction(d, p, &tl->el);
+  HowFarToMSC(d, p, &tl->el, tl->rng);
+}
+
+static int ElectronPerform(const struct G4HepEmData* d, const struct G4HepEmParameters* p, otl* tl) { /* :461-483 */
+  otrack* t = &tl->el.t;
+  t->edep = 0;
+  tl->el.pstep = t->gstep;
+  const int isEl = t->charge < 0.0;
+  if (t->gstep <= 0.) return 0;
+  const int stopped = PerformContinuous(d, p, &tl->el, tl->rng);
+  if (stopped) {
+    if (!isEl) AnnihilateAtRest(tl);
+    return 1;
+  }
+  PerformDiscrete(d, p, tl);
+  return 0;
+}
+
+/* ---- G4HepEmGammaManager.icc --------------------------------------------------- */
+static double GetMacXSecPE(const struct G4HepEmData* d, int imat, double ekin) { /* :155-170 */
+  const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[imat];
+  int interval = 0;
+  if (ekin >= md->fSandiaEnergies[0]) {
+    for (int i = md->fNumOfSandiaIntervals - 1; i >= 0; i--) {
+      if (ekin >= md->fSandiaEnergies[i]) { interval = i; break; }
+    }
+  }
+  const double* c = &md->fSandiaCoefficients[4 * interval];
+  const double inv = 1 / ekin;
+  return inv * (c[0] + inv * (c[1] + inv * (c[2] + inv * c[3])));
+}
+static double GetTotalMacXSec(const struct G4HepEmData* d, ogm* g) { /* :108-153 */
+  otrack* t = &g->t;
+  const double ekin = t->ekin;
+  const double lekin = GetLogEKin(t);
+  const int imat = d->fTheMatCutData->fMatCutData[t->imc].fHepEmMatIndex;
+  const struct G4HepEmGammaData* gd = d->fTheGammaData;
+  if (ekin > gd->fEMax1) {
+    const int istart = imat * gd->fDataPerMat + gd->fNumData0 + gd->fNumData1;
+    return GetSplineLog4(gd->fEGridSize2, &gd->fMacXsecData[istart], ekin, lekin, gd->fLogEMin2, gd->fEILDelta2, 1);
+  }
+  if (ekin > gd->fEMax0) {
+    const int istart = imat * gd->fDataPerMat + gd->fNumData0;
+    double mx[2];
+    GetLinearLog2All(gd->fEGridSize1, &gd->fMacXsecData[istart], ekin, lekin, gd->fLogEMin1, gd->fEILDelta1, mx);
+    g->pe_mxsec = mx[1];
+    return mx[0];
+  }
+  const int istart = imat * gd->fDataPerMat;
+  const double comp = GetLinearLog(gd->fEGridSize0, &gd->fMacXsecData[istart], ekin, lekin, gd->fLogEMin0, gd->fEILDelta0);
+  const double pe = dmax(0.0, GetMacXSecPE(d, imat, ekin));
+  g->pe_mxsec = pe;
+  return comp + pe;
+}
+static void SampleInteraction(const struct G4HepEmData* d, ogm* g, double urnd) { /* :187-219 */
+  otrack* t = &g->t;
+  const double ekin = t->ekin;
+  const double tmfp = t->mfp[0];
+  t->nial[0] = -1.0;
+  const struct G4HepEmGammaData* gd = d->fTheGammaData;
+  if (ekin > gd->fEMax1) {
+    const double lekin = GetLogEKin(t);
+    const int imat = d->fTheMatCutData->fMatCutData[t->imc].fHepEmMatIndex;
+    const int istart = imat * gd->fDataPerMat + gd->fNumData0 + gd->fNumData1;
+    double mxSec = 0.0, cProb = 0.0;
+    int pid = 2;
+    do {
+      if (pid == 5) { ++pid; break; }  /* gamma-nuclear bin: value unused (see header) */
+      mxSec = GetSplineLog4(gd->fEGridSize2, &gd->fMacXsecData[istart], ekin, lekin, gd->fLogEMin2, gd->fEILDelta2, pid);
+      cProb += mxSec * tmfp;
+      ++pid;
+    } while (pid < 6 && urnd > cProb);
+    t->winner = pid - 3;
+    g->pe_mxsec = mxSec;
+    return;
+  }
+  const double mxPE = g->pe_mxsec;
+  t->winner = (urnd > tmfp * mxPE) ? 1 : 2;
+}
+
+/* ---- gamma interactions ------------------------------------------------------- */
+static double ComptonSample(double primE, double* primDir, const double* orgDir, orng* rng) { /* Compton.icc:67-103 */
+  const double kappa = primE * kInvElectronMassC2;
+  const double eps0 = 1. / (1. + 2. * kappa);
+  const double eps02 = eps0 * eps0;
+  const double al1 = -HLog(eps0);
+  const double al2 = al1 + 0.5 * (1. - eps02);
+  double eps, eps2, gf, oneMinusCost, sint2, r[3];
+  do {
+    flatArray(rng, 3, r);
+    if (al1 > al2 * r[0]) {
+      eps = HExp(-al1 * r[1]);
+      eps2 = eps * eps;
+    } else {
+      eps2 = eps02 + (1. - eps02) * r[1];
+      eps = sqrt(eps2);
+    }
+    oneMinusCost = (1. - eps) / (eps * kappa);
+    sint2 = oneMinusCost * (2. - oneMinusCost);
+    gf = 1. - eps * sint2 / (1. + eps2);
+  } while (gf < r[2]);
+  const double cost = 1.0 - oneMinusCost;
+  const double sint = sqrt(dmax(0., sint2));
+  const double phi = k2Pi * flat(rng);
+  primDir[0] = sint * cos(phi);
+  primDir[1] = sint * sin(phi);
+  primDir[2] = cost;
+  RotateToReferenceFrame(primDir, orgDir);
+  return primE * eps;
+}
+static void ComptonPerform(otl* tl) { /* Compton.icc:17-65 */
+  otrack* t = &tl->gm.t;
+  const double primE = t->ekin;
+  const double thr = 0.0001;
+  if (primE < thr) return;
+  const double org[3] = {t->dir[0], t->dir[1], t->dir[2]};
+  const double postE = ComptonSample(primE, t->dir, org, tl->rng);
+  const double secE = primE - postE;
+  double edep = 0.0;
+  if (secE > thr) {
+    osec* s = AddSecEl(tl, HEPEM_ELECTRON);
+    s->dir[0] = primE * org[0] - postE * t->dir[0];
+    s->dir[1] = primE * org[1] - postE * t->dir[1];
+    s->dir[2] = primE * org[2] - postE * t->dir[2];
+    const double norm = 1.0 / sqrt(s->dir[0] * s->dir[0] + s->dir[1] * s->dir[1] + s->dir[2] * s->dir[2]);
+    s->dir[0] *= norm;
+    s->dir[1] *= norm;
+    s->dir[2] *= norm;
+    s->ekin = secE;
+  } else {
+    edep += secE;
+  }
+  if (postE > thr) {
+    SetEKin(t, postE);
+  } else {
+    edep += postE;
+    SetEKin(t, 0.0);
+  }
+  t->edep = edep;
+}
+
+static int ConvSelectTargetAtom(const struct G4HepEmGammaData* gd, int imat, double ekin, double lekin, double urndn) { /* Conversion.icc:151-176 */
+  const int start = gd->fElemSelectorConvStartIndexPerMat[imat];
+  const double* data = &gd->fElemSelectorConvData[start];
+  const int numData = gd->fElemSelectorConvEgridSize;
+  const int numElem = (int)data[0];
+  const double logE0 = gd->fElemSelectorConvLogMinEkin, invLD = gd->fElemSelectorConvEILDelta;
+  const double* xdata = gd->fElemSelectorConvEgrid;
+  const double xv = dmax(xdata[0], dmin(xdata[numData - 1], ekin));
+  const int idx = (int)dmax(0.0, dmin((lekin - logE0) * invLD, numData - 2.0));
+  const double x1 = xdata[idx], x2 = xdata[idx + 1];
+  const double dl = x2 - x1;
+  const double b = dmax(0., dmin(1., (xv - x1) / dl));
+  const int i0 = idx * (numElem - 1) + 1;
+  const int i1 = i0 + (numElem - 1);
+  int ie = 0;
+  while (ie < numElem - 1 && urndn > data[i0 + ie] + b * (data[i1 + ie] - data[i0 + ie])) ++ie;
+  return ie;
+}
+static void ConvScreen12(double delta, double* f1, double* f2) { /* Conversion.icc:266-274 */
+  if (delta > 1.4) {
+    *f1 = 42.038 - 8.29 * HLog(delta + 0.958);
+    *f2 = *f1;
+  } else {
+    *f1 = 42.184 - delta * (7.444 - 1.623 * delta);
+    *f2 = 41.326 - delta * (5.848 - 0.902 * delta);
+  }
+}
+static double ConvScreen1(double delta) { return (delta > 1.4) ? 42.038 - 8.29 * HLog(delta + 0.958) : 42.184 - delta * (7.444 - 1.623 * delta); }
+static double ConvScreen2(double delta) { return (delta > 1.4) ? 42.038 - 8.29 * HLog(delta + 0.958) : 41.326 - delta * (5.848 - 0.902 * delta); }
+static void ConvPhi12(double delta, double* phi1, double* phi2) { /* Conversion.icc:240-248 */
+  if (delta > 1.4) {
+    *phi1 = 21.0190 - 4.145 * HLog(delta + 0.958);
+    *phi2 = *phi1;
+  } else {
+    *phi1 = 20.806 - delta * (3.190 - 0.5710 * delta);
+    *phi2 = 20.234 - delta * (2.126 - 0.0903 * delta);
+  }
+}
+static double ConvRateNoLPM(double normCond, double epsMin, double epsRange, double deltaFactor, double invF10, double invF20,
+                            double fz, orng* rng) { /* Conversion.icc:179-200 */
+  double r[3], greject = 0., eps = 0.;
+  do {
+    flatArray(rng, 3, r);
+    if (normCond > r[0]) {
+      eps = 0.5 - epsRange * pow(r[1], 1. / 3.);
+      const double delta = deltaFactor / (eps * (1. - eps));
+      greject = (ConvScreen1(delta) - fz) * invF10;
+    } else {
+      eps = epsMin + epsRange * r[1];
+      const double delta = deltaFactor / (eps * (1. - eps));
+      greject = (ConvScreen2(delta) - fz) * invF20;
+    }
+  } while (greject < r[2]);
+  return eps;
+}
+static double ConvRateWithLPM(double normCond, double epsMin, double epsRange, double deltaFactor, double invF10, double invF20,
+                              double fz, orng* rng, double eGamma, double lpmEnergy, const struct G4HepEmElemData* ed) { /* :203-237 */
+  const double z23 = ed->fZet23, ilVarS1 = ed->fILVarS1, ilVarS1Cond = ed->fILVarS1Cond;
+  double r[3], greject = 0., eps = 0.;
+  do {
+    flatArray(rng, 3, r);
+    double fXi, fG, fPhi, phi1, phi2;
+    if (normCond > r[0]) {
+      eps = 0.5 - epsRange * pow(r[1], 1. / 3.);
+      const double delta = deltaFactor / (eps * (1. - eps));
+      ConvPhi12(delta, &phi1, &phi2);
+      EvaluateLPMFunctions(&fXi, &fG, &fPhi, eGamma, eps * eGamma, lpmEnergy, z23, ilVarS1, ilVarS1Cond, 0.0, -1.0);
+      greject = fXi * ((2. * fPhi + fG) * phi1 - fG * phi2 - fPhi * fz) * invF10;
+    } else {
+      eps = epsMin + epsRange * r[1];
+      const double delta = deltaFactor / (eps * (1. - eps));
+      ConvPhi12(delta, &phi1, &phi2);
+      EvaluateLPMFunctions(&fXi, &fG, &fPhi, eGamma, eps * eGamma, lpmEnergy, z23, ilVarS1, ilVarS1Cond, 0.0, -1.0);
+      greject = fXi * ((fPhi + 0.5 * fG) * phi1 + 0.5 * fG * phi2 - 0.5 * (fG + fPhi) * fz) * invF20;
+    }
+  } while (greject < r[2]);
+  return eps;
+}
+static void ConvPerform(otl* tl, const struct G4HepEmData* d) { /* Conversion.icc:19-53,56-146 */
+  otrack* t = &tl->gm.t;
+  const double primE = t->ekin;
+  if (primE < 2. * kElectronMassC2) return;
+  const double lPrimE = GetLogEKin(t);
+  const int imc = t->imc;
+  orng* rng = tl->rng;
+  /* SampleKinEnergies */
+  const int imat = d->fTheMatCutData->fMatCutData[imc].fHepEmMatIndex;
+  const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[imat];
+  const int elemIndx = (md->fNumOfElement > 1) ? ConvSelectTargetAtom(d->fTheGammaData, imat, primE, lPrimE, flat(rng)) : 0;
+  const int iZet = md->fElementVect[elemIndx];
+  const double lpmEnr = kLPMconstant * md->fRadiationLength;
+  const int zcl = iZet < d->fTheElementData->fMaxZet ? iZet : d->fTheElementData->fMaxZet;
+  const struct G4HepEmElemData* ed = &d->fTheElementData->fElementData[zcl];
+  const double eps0 = kElectronMassC2 / primE;
+  double eps = 0.0;
+  if (primE < 2.0) {
+    eps = eps0 + (0.5 - eps0) * flat(rng);
+  } else {
+    const double deltaFactor = eps0 * 136. / ed->fZet13;
+    const double deltaMin = 4. * deltaFactor;
+    const double deltaMax = (primE < 50.0) ? ed->fDeltaMaxLow : ed->fDeltaMaxHigh;
+    const double logZ13 = 0.333333 * ed->fLogZ;
+    const double FZ = (primE < 50.0) ? 8. * logZ13 : 8. * (logZ13 + ed->fCoulomb);
+    const double epsp = 0.5 - 0.5 * sqrt(1. - deltaMin / deltaMax);
+    const double epsMin = dmax(eps0, epsp);
+    const double epsRange = 0.5 - epsMin;
+    double F10, F20;
+    ConvScreen12(deltaMin, &F10, &F20);
+    F10 -= FZ;
+    F20 -= FZ;
+    const double NormF1 = dmax(F10 * epsRange * epsRange, 0.);
+    const double NormF2 = dmax(1.5 * F20, 0.);
+    const double NormCond = NormF1 / (NormF1 + NormF2);
+    eps = (primE < 100000.0) ? ConvRateNoLPM(NormCond, epsMin, epsRange, deltaFactor, 1. / F10, 1. / F20, FZ, rng)
+                             : ConvRateWithLPM(NormCond, epsMin, epsRange, deltaFactor, 1. / F10, 1. / F20, FZ, rng, primE, lpmEnr, ed);
+  }
+  double eTot, pTot;
+  if (flat(rng) > 0.5) {
+    eTot = (1. - eps) * primE;
+    pTot = eps * primE;
+  } else {
+    pTot = (1. - eps) * primE;
+    eTot = eps * primE;
+  }
+  const double eKin = dmax(0., eTot - kElectronMassC2);
+  const double pKin = dmax(0., pTot - kElectronMassC2);
+  osec* se = AddSecEl(tl, HEPEM_ELECTRON);
+  osec* sp = AddSecEl(tl, HEPEM_POSITRON);
+  /* SampleDirections: Conversion.icc:123-146 */
+  const double phi = k2Pi * flat(rng);
+  const double cosPhi = cos(phi), sinPhi = sin(phi);
+  const double costEl = SampleCostModifiedTsai(eKin, rng);
+  const double sintEl = sqrt((1.0 - costEl) * (1.0 + costEl));
+  se->dir[0] = sintEl * cosPhi;
+  se->dir[1] = sintEl * sinPhi;
+  se->dir[2] = costEl;
+  RotateToReferenceFrame(se->dir, t->dir);
+  const double costPos = SampleCostModifiedTsai(pKin, rng);
+  const double sintPos = sqrt((1.0 - costPos) * (1.0 + costPos));
+  sp->dir[0] = -sintPos * cosPhi;
+  sp->dir[1] = -sintPos * sinPhi;
+  sp->dir[2] = costPos;
+  RotateToReferenceFrame(sp->dir, t->dir);
+  se->ekin = eKin;
+  sp->ekin = pKin;
+  SetEKin(t, 0.0);
+}
+
+static double PESelectElementBindingEnergy(const struct G4HepEmData* d, int imc, double mxsec, double ekin, orng* rng) { /* Photoelectric.icc:39-78 */
+  const int imat = d->fTheMatCutData->fMatCutData[imc].fHepEmMatIndex;
+  const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[imat];
+  int ielem = 0;
+  if (md->fNumOfElement > 1) {
+    const double x = flat(rng) * mxsec;
+    double sum = 0;
+    const double invE = 1 / ekin;
+    for (int i = 0; i < md->fNumOfElement; i++) {
+      const struct G4HepEmElemData* ed = &d->fTheElementData->fElementData[md->fElementVect[i]];
+      int interval = 0;
+      if (ekin >= ed->fSandiaEnergies[0]) {
+        for (int j = ed->fNumOfSandiaIntervals - 1; j >= 0; j--) {
+          if (ekin >= ed->fSandiaEnergies[j]) { interval = j; break; }
+        }
+      }
+      const double* c = &ed->fSandiaCoefficients[4 * interval];
+      sum += md->fNumOfAtomsPerVolumeVect[i] * invE * (c[0] + invE * (c[1] + invE * (c[2] + invE * c[3])));
+      if (x <= sum) { ielem = i; break; }
+    }
+  }
+  return d->fTheElementData->fElementData[md->fElementVect[ielem]].fKShellBindingEnergy;
+}
+static void PEPerform(otl* tl, const struct G4HepEmData* d) { /* Photoelectric.icc:13-37,80-121 */
+  otrack* t = &tl->gm.t;
+  const double gE = t->ekin;
+  const double be = PESelectElementBindingEnergy(d, t->imc, tl->gm.pe_mxsec, gE, tl->rng);
+  const double thr = 0.000001;
+  const double peE = gE - be;
+  if (peE > thr) {
+    osec* s = AddSecEl(tl, HEPEM_ELECTRON);
+    const double tau = peE * kInvElectronMassC2;
+    const double gamma = 1.0 + tau;
+    const double beta = sqrt(tau * (tau + 2.0)) / gamma;
+    const double ac = (1.0 - beta) / beta;
+    const double a1 = 0.5 * beta * gamma * tau * (gamma - 2.0);
+    const double a2 = ac + 2.0;
+    const double gtmax = 2.0 * (a1 + 1.0 / ac);
+    double tsam = 0.0, gtr = 0.0, r[2];
+    do {
+      flatArray(tl->rng, 2, r);
+      tsam = 2.0 * ac * (2.0 * r[0] + a2 * sqrt(r[0])) / (a2 * a2 - 4.0 * r[0]);
+      gtr = (2.0 - tsam) * (a1 + 1.0 / (ac + tsam));
+    } while (r[1] * gtmax > gtr);
+    const double costheta = 1.0 - tsam;
+    const double sint = sqrt(tsam * (2.0 - tsam));
+    const double phi = k2Pi * flat(tl->rng);
+    s->dir[0] = sint * cos(phi);
+    s->dir[1] = sint * sin(phi);
+    s->dir[2] = costheta;
+    RotateToReferenceFrame(s->dir, t->dir);
+    s->ekin = peE;
+    t->edep = be;
+  } else {
+    t->edep = gE;
+  }
+  SetEKin(t, 0.0);
+}
+
+static void GammaHowFar(const struct G4HepEmData* d, otl* tl) { /* GammaManager.icc:27-48 */
+  otrack* t = &tl->gm.t;
+  if (t->nial[0] <= 0.0) t->nial[0] = -HLog(flat(tl->rng));
+  const double tot = GetTotalMacXSec(d, &tl->gm);
+  const double mfp = (tot > 0.) ? 1. / tot : kALargeValue;
+  t->mfp[0] = mfp;
+  t->gstep = mfp * t->nial[0];
+}
+static void GammaPerform(const struct G4HepEmData* d, const struct G4HepEmParameters* p, otl* tl) { /* GammaManager.icc:54-105 */
+  otrack* t = &tl->gm.t;
+  t->nial[0] -= t->gstep / t->mfp[0];
+  t->edep = 0.0;
+  if (t->on_boundary) return;
+  const int iDProc = t->winner;
+  t->nial[iDProc] = -1.0;
+  switch (iDProc) {
+    case 0: ConvPerform(tl, d); break;
+    case 1: ComptonPerform(tl); break;
+    case 2: PEPerform(tl, d); break;
+    default: break;
+  }
+  const double finalEkin = t->ekin;
+  if (finalEkin > 0.0 && finalEkin <= p->fGammaTrackingCut) {
+    SetEKin(t, 0.0);
+    t->edep += finalEkin;
+  }
+}
+
+/* ---- drivers ------------------------------------------------------------------------ */
+int oracle_lookup_eloss_xsec(const struct G4HepEmData* d, int ptype, int n, const int32_t* imc, const double* ekin,
+                             const double* lekin, double* out) {
+  if (!d || n < 0 || (ptype != 0 && ptype != 1)) return -1;
+  const struct G4HepEmElectronData* e = ElData(d, ptype == 0);
+  for (int i = 0; i < n; ++i) {
+    const double r = GetRestRange(e, imc[i], ekin[i], lekin[i]);
+    out[i] = GetRestDEDX(e, imc[i], ekin[i], lekin[i]);
+    out[n + i] = r;
+    out[2 * n + i] = GetInvRange(e, imc[i], r);
+    out[3 * n + i] = GetRestMacXSec(e, imc[i], ekin[i], lekin[i], 1);
+    out[4 * n + i] = GetRestMacXSec(e, imc[i], ekin[i], lekin[i], 0);
+  }
+  return 0;
+}
+
+int oracle_probe_electron(const struct G4HepEmData* d, int ptype, int which, int n, const int32_t* imc, const double* ekin,
+                          const double* lekin, double* out) {
+  const struct G4HepEmElectronData* e = ElData(d, ptype == 0);
+  for (int i = 0; i < n; ++i) {
+    const int imat = d->fTheMatCutData->fMatCutData[imc[i]].fHepEmMatIndex;
+    const double ne = d->fTheMaterialData->fMaterialData[imat].fElectronDensity;
+    switch (which) {
+      case 0: out[i] = GetRestMacXSecForStepping(e, imc[i], ekin[i], lekin[i], 1); break;
+      case 1: out[i] = GetRestMacXSecForStepping(e, imc[i], ekin[i], lekin[i], 0); break;
+      case 2: out[i] = GetTransportMFP(e, imat, ekin[i], lekin[i]); break;
+      case 3: out[i] = GetMacXSecNuclear(e, imat, ekin[i], lekin[i]); break;
+      case 4: out[i] = ComputeMacXsecAnnihilation(ekin[i], ne); break;
+      case 5: out[i] = ComputeMacXsecAnnihilationForStepping(ekin[i], ne); break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+
+int oracle_probe_selector(const struct G4HepEmData* d, int ptype, int which, int n, const int32_t* imc, const double* ekin,
+                          const double* lekin, const double* urnd, int32_t* out) {
+  const struct G4HepEmElectronData* e = ElData(d, ptype == 0);
+  for (int i = 0; i < n; ++i) {
+    const int imat = d->fTheMatCutData->fMatCutData[imc[i]].fHepEmMatIndex;
+    if (d->fTheMaterialData->fMaterialData[imat].fNumOfElement < 2) { out[i] = 0; continue; }
+    switch (which) {
+      case 0: out[i] = BremSelectTargetAtom(e, imc[i], ekin[i], lekin[i], urnd[i], 1); break;
+      case 1: out[i] = BremSelectTargetAtom(e, imc[i], ekin[i], lekin[i], urnd[i], 0); break;
+      case 2: out[i] = ConvSelectTargetAtom(d->fTheGammaData, imat, ekin[i], lekin[i], urnd[i]); break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+
+int oracle_probe_gamma(const struct G4HepEmData* d, int n, const int32_t* imc, const double* ekin, const double* lekin,
+                       const double* urnd, double* out_xs, double* out_pe, int32_t* out_pid) {
+  for (int i = 0; i < n; ++i) {
+    ogm g;
+    TrackReset(&g.t);
+    g.pe_mxsec = 0.0;
+    g.t.ekin = ekin[i];
+    g.t.lekin = lekin[i];
+    g.t.imc = imc[i];
+    const double tot = GetTotalMacXSec(d, &g);
+    g.t.mfp[0] = (tot > 0.) ? 1. / tot : kALargeValue;
+    out_xs[i] = tot;
+    SampleInteraction(d, &g, urnd[i]);
+    out_pe[i] = g.pe_mxsec;
+    out_pid[i] = g.t.winner;
+  }
+  return 0;
+}
+
+typedef struct step_job {
+  const struct G4HepEmData* d;
+  const struct G4HepEmParameters* p;
+  uint64_t seed;
+  const hepem_tracks* in;
+  hepem_tracks* out;
+  hepem_step_out* res;
+  int begin, end;
+} step_job;
+
+static void step_one(const step_job* J, int i) {
+  const struct G4HepEmData* d = J->d;
+  const struct G4HepEmParameters* p = J->p;
+  const hepem_tracks* in = J->in;
+  hepem_tracks* out = J->out;
+  hepem_step_out* res = J->res;
+  const int n = in->n;
+  /* tracks with an invalid couple index or a non-positive/non-finite energy
+     are passed through unchanged, flagged with winner -99 (library contract) */
+  {
+    const int imc = in->imc[i];
+    const double e = in->ekin[i];
+    if (!(imc >= 0 && imc < d->fTheMatCutData->fNumMatCutData && e > 0.0 && e < 1.0e300)) {
+      res->true_step[i] = res->geom_step[i] = res->edep[i] = 0.0;
+      for (int c = 0; c < 3; ++c) res->displacement[c * n + i] = 0.0;
+      res->winner[i] = -99;
+      res->stopped[i] = 1;
+      res->nsec[i] = 0;
+      for (int s = 0; s < 2; ++s) {
+        res->sec_ptype[s * n + i] = -1;
+        res->sec_ekin[s * n + i] = 0.0;
+        for (int c = 0; c < 3; ++c) res->sec_dir[(c * 2 + s) * n + i] = 0.0;
+      }
+      if (out != in) {
+        out->ptype[i] = in->ptype[i]; out->id[i] = in->id[i]; out->imc[i] = imc; out->ekin[i] = e;
+        for (int c = 0; c < 3; ++c) out->dir[c * n + i] = in->dir[c * n + i];
+        out->safety[i] = in->safety[i]; out->on_boundary[i] = in->on_boundary[i];
+        for (int k = 0; k < 4; ++k) out->num_ia_left[k * n + i] = in->num_ia_left[k * n + i];
+        out->rng_ctr[i] = in->rng_ctr[i]; out->rng_gauss[i] = in->rng_gauss[i]; out->rng_has_gauss[i] = in->rng_has_gauss[i];
+        if (in->ptype[i] != HEPEM_GAMMA) {
+          out->msc_init_range[i] = in->msc_init_range[i]; out->msc_dyn_range_factor[i] = in->msc_dyn_range_factor[i];
+          out->msc_tlimit_min[i] = in->msc_tlimit_min[i]; out->msc_first_step[i] = in->msc_first_step[i];
+        }
+      }
+      return;
+    }
+  }
+  orng rng;
+  rng.key[0] = (uint32_t)J->seed;
+  rng.key[1] = (uint32_t)(J->seed >> 32);
+  rng.id = (uint32_t)in->id[i];
+  rng.ctr = in->rng_ctr[i];
+  rng.has_gauss = in->rng_has_gauss[i];
+  rng.gauss = in->rng_gauss[i];
+  otl tl;
+  memset(&tl, 0, sizeof(tl));
+  tl.rng = &rng;
+  const int ptype = in->ptype[i];
+  otrack* t;
+  double displ[3] = {0., 0., 0.};
+  double trueStep, geomStep;
+  if (ptype == HEPEM_GAMMA) {
+    t = &tl.gm.t;
+    TrackReset(t);
+    tl.gm.pe_mxsec = 0.0;
+  } else {
+    t = &tl.el.t;
+    TrackReset(t);
+    MscReset(&tl.el.msc);
+    tl.el.range = 0.0;
+    tl.el.pstep = 0.0;
+    t->charge = ptype == HEPEM_ELECTRON ? -1.0 : 1.0;
+    tl.el.msc.init_range = in->msc_init_range[i];
+    tl.el.msc.dyn_range_factor = in->msc_dyn_range_factor[i];
+    tl.el.msc.tlimit_min = in->msc_tlimit_min[i];
+    tl.el.msc.first_step = in->msc_first_step[i];
+  }
+  SetEKin(t, in->ekin[i]);
+  t->dir[0] = in->dir[i];
+  t->dir[1] = in->dir[n + i];
+  t->dir[2] = in->dir[2 * n + i];
+  t->imc = in->imc[i];
+  t->safety = in->safety[i];
+  t->on_boundary = in->on_boundary[i];
+  for (int k = 0; k < 4; ++k) t->nial[k] = in->num_ia_left[k * n + i];
+  int stopped;
+  if (ptype == HEPEM_GAMMA) {
+    /* G4HepEmTrackingManager.cc:999-1000,1092-1108: HowFar, (no geometry),
+       then UpdateNumIALeft on a boundary or SelectInteraction + Perform */
+    GammaHowFar(d, &tl);
+    if (t->on_boundary) {
+      t->nial[0] -= t->gstep / t->mfp[0];
+      t->edep = 0.0;
+    } else {
+      SampleInteraction(d, &tl.gm, flat(&rng));
+      if (t->winner != 3) GammaPerform(d, p, &tl);
+      else t->edep = 0.0;
+    }
+    trueStep = geomStep = t->gstep;
+    stopped = t->ekin <= 0.0;
+  } else {
+    ElectronHowFar(d, p, &tl);
+    ElectronPerform(d, p, &tl);
+    trueStep = tl.el.pstep;
+    geomStep = t->gstep;
+    displ[0] = tl.el.msc.displ[0];
+    displ[1] = tl.el.msc.displ[1];
+    displ[2] = tl.el.msc.displ[2];
+    stopped = t->ekin <= 0.0;
+    out->msc_init_range[i] = tl.el.msc.init_range;
+    out->msc_dyn_range_factor[i] = tl.el.msc.dyn_range_factor;
+    out->msc_tlimit_min[i] = tl.el.msc.tlimit_min;
+    out->msc_first_step[i] = (uint8_t)tl.el.msc.first_step;
+  }
+  /* state out */
+  out->ekin[i] = t->ekin;
+  out->dir[i] = t->dir[0];
+  out->dir[n + i] = t->dir[1];
+  out->dir[2 * n + i] = t->dir[2];
+  for (int k = 0; k < 4; ++k) out->num_ia_left[k * n + i] = t->nial[k];
+  out->rng_ctr[i] = rng.ctr;
+  out->rng_gauss[i] = rng.gauss;
+  out->rng_has_gauss[i] = (uint8_t)rng.has_gauss;
+  if (out != in) {
+    out->ptype[i] = in->ptype[i];
+    out->id[i] = in->id[i];
+    out->imc[i] = in->imc[i];
+    out->safety[i] = in->safety[i];
+    out->on_boundary[i] = in->on_boundary[i];
+  }
+  /* results */
+  res->true_step[i] = trueStep;
+  res->geom_step[i] = geomStep;
+  res->edep[i] = t->edep;
+  res->displacement[i] = displ[0];
+  res->displacement[n + i] = displ[1];
+  res->displacement[2 * n + i] = displ[2];
+  res->winner[i] = t->winner;
+  res->stopped[i] = (uint8_t)stopped;
+  /* secondaries: electron list first, then gammas (StackSecondaries order) */
+  int ns = 0;
+  osec* list[4];
+  for (int k = 0; k < tl.nsec_el; ++k) list[ns++] = &tl.sec_el[k];
+  for (int k = 0; k < tl.nsec_gm; ++k) list[ns++] = &tl.sec_gm[k];
+  res->nsec[i] = ns;
+  for (int s = 0; s < 2; ++s) {
+    if (s < ns) {
+      res->sec_ptype[s * n + i] = (int8_t)list[s]->ptype;
+      res->sec_ekin[s * n + i] = list[s]->ekin;
+      for (int c = 0; c < 3; ++c) res->sec_dir[(c * 2 + s) * n + i] = list[s]->dir[c];
+    } else {
+      res->sec_ptype[s * n + i] = -1;
+      res->sec_ekin[s * n + i] = 0.0;
+      for (int c = 0; c < 3; ++c) res->sec_dir[(c * 2 + s) * n + i] = 0.0;
+    }
+  }
+}
+
+static void* step_worker(void* arg) {
+  const step_job* J = (const step_job*)arg;
+  for (int i = J->begin; i < J->end; ++i) step_one(J, i);
+  return NULL;
+}
+
+int oracle_step(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed, const hepem_tracks* in,
+                hepem_tracks* out, hepem_step_out* res, int nthreads) {
+  if (!d || !p || !in || !out || !res || in->n < 0) return -1;
+  const int n = in->n;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if (nthreads > n) nthreads = n > 0 ? n : 1;
+  step_job jobs[256];
+  pthread_t th[256];
+  const int chunk = (n + nthreads - 1) / nthreads;
+  for (int k = 0; k < nthreads; ++k) {
+    jobs[k].d = d; jobs[k].p = p; jobs[k].seed = seed;
+    jobs[k].in = in; jobs[k].out = out; jobs[k].res = res;
+    jobs[k].begin = k * chunk < n ? k * chunk : n;
+    jobs[k].end = (k + 1) * chunk < n ? (k + 1) * chunk : n;
+  }
+  if (nthreads == 1) {
+    step_worker(&jobs[0]);
+    return 0;
+  }
+  for (int k = 0; k < nthreads; ++k) pthread_create(&th[k], NULL, step_worker, &jobs[k]);
+  for (int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+  return 0;
+}

@@ -1,0 +1,54 @@
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+GOLDEN = ROOT / "tests" / "golden" / "golden_v1.npz"
+SET1 = ["G4_Pb", "G4_lAr"]
+SET2 = ["G4_PbWO4", "G4_Pb", "G4_lAr", "G4_Si"]
+CUT = 0.7
+STEP_SEED = 0x5EED2026
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+@pytest.fixture(scope="session")
+def tables1():
+    from g4hepem_amd import Tables
+    return Tables(SET1, CUT)
+
+
+@pytest.fixture(scope="session")
+def tables2():
+    from g4hepem_amd import Tables
+    return Tables(SET2, CUT)
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle.oracle_py import Oracle
+    return Oracle()
+
+
+def golden_tracks(g, prefix):
+    from oracle.oracle_py import TRACK_FIELDS
+    return {k: g[f"{prefix}_{k}"] for k in TRACK_FIELDS}
+
+
+def same_bits(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype.kind == "f":
+        return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    return np.array_equal(a, b)
