@@ -216,23 +216,33 @@ struct Rng {
 // kernel argument, carries the array bases (in SGPRs) and the scalars. Arrays
 // of table values use the precision R of the form (double or float); per-couple,
 // per-material and per-element scalars stay double.
+// Abscissas (energy, range and selector grids) are read from the *64 arrays,
+// which are double in both forms: the interval fraction (x - x1)/(x2 - x1) of an
+// interpolation loses all precision to cancellation when x and x1 are rounded
+// to float near a grid node. In the FP64 form they alias the R arrays.
 template <typename R>
 struct DevEl {
   const R* lossGrid;    // [N] loss energy grid
+  const double* lossGrid64;
   const R* lossData;    // [5N per couple] [R,R''] [dEdx,dEdx''] [InvR'']
+  const double* lossData64;
   int N;
   R lossLogMin, lossILD;
   const int* xsStart;   // [couple] start of the ioni block in xsData
   const R* xsData;      // restricted macroscopic cross sections
+  const double* xsData64;
   const R* nucGrid;     // [nucN]
+  const double* nucGrid64;
   const R* nucData;     // [2 nucN per material]
   int nucN;
   R nucLogMin, nucILD;
   const R* tr1;         // [2N per material]
   const int* selSBStart;
   const R* selSB;
+  const double* selSB64;
   const int* selRBStart;
   const R* selRB;
+  const double* selRB64;
 };
 
 template <typename R>
@@ -293,12 +303,14 @@ struct DevTables {
   int sbNumKappa;
   // gamma macroscopic cross sections (3 windows) and the conversion selector
   const R* gmData;
+  const double* gmData64;
   int gmPerMat, gmOff1, gmOff2, gmN0, gmN1, gmN2;
   double gEMax0, gEMax1;
   R gLogEMin0, gILD0, gLogEMin1, gILD1, gLogEMin2, gILD2;
   const int* convSelStart;
   const R* convSelData;
   const R* convSelGrid;
+  const double* convSelGrid64;
   int convSelN;
   R convSelLogMin, convSelILD;
   // parameters

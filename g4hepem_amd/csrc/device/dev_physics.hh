@@ -27,59 +27,63 @@ template <typename R> HD void Rotate(R* dir, const R* ref) {  // RotateToReferen
     dir[2] = -dir[2];
   }
 }
-template <typename R> HD R Spline(R x1, R x2, R y1, R y2, R s1, R s2, R x) {  // GetSpline :49-61
-  const R dl = x2 - x1;
-  const R b = tmax((R)0., tmin((R)1., (x - x1) / dl));
+// Interpolation: abscissas x, x1, x2 are double in both forms (see DevEl),
+// ordinates and second derivatives have the form's precision R.
+template <typename R> HD R Spline(double x1, double x2, R y1, R y2, R s1, R s2, double x) {  // GetSpline :49-61
+  const double dl = x2 - x1;
+  const R b = (R)tmax(0., tmin(1., (x - x1) / dl));
   const R os = (R)0.166666666667;
   const R c0 = ((R)2.0 - b) * s1;
   const R c1 = ((R)1.0 + b) * s2;
-  return y1 + b * (y2 - y1) + (b * (b - (R)1.0)) * (c0 + c1) * (dl * dl * os);
+  return y1 + b * (y2 - y1) + (b * (b - (R)1.0)) * (c0 + c1) * ((R)dl * (R)dl * os);
 }
-template <typename R> HD R Linear(R x1, R x2, R y1, R y2, R x) {  // GetLinear :63-71
-  const R dl = x2 - x1;
-  const R b = tmax((R)0., tmin((R)1., (x - x1) / dl));
+template <typename R> HD R Linear(double x1, double x2, R y1, R y2, double x) {  // GetLinear :63-71
+  const double dl = x2 - x1;
+  const R b = (R)tmax(0., tmin(1., (x - x1) / dl));
   return y1 + b * (y2 - y1);
 }
 template <typename R> HD int LogBin(R logx, R logxmin, R invLD, int ndata) {
   return (int)tmax((R)0., tmin((logx - logxmin) * invLD, (R)(ndata - 2)));
 }
 // x grid separate, [y, y''] interleaved (GetSplineLog :84-91)
-template <typename R> HD R SplineLog2(int nd, const R* xd, const R* yd, R x, R logx, R lmin, R ild) {
-  const R xv = tmax(xd[0], tmin(xd[nd - 1], x));
+template <typename R> HD R SplineLog2(int nd, const double* xd, const R* yd, double x, R logx, R lmin, R ild) {
+  const double xv = tmax(xd[0], tmin(xd[nd - 1], x));
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i2 = 2 * idx;
   return Spline(xd[idx], xd[idx + 1], yd[i2], yd[i2 + 2], yd[i2 + 1], yd[i2 + 3], xv);
 }
-// [x, y, y''] interleaved (GetSplineLog :95-102)
-template <typename R> HD R SplineLog3(int nd, const R* d, R x, R logx, R lmin, R ild) {
-  const R xv = tmax(d[0], tmin(d[3 * (nd - 1)], x));
+// [x, y, y''] interleaved (GetSplineLog :95-102); x read from the double copy dx
+template <typename R> HD R SplineLog3(int nd, const double* dx, const R* d, double x, R logx, R lmin, R ild) {
+  const double xv = tmax(dx[0], tmin(dx[3 * (nd - 1)], x));
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i3 = 3 * idx;
-  return Spline(d[i3], d[i3 + 3], d[i3 + 1], d[i3 + 4], d[i3 + 2], d[i3 + 5], xv);
+  return Spline(dx[i3], dx[i3 + 3], d[i3 + 1], d[i3 + 4], d[i3 + 2], d[i3 + 5], xv);
 }
-template <typename R> HD R LinearLog(int nd, const R* d, R x, R logx, R lmin, R ild) {  // :125-133
+template <typename R> HD R LinearLog(int nd, const double* dx, const R* d, double x, R logx, R lmin, R ild) {  // :125-133
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i0 = 2 * idx, i1 = i0 + 2;
-  return Linear(d[i0], d[i1], d[i0 + 1], d[i1 + 1], x);
+  return Linear(dx[i0], dx[i1], d[i0 + 1], d[i1 + 1], x);
 }
-template <typename R> HD void LinearLog2All(int nd, const R* d, R x, R logx, R lmin, R ild, R res[2]) {  // :146-161
+template <typename R>
+HD void LinearLog2All(int nd, const double* dx, const R* d, double x, R logx, R lmin, R ild, R res[2]) {  // :146-161
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i0 = 3 * idx, i1 = i0 + 3;
-  const R dl = d[i1] - d[i0];
-  const R b = tmax((R)0., tmin((R)1., (x - d[i0]) / dl));
+  const double dl = dx[i1] - dx[i0];
+  const R b = (R)tmax(0., tmin(1., (x - dx[i0]) / dl));
 #pragma unroll
   for (int i = 1; i < 3; ++i) {
     const R y1 = d[i0 + i], y2 = d[i1 + i];
     res[i - 1] = tmax((R)0.0, y1 + b * (y2 - y1));
   }
 }
-template <typename R> HD R SplineLog4(int nd, const R* d, R x, R logx, R lmin, R ild, int iwhich) {  // :178-187
+template <typename R>
+HD R SplineLog4(int nd, const double* dx, const R* d, double x, R logx, R lmin, R ild, int iwhich) {  // :178-187
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i0 = 9 * idx, i1 = i0 + 9;
   iwhich = (iwhich - 1) * 2 + 1;
-  return Spline(d[i0], d[i1], d[i0 + iwhich], d[i1 + iwhich], d[i0 + iwhich + 1], d[i1 + iwhich + 1], x);
+  return Spline(dx[i0], dx[i1], d[i0 + iwhich], d[i1 + iwhich], d[i0 + iwhich + 1], d[i1 + iwhich + 1], x);
 }
-template <typename R> HD int FindLowerBin(const R* xd, int num, R x, int step) {  // FindLowerBinIndex :227-236
+HD int FindLowerBin(const double* xd, int num, double x, int step) {  // FindLowerBinIndex :227-236
   int ml = -1, mu = num - 1;
   while (abs(mu - ml) > 1) {
     const int mav = (int)(0.5 * (ml + mu));
@@ -118,58 +122,60 @@ template <typename R> struct SecList {
 };
 
 // ---- G4HepEmElectronManager.icc: table accessors ------------------------------------------
-template <typename R> HD R GetRestRange(const DevEl<R>& e, int imc, R ekin, R lekin) {  // :486-493
-  return tmax((R)0.0, SplineLog2(e.N, e.lossGrid, e.lossData + 5 * e.N * imc, ekin, lekin, e.lossLogMin, e.lossILD));
+template <typename R> HD R GetRestRange(const DevEl<R>& e, int imc, double ekin, R lekin) {  // :486-493
+  return tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.lossData + 5 * e.N * imc, ekin, lekin, e.lossLogMin, e.lossILD));
 }
-template <typename R> HD R GetRestDEDX(const DevEl<R>& e, int imc, R ekin, R lekin) {  // :495-502
-  return tmax((R)0.0, SplineLog2(e.N, e.lossGrid, e.lossData + e.N * (5 * imc + 2), ekin, lekin, e.lossLogMin, e.lossILD));
+template <typename R> HD R GetRestDEDX(const DevEl<R>& e, int imc, double ekin, R lekin) {  // :495-502
+  return tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.lossData + e.N * (5 * imc + 2), ekin, lekin, e.lossLogMin, e.lossILD));
 }
 // :504-520; the low-range quadratic branch is an FP64 island in both forms
 template <typename R> HD double GetInvRange(const DevEl<R>& e, int imc, double range) {
   const int N = e.N;
-  const R* xd = e.lossData + 5 * N * imc;
-  const double minRange = (double)xd[0];
+  const double* xd = e.lossData64 + 5 * N * imc;
+  const double minRange = xd[0];
   if (range < minRange) {
     const double dum = range / minRange;
-    return tmax(0.0, (double)e.lossGrid[0] * dum * dum);
+    return tmax(0.0, e.lossGrid64[0] * dum * dum);
   }
-  const R r = (R)range;
-  const int i = FindLowerBin(xd, N, r, 2);
-  const R* sd = xd + 4 * N;
-  return (double)tmax((R)0.0, Spline(xd[2 * i], xd[2 * (i + 1)], e.lossGrid[i], e.lossGrid[i + 1], sd[i], sd[i + 1], r));
+  const int i = FindLowerBin(xd, N, range, 2);
+  const R* sd = e.lossData + 5 * N * imc + 4 * N;
+  return (double)tmax((R)0.0, Spline(xd[2 * i], xd[2 * (i + 1)], e.lossGrid[i], e.lossGrid[i + 1], sd[i], sd[i + 1], range));
 }
-template <typename R> HD R GetRestMacXSec(const DevEl<R>& e, int imc, R ekin, R lekin, bool isioni) {  // :522-532
+template <typename R> HD R GetRestMacXSec(const DevEl<R>& e, int imc, double ekin, R lekin, bool isioni) {  // :522-532
   const int iIoni = e.xsStart[imc];
-  const int nIoni = (int)e.xsData[iIoni];
+  const int nIoni = (int)e.xsData64[iIoni];
   const int iStart = isioni ? iIoni : iIoni + 3 * nIoni + 5;
+  const double* x64 = e.xsData64 + iStart;
   const R* x = e.xsData + iStart;
-  const int nData = (int)x[0];
-  if (ekin < x[5]) return (R)0.0;
-  return tmax((R)0.0, SplineLog3(nData, x + 5, ekin, lekin, x[3], x[4]));
+  const int nData = (int)x64[0];
+  if (ekin < x64[5]) return (R)0.0;
+  return tmax((R)0.0, SplineLog3(nData, x64 + 5, x + 5, ekin, lekin, x[3], x[4]));
 }
-template <typename R> HD R GetMacXSecNuclear(const DevEl<R>& e, int imat, R ekin, R lekin) {  // :534-541
-  if (ekin < e.nucGrid[0]) return (R)0.0;
-  return tmax((R)0.0, SplineLog2(e.nucN, e.nucGrid, e.nucData + imat * 2 * e.nucN, ekin, lekin, e.nucLogMin, e.nucILD));
+template <typename R> HD R GetMacXSecNuclear(const DevEl<R>& e, int imat, double ekin, R lekin) {  // :534-541
+  if (ekin < e.nucGrid64[0]) return (R)0.0;
+  return tmax((R)0.0, SplineLog2(e.nucN, e.nucGrid64, e.nucData + imat * 2 * e.nucN, ekin, lekin, e.nucLogMin, e.nucILD));
 }
-template <typename R> HD R GetRestMacXSecForStepping(const DevEl<R>& e, int imc, R ekin, R lekin, bool isioni) {  // :544-568
+template <typename R> HD R GetRestMacXSecForStepping(const DevEl<R>& e, int imc, double ekin, R lekin, bool isioni) {  // :544-568
   const R log08 = (R)-0.22314355131420971;
   const int iIoni = e.xsStart[imc];
-  const int nIoni = (int)e.xsData[iIoni];
+  const int nIoni = (int)e.xsData64[iIoni];
   const int iStart = isioni ? iIoni : iIoni + 3 * nIoni + 5;
+  const double* x64 = e.xsData64 + iStart;
   const R* x = e.xsData + iStart;
-  const int nData = (int)x[0];
-  const R minE = x[5], maxE = x[1], maxV = x[2];
+  const int nData = (int)x64[0];
+  const double minE = x64[5], maxE = x64[1];
+  const R maxV = x[2];
   if (ekin > maxE) {
-    const R ekinReduced = (R)0.8 * ekin;
+    const double ekinReduced = 0.8 * ekin;
     if (ekinReduced < maxE) return tmax((R)0.0, maxV);
     ekin = ekinReduced;
     lekin += log08;
   }
   if (ekin < minE) return (R)0.0;
-  return tmax((R)0.0, SplineLog3(nData, x + 5, ekin, lekin, x[3], x[4]));
+  return tmax((R)0.0, SplineLog3(nData, x64 + 5, x + 5, ekin, lekin, x[3], x[4]));
 }
-template <typename R> HD R GetTransportMFP(const DevEl<R>& e, int im, R ekin, R lekin) {  // :576-583
-  const R tr = tmax((R)0.0, SplineLog2(e.N, e.lossGrid, e.tr1 + 2 * e.N * im, ekin, lekin, e.lossLogMin, e.lossILD));
+template <typename R> HD R GetTransportMFP(const DevEl<R>& e, int im, double ekin, R lekin) {  // :576-583
+  const R tr = tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.tr1 + 2 * e.N * im, ekin, lekin, e.lossLogMin, e.lossILD));
   return tr > (R)0. ? (R)1. / tr : (R)kALargeValue;
 }
 template <typename R> HD R MacXsecAnnihilation(R ekin, R eDens) {  // :585-593
@@ -565,19 +571,22 @@ template <typename R> HD void IoniPerform(const DevTables<R>& T, Trk<R>& t, SecL
 }
 
 // ---- G4HepEmElectronInteractionBrem.icc ---------------------------------------------------------
-template <typename R> HD int BremSelectTargetAtom(const DevEl<R>& e, int imc, R ekin, R lekin, R urndn, bool isSB) {  // :266-297
+template <typename R>
+HD int BremSelectTargetAtom(const DevEl<R>& e, int imc, double ekin, R lekin, R urndn, bool isSB) {  // :266-297
   const int start = isSB ? e.selSBStart[imc] : e.selRBStart[imc];
+  const double* d64 = (isSB ? e.selSB64 : e.selRB64) + start;
   const R* data = (isSB ? e.selSB : e.selRB) + start;
-  const int numData = (int)data[0];
-  const int numElem = (int)data[1];
+  const int numData = (int)d64[0];
+  const int numElem = (int)d64[1];
   const R logE0 = data[2], invLD = data[3];
+  const double* xd64 = d64 + 4;
   const R* xd = data + 4;
-  const R xv = tmax(xd[0], tmin(xd[numElem * (numData - 1)], ekin));
+  const double xv = tmax(xd64[0], tmin(xd64[numElem * (numData - 1)], ekin));
   const int idxEkin = (int)tmax((R)0.0, tmin((lekin - logE0) * invLD, (R)(numData - 2)));
   int i0 = idxEkin * numElem, i1 = i0 + numElem;
-  const R x1 = xd[i0++], x2 = xd[i1++];
-  const R dl = x2 - x1;
-  const R b = tmax((R)0., tmin((R)1., (xv - x1) / dl));
+  const double x1 = xd64[i0++], x2 = xd64[i1++];
+  const double dl = x2 - x1;
+  const R b = (R)tmax(0., tmin(1., (xv - x1) / dl));
   int ie = 0;
   while (ie < numElem - 1 && urndn > xd[i0 + ie] + b * (xd[i1 + ie] - xd[i0 + ie])) ++ie;
   return ie;
@@ -598,11 +607,13 @@ template <typename R> HD int BremLinSearch(const R* v, int size, R val) {  // :3
   }
   return i;
 }
-template <typename R> HD R SampleETransferSB(const DevTables<R>& T, R primEkin, R lEkin, int imc, Rng& rng, bool isEl) {  // :73-182
+template <typename R>
+HD R SampleETransferSB(const DevTables<R>& T, double primEkinD, R lEkin, int imc, Rng& rng, bool isEl) {  // :73-182
+  const R primEkin = (R)primEkinD;
   const R gamCut = (R)T.mcGamCut[imc], logGamCut = (R)T.mcLogGamCut[imc];
   const int imat = T.mcMat[imc];
   const int nel = T.matNElem[imat];
-  const int elemIndx = nel > 1 ? BremSelectTargetAtom(T.el[isEl ? 0 : 1], imc, primEkin, lEkin, (R)rng.flat(), true) : 0;
+  const int elemIndx = nel > 1 ? BremSelectTargetAtom(T.el[isEl ? 0 : 1], imc, primEkinD, lEkin, (R)rng.flat(), true) : 0;
   const int iZet = T.matElemZ[T.matElemStart[imat] + elemIndx];
   const R dZet = (R)iZet;
   const int iStart = T.sbStartZ[iZet];
@@ -666,11 +677,13 @@ template <typename R> HD R SampleETransferSB(const DevTables<R>& T, R primEkin, 
   } while (r[1] > suppression);
   return eGamma;
 }
-template <typename R> HD R SampleETransferRB(const DevTables<R>& T, R primEkin, R lEkin, int imc, Rng& rng, bool isEl) {  // :184-263
+template <typename R>
+HD R SampleETransferRB(const DevTables<R>& T, double primEkinD, R lEkin, int imc, Rng& rng, bool isEl) {  // :184-263
+  const R primEkin = (R)primEkinD;
   const R gamCut = (R)T.mcGamCut[imc];
   const int imat = T.mcMat[imc];
   const int nel = T.matNElem[imat];
-  const int elemIndx = nel > 1 ? BremSelectTargetAtom(T.el[isEl ? 0 : 1], imc, primEkin, lEkin, (R)rng.flat(), false) : 0;
+  const int elemIndx = nel > 1 ? BremSelectTargetAtom(T.el[isEl ? 0 : 1], imc, primEkinD, lEkin, (R)rng.flat(), false) : 0;
   const int iZet = T.matElemZ[T.matElemStart[imat] + elemIndx];
   const R dZet = (R)iZet;
   const int z = iZet < T.maxZ ? iZet : T.maxZ;
@@ -747,7 +760,8 @@ template <typename R> HD void BremPerform(const DevTables<R>& T, Trk<R>& t, SecL
   const R lEkin = t.logE();
   const R pE = (R)primEkin;
   if (pE <= (R)T.mcGamCut[t.imc]) return;
-  const R eGamma = isSB ? SampleETransferSB(T, pE, lEkin, t.imc, rng, t.isEl) : SampleETransferRB(T, pE, lEkin, t.imc, rng, t.isEl);
+  const R eGamma = isSB ? SampleETransferSB(T, primEkin, lEkin, t.imc, rng, t.isEl)
+                        : SampleETransferRB(T, primEkin, lEkin, t.imc, rng, t.isEl);
   Sec<R>* s = sl.addGm();
   BremSampleDirections(pE, eGamma, s->dir, t.dir, rng);
   t.setE(primEkin - (double)eGamma);
@@ -843,9 +857,9 @@ HD void ConvertTrueToGeometricLength(const DevTables<R>& T, Msc<R>& m, R ekin, R
   } else {
     const R rfin = tmax(range - m.trueStep, (R)0.01 * range);
     const DevEl<R>& e = T.el[isEl ? 0 : 1];
-    const R t1 = (R)GetInvRange(e, imc, (double)rfin);
+    const double t1 = GetInvRange(e, imc, (double)rfin);
     const int imat = T.mcMat[imc];
-    const R lambda1 = GetTransportMFP(e, imat, t1, vlog(t1));
+    const R lambda1 = GetTransportMFP(e, imat, t1, vlog((R)t1));
     m.par1 = (m.lambtr1 - lambda1) / (m.lambtr1 * m.trueStep);
     m.par2 = (R)1. / (m.par1 * m.lambtr1);
     m.par3 = (R)1. + m.par2;
@@ -876,7 +890,7 @@ template <typename R> HD void ConvertGeometricToTrueLength(Msc<R>& m, R range, R
 template <typename R> HD void HowFarToDiscreteInteraction(const DevTables<R>& T, ElState<R>& el) {  // :48-101
   Trk<R>& t = el.t;
   int winner = -1;
-  const R ekin = (R)t.ekin;
+  const double ekin = t.ekin;
   const R lekin = t.logE();
   const int imc = t.imc;
   const DevEl<R>& e = T.el[t.isEl ? 0 : 1];
@@ -889,7 +903,7 @@ template <typename R> HD void HowFarToDiscreteInteraction(const DevTables<R>& T,
   R mx[4];
   mx[0] = GetRestMacXSecForStepping(e, imc, ekin, lekin, true);
   mx[1] = GetRestMacXSecForStepping(e, imc, ekin, lekin, false);
-  mx[2] = t.isEl ? (R)0.0 : MacXsecAnnihilation((R)0.8 * ekin, (R)T.matElDens[imat]);
+  mx[2] = t.isEl ? (R)0.0 : MacXsecAnnihilation((R)(0.8 * ekin), (R)T.matElDens[imat]);
   mx[3] = GetMacXSecNuclear(e, imat, ekin, lekin);
 #pragma unroll
   for (int ip = 0; ip < 4; ++ip) {
@@ -915,6 +929,7 @@ template <typename R> HD void HowFarToMSC(const DevTables<R>& T, ElState<R>& el,
   const int imc = t.imc;
   const DevEl<R>& e = T.el[t.isEl ? 0 : 1];
   const int imat = T.mcMat[imc], ireg = T.mcReg[imc];
+  const double ekinD = t.ekin;
   Msc<R>& m = el.msc;
   m.trueStep = pStep;
   m.zpath = pStep;
@@ -925,7 +940,7 @@ template <typename R> HD void HowFarToMSC(const DevTables<R>& T, ElState<R>& el,
   m.dir[2] = (R)1.;
   if (pStep > (R)5.E-8 && ekin > (R)1.0E-3) {
     m.active = true;
-    m.lambtr1 = GetTransportMFP(e, imat, ekin, lekin);
+    m.lambtr1 = GetTransportMFP(e, imat, ekinD, lekin);
     UMSCStepLimit(T, m, ekin, imat, ireg, range, t.safety, t.onBoundary, t.isEl, rng);
     ConvertTrueToGeometricLength(T, m, ekin, range, imc, t.isEl);
     const R mscTrue = m.trueStep;
@@ -975,7 +990,7 @@ template <typename R> HD bool ApplyMeanEnergyLoss(const DevTables<R>& T, ElState
   const DevEl<R>& e = T.el[t.isEl ? 0 : 1];
   const int imc = t.imc;
   const R lekin = t.logE();
-  double eloss = (double)(pStep * GetRestDEDX(e, imc, (R)ekin, lekin));
+  double eloss = (double)(pStep * GetRestDEDX(e, imc, ekin, lekin));
   const double linLim = T.regLinLim[T.mcReg[imc]];
   if (eloss > ekin * linLim) {
     const double postRange = (double)range - (double)pStep;
@@ -1001,14 +1016,15 @@ template <typename R> HD void SampleMSC(const DevTables<R>& T, ElState<R>& el, R
   const R range = el.range;
   Msc<R>& m = el.msc;
   if (m.active && (pStep > tmax((R)1.0E-8, (R)1.0e-16 * m.lambtr1))) {
-    R postEkin = preEkin;
+    double postEkinD = el.preEkin;
     R postLEkin = el.preLEkin;
     if (pStep > range * (R)0.01) {
-      postEkin = (R)t.ekin;
+      postEkinD = t.ekin;
       postLEkin = t.logE();
     }
+    const R postEkin = (R)postEkinD;
     const int imat = T.mcMat[imc];
-    const R postTr1mfp = GetTransportMFP(T.el[t.isEl ? 0 : 1], imat, postEkin, postLEkin);
+    const R postTr1mfp = GetTransportMFP(T.el[t.isEl ? 0 : 1], imat, postEkinD, postLEkin);
     UMSCSampleScattering(T, m, pStep, preEkin, m.lambtr1, postEkin, postTr1mfp, imat, t.isEl, T.posCor != 0, T.displace != 0, rng);
     if (!m.noScat) {
       Rotate(m.dir, t.dir);
@@ -1055,10 +1071,10 @@ template <typename R> HD bool CheckDelta(const DevTables<R>& T, Trk<R>& t, R rnd
   const int iDProc = t.winner;
   const int imc = t.imc;
   const int imat = T.mcMat[imc];
-  const R ekin = (R)t.ekin;
+  const double ekin = t.ekin;
   const R lekin = t.logE();
   const R mxsec = iDProc < 2 ? GetRestMacXSec(e, imc, ekin, lekin, iDProc == 0)
-                : (iDProc < 3 ? MacXsecAnnihilation(ekin, (R)T.matElDens[imat]) : GetMacXSecNuclear(e, imat, ekin, lekin));
+                : (iDProc < 3 ? MacXsecAnnihilation((R)ekin, (R)T.matElDens[imat]) : GetMacXSecNuclear(e, imat, ekin, lekin));
   return mxsec <= (R)0.0 || rnd > mxsec * t.mfp[iDProc];
 }
 
@@ -1127,23 +1143,23 @@ template <typename R> HD R GetMacXSecPE(const DevTables<R>& T, int imat, R ekin)
   return SandiaEval(T.matSandiaE + s, T.matSandiaC + 4 * s, T.matSandiaN[imat], ekin);
 }
 template <typename R> HD R GetTotalMacXSec(const DevTables<R>& T, Trk<R>& t, R& peMxSec) {  // :108-153
-  const R ekin = (R)t.ekin;
+  const double ekin = t.ekin;
   const R lekin = t.logE();
   const int imat = T.mcMat[t.imc];
-  if (t.ekin > T.gEMax1) {
-    const R* d = T.gmData + imat * T.gmPerMat + T.gmOff2;
-    return SplineLog4(T.gmN2, d, ekin, lekin, T.gLogEMin2, T.gILD2, 1);
+  if (ekin > T.gEMax1) {
+    const int off = imat * T.gmPerMat + T.gmOff2;
+    return SplineLog4(T.gmN2, T.gmData64 + off, T.gmData + off, ekin, lekin, T.gLogEMin2, T.gILD2, 1);
   }
-  if (t.ekin > T.gEMax0) {
-    const R* d = T.gmData + imat * T.gmPerMat + T.gmOff1;
+  if (ekin > T.gEMax0) {
+    const int off = imat * T.gmPerMat + T.gmOff1;
     R mx[2];
-    LinearLog2All(T.gmN1, d, ekin, lekin, T.gLogEMin1, T.gILD1, mx);
+    LinearLog2All(T.gmN1, T.gmData64 + off, T.gmData + off, ekin, lekin, T.gLogEMin1, T.gILD1, mx);
     peMxSec = mx[1];
     return mx[0];
   }
-  const R* d = T.gmData + imat * T.gmPerMat;
-  const R comp = LinearLog(T.gmN0, d, ekin, lekin, T.gLogEMin0, T.gILD0);
-  const R pe = tmax((R)0.0, GetMacXSecPE(T, imat, ekin));
+  const int off = imat * T.gmPerMat;
+  const R comp = LinearLog(T.gmN0, T.gmData64 + off, T.gmData + off, ekin, lekin, T.gLogEMin0, T.gILD0);
+  const R pe = tmax((R)0.0, GetMacXSecPE(T, imat, (R)ekin));
   peMxSec = pe;
   return comp + pe;
 }
@@ -1153,14 +1169,14 @@ template <typename R> HD void SampleInteraction(const DevTables<R>& T, Trk<R>& t
   const R tmfp = t.mfp[0];
   t.nial[0] = (R)-1.0;
   if (t.ekin > T.gEMax1) {
-    const R ekin = (R)t.ekin;
+    const double ekin = t.ekin;
     const R lekin = t.logE();
-    const R* d = T.gmData + T.mcMat[t.imc] * T.gmPerMat + T.gmOff2;
+    const int off = T.mcMat[t.imc] * T.gmPerMat + T.gmOff2;
     R mxSec = (R)0.0, cProb = (R)0.0;
     int pid = 2;
     do {
       if (pid == 5) { ++pid; break; }
-      mxSec = SplineLog4(T.gmN2, d, ekin, lekin, T.gLogEMin2, T.gILD2, pid);
+      mxSec = SplineLog4(T.gmN2, T.gmData64 + off, T.gmData + off, ekin, lekin, T.gLogEMin2, T.gILD2, pid);
       cProb += mxSec * tmfp;
       ++pid;
     } while (pid < 6 && urnd > cProb);
@@ -1229,16 +1245,17 @@ template <typename R> HD void ComptonPerform(Trk<R>& t, SecList<R>& sl, Rng& rng
   t.edep = edep;
 }
 
-template <typename R> HD int ConvSelectTargetAtom(const DevTables<R>& T, int imat, R ekin, R lekin, R urndn) {  // Conversion.icc:151-176
+template <typename R>
+HD int ConvSelectTargetAtom(const DevTables<R>& T, int imat, double ekin, R lekin, R urndn) {  // Conversion.icc:151-176
   const R* data = T.convSelData + T.convSelStart[imat];
   const int numData = T.convSelN;
   const int numElem = (int)data[0];
-  const R* xd = T.convSelGrid;
-  const R xv = tmax(xd[0], tmin(xd[numData - 1], ekin));
+  const double* xd = T.convSelGrid64;
+  const double xv = tmax(xd[0], tmin(xd[numData - 1], ekin));
   const int idx = (int)tmax((R)0.0, tmin((lekin - T.convSelLogMin) * T.convSelILD, (R)(numData - 2)));
-  const R x1 = xd[idx], x2 = xd[idx + 1];
-  const R dl = x2 - x1;
-  const R b = tmax((R)0., tmin((R)1., (xv - x1) / dl));
+  const double x1 = xd[idx], x2 = xd[idx + 1];
+  const double dl = x2 - x1;
+  const R b = (R)tmax(0., tmin(1., (xv - x1) / dl));
   const int i0 = idx * (numElem - 1) + 1;
   const int i1 = i0 + (numElem - 1);
   int ie = 0;
@@ -1278,7 +1295,7 @@ template <typename R> HD void ConvPerform(const DevTables<R>& T, Trk<R>& t, SecL
   const R lPrimE = t.logE();
   const int imat = T.mcMat[t.imc];
   const int nel = T.matNElem[imat];
-  const int elemIndx = nel > 1 ? ConvSelectTargetAtom(T, imat, primE, lPrimE, (R)rng.flat()) : 0;
+  const int elemIndx = nel > 1 ? ConvSelectTargetAtom(T, imat, primEd, lPrimE, (R)rng.flat()) : 0;
   const int iZet = T.matElemZ[T.matElemStart[imat] + elemIndx];
   const R lpmEnr = (R)kLPMconstant * (R)T.matRadLen[imat];
   const int z = iZet < T.maxZ ? iZet : T.maxZ;

@@ -38,6 +38,16 @@ struct Packer {
     for (size_t i = 0; i < n; ++i) v[i] = (R)src[i];
     add(field, v);
   }
+  // a table whose abscissas are also read in double: in the FP64 form both
+  // fields alias one array, in the FP32 form a double copy is added
+  void addX(const double*& field, const double*& field64, const double* src, size_t n) {
+    add(field, src, n);
+    fix.emplace_back(reinterpret_cast<const void**>(&field64), fix.back().second);
+  }
+  void addX(const float*& field, const double*& field64, const double* src, size_t n) {
+    addR(field, src, n);
+    add(field64, src, n);
+  }
 };
 
 template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronData* e) {
@@ -45,20 +55,20 @@ template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronD
   E.N = N;
   E.lossLogMin = (R)e->fELossLogMinEkin;
   E.lossILD = (R)e->fELossEILDelta;
-  P.addR(E.lossGrid, e->fELossEnergyGrid, N);
-  P.addR(E.lossData, e->fELossData, (size_t)5 * N * e->fNumMatCuts);
+  P.addX(E.lossGrid, E.lossGrid64, e->fELossEnergyGrid, N);
+  P.addX(E.lossData, E.lossData64, e->fELossData, (size_t)5 * N * e->fNumMatCuts);
   P.add(E.xsStart, e->fResMacXSecStartIndexPerMatCut, e->fNumMatCuts);
-  P.addR(E.xsData, e->fResMacXSecData, e->fResMacXSecNumData);
+  P.addX(E.xsData, E.xsData64, e->fResMacXSecData, e->fResMacXSecNumData);
   E.nucN = e->fENucEnergyGridSize;
   E.nucLogMin = (R)e->fENucLogMinEkin;
   E.nucILD = (R)e->fENucEILDelta;
-  P.addR(E.nucGrid, e->fENucEnergyGrid, e->fENucEnergyGridSize);
+  P.addX(E.nucGrid, E.nucGrid64, e->fENucEnergyGrid, e->fENucEnergyGridSize);
   P.addR(E.nucData, e->fENucMacXsecData, (size_t)2 * e->fENucEnergyGridSize * e->fNumMaterials);
   P.addR(E.tr1, e->fTr1MacXSecData, (size_t)2 * N * e->fNumMaterials);
   P.add(E.selSBStart, e->fElemSelectorBremSBStartIndexPerMatCut, e->fNumMatCuts);
-  P.addR(E.selSB, e->fElemSelectorBremSBData, e->fElemSelectorBremSBNumData);
+  P.addX(E.selSB, E.selSB64, e->fElemSelectorBremSBData, e->fElemSelectorBremSBNumData);
   P.add(E.selRBStart, e->fElemSelectorBremRBStartIndexPerMatCut, e->fNumMatCuts);
-  P.addR(E.selRB, e->fElemSelectorBremRBData, e->fElemSelectorBremRBNumData);
+  P.addX(E.selRB, E.selRB64, e->fElemSelectorBremRBData, e->fElemSelectorBremRBNumData);
 }
 
 template <typename R> void PackTables(Packer& P, DevTables<R>& T, const G4HepEmData* d, const G4HepEmParameters* p) {
@@ -179,7 +189,7 @@ template <typename R> void PackTables(Packer& P, DevTables<R>& T, const G4HepEmD
   T.sbILDE = sb->fILDeltaElEnergy;
   T.sbNumKappa = sb->fNumKappa;
   const G4HepEmGammaData* g = d->fTheGammaData;
-  P.addR(T.gmData, g->fMacXsecData, (size_t)g->fNumMaterials * g->fDataPerMat);
+  P.addX(T.gmData, T.gmData64, g->fMacXsecData, (size_t)g->fNumMaterials * g->fDataPerMat);
   T.gmPerMat = g->fDataPerMat;
   T.gmOff1 = g->fNumData0;
   T.gmOff2 = g->fNumData0 + g->fNumData1;
@@ -196,7 +206,7 @@ template <typename R> void PackTables(Packer& P, DevTables<R>& T, const G4HepEmD
   T.gILD2 = (R)g->fEILDelta2;
   P.add(T.convSelStart, g->fElemSelectorConvStartIndexPerMat, g->fNumMaterials);
   P.addR(T.convSelData, g->fElemSelectorConvData, g->fElemSelectorConvNumData);
-  P.addR(T.convSelGrid, g->fElemSelectorConvEgrid, g->fElemSelectorConvEgridSize);
+  P.addX(T.convSelGrid, T.convSelGrid64, g->fElemSelectorConvEgrid, g->fElemSelectorConvEgridSize);
   T.convSelN = g->fElemSelectorConvEgridSize;
   T.convSelLogMin = (R)g->fElemSelectorConvLogMinEkin;
   T.convSelILD = (R)g->fElemSelectorConvEILDelta;

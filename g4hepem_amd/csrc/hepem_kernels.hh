@@ -281,7 +281,8 @@ __global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, in
     for (int k = 0; k < 5; ++k) out[k * n + i] = __longlong_as_double(0x7FF8000000000000LL);
     return;
   }
-  const R E = (R)ekin[i], lE = (R)lekin[i];
+  const double E = ekin[i];
+  const R lE = (R)lekin[i];
   const R r = GetRestRange(e, mc, E, lE);
   out[i] = (double)GetRestDEDX(e, mc, E, lE);
   out[n + i] = (double)r;
