@@ -55,12 +55,50 @@ def algorithmic_bytes(ptype: np.ndarray, out_of_place: bool = True) -> int:
     return ne * e_bytes + (len(ptype) - ne) * g_bytes
 
 
+def measured_traffic(kernel_profile: str):
+    """HBM bytes per launch of the step kernel from the committed hardware
+    counter summary (FETCH_SIZE + WRITE_SIZE passes, each scaled by the same
+    counters over a plain copy of known size); None when absent."""
+    path = os.path.join(ROOT, "profiles", kernel_profile)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1]), "profiles/" + kernel_profile
+    except OSError:
+        pass
+    return None, None
+
+
 def time_steps(ctx, tin, tout, res, work, steps, warmup, seed):
+    """Times `steps` back-to-back steps (prepare + run) with HIP events on the
+    launch stream; the run (physics kernel) of each step is bracketed by its
+    own event pair. Argument structs are built once so the host enqueues ahead
+    of the GPU."""
+    import ctypes as C
+
     import torch
+    from g4hepem_amd import lib
+    L = lib()
     stream = torch.cuda.current_stream()
+    sh = C.c_void_p(int(stream.cuda_stream))
+    si, so, rr = tin.struct(), tout.struct(), res.struct()
+    psi, pso, prr = C.byref(si), C.byref(so), C.byref(rr)
+    wp = C.c_void_p(work.data_ptr())
+    h = ctx.h
+    sd = C.c_uint64(seed)
+
+    def prepare():
+        rc = L.hepem_step_prepare(h, psi, wp, sh)
+        assert rc == 0, rc
+
+    def run():
+        rc = L.hepem_step_run(h, sd, psi, pso, prr, wp, sh)
+        assert rc == 0, rc
+
     for _ in range(warmup):
-        ctx.step_prepare(tin, work)
-        ctx.step_run(seed, tin, tout, res, work)
+        prepare()
+        run()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -68,9 +106,9 @@ def time_steps(ctx, tin, tout, res, work, steps, warmup, seed):
     wall0 = time.perf_counter()
     t_start.record(stream)
     for k in range(steps):
-        ctx.step_prepare(tin, work)
+        prepare()
         ev[k][0].record(stream)
-        ctx.step_run(seed, tin, tout, res, work)
+        run()
         ev[k][1].record(stream)
     t_end.record(stream)
     torch.cuda.synchronize()
@@ -101,27 +139,25 @@ def time_lookups(ctx, n, reps, seed=2026):
 
 
 def cpu_baseline(tables, tracks: dict, min_seconds: float, seed: int):
-    """The oracle (C restatement) on the host cores: repeated passes over a
-    bounded prefix of the same workload until min_seconds of CPU work."""
+    """The oracle (C restatement of the reference path) on the host cores:
+    repeated passes over the first 250k tracks of the same batch until
+    min_seconds of work are done."""
     from oracle.oracle_py import Oracle
     o = Oracle()
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(int(cores), 64))
-    n_sample = min(len(tracks["ekin"]), 250_000)
-    sample = {k: v[: n_sample * (len(v) // len(tracks["ekin"]))] if len(v) != len(tracks["ekin"]) else v[:n_sample]
-              for k, v in tracks.items()}
-    # planar fields must be re-sliced component by component
     n_full = len(tracks["ekin"])
+    n_sample = min(n_full, 250_000)
+    sample = {}
     for k, v in tracks.items():
         c = len(v) // n_full
-        if c > 1:
-            sample[k] = np.concatenate([v[i * n_full: i * n_full + n_sample] for i in range(c)])
+        sample[k] = np.concatenate([v[i * n_full: i * n_full + n_sample] for i in range(c)])
     t0 = time.perf_counter()
     done = 0
     while True:
         o.step(tables, seed, sample, nthreads=cores)
         done += n_sample
-        if time.perf_counter() - t0 >= min_seconds or done >= 40 * n_sample:
+        if time.perf_counter() - t0 >= min_seconds:
             break
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "track-steps/s", "cores": cores, "kind": "port",
@@ -139,6 +175,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline FP64 step only (for profiling)")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="before the steps, copy a buffer of this many MiB device-to-device "
+                         "(reference kernel of known traffic for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -159,11 +198,19 @@ def main():
     res = StepOutSoA(n, "torch", dev)
     work = Context.work_buffer(n, dev)
 
+    if args.calibration_copy_mib > 0:
+        nb = args.calibration_copy_mib * 1024 * 1024
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx64 = Context(tables2, HEPEM_FP64)
     ms64, run64, wall64 = time_steps(ctx64, tin, tout, res, work, args.steps, args.warmup, args.seed)
     value = n / (ms64 * 1e-3)
     abytes = algorithmic_bytes(tracks["ptype"])
     achieved = abytes / (run64 * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic("k_step_fp64_1M_pmc.md")
     line = {
         "metric": "track-steps/s on 1 MI355X, FP64 vs FP32-table form, + HBM GB/s vs peak",
         "value": value,
@@ -184,7 +231,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": "k_step<double>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel_ms": run64, "algorithmic_bytes_per_launch": abytes,
         },
     }

@@ -114,12 +114,24 @@ template <typename R> struct Msc {
   bool noScat, isDisplace, firstStep, active;
 };
 template <typename R> struct Sec { int ptype; double ekin; R dir[3]; };
+// Secondaries of one step. Each interaction creates at most two, all of one
+// kind (electron list or gamma list of G4HepEmTLData), so creation order is
+// also the reference's stacking order (electrons first, then gammas) and the
+// two slots are addressed statically (no run-time indexed arrays).
 template <typename R> struct SecList {
-  int nel, ngm;
-  Sec<R> el[2], gm[2];
-  HD Sec<R>* addEl(int pt) { Sec<R>* s = &el[nel++]; s->ptype = pt; return s; }
-  HD Sec<R>* addGm() { Sec<R>* s = &gm[ngm++]; s->ptype = 2; return s; }
+  int ns;
+  Sec<R> s0, s1;
 };
+// G4HepEmTrack::SetNumIALeft / GetMFP at a run-time process index, written as
+// selects over the four slots so the arrays stay in registers
+template <typename R> HD void ResetNumIALeft(R* nial, int ip) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k == ip) nial[k] = (R)-1.0;
+}
+template <typename R> HD R PickMFP(const R* mfp, int ip) {
+  return ip == 0 ? mfp[0] : ip == 1 ? mfp[1] : ip == 2 ? mfp[2] : mfp[3];
+}
 
 // ---- G4HepEmElectronManager.icc: table accessors ------------------------------------------
 template <typename R> HD R GetRestRange(const DevEl<R>& e, int imc, double ekin, R lekin) {  // :486-493
@@ -564,7 +576,9 @@ template <typename R> HD void IoniPerform(const DevTables<R>& T, Trk<R>& t, SecL
   const R maxET = t.isEl ? (R)0.5 * pE : pE;
   if (maxET <= elCut) return;
   const R delta = t.isEl ? SampleETransferMoller(elCut, pE, rng) : SampleETransferBhabha(elCut, pE, rng);
-  Sec<R>* s = sl.addEl(0);
+  Sec<R>* s = &sl.s0;
+  sl.ns = 1;
+  s->ptype = 0;
   IoniSampleDirections(pE, delta, s->dir, t.dir, rng);
   t.setE(primEkin - (double)delta);
   s->ekin = (double)delta;
@@ -762,7 +776,9 @@ template <typename R> HD void BremPerform(const DevTables<R>& T, Trk<R>& t, SecL
   if (pE <= (R)T.mcGamCut[t.imc]) return;
   const R eGamma = isSB ? SampleETransferSB(T, primEkin, lEkin, t.imc, rng, t.isEl)
                         : SampleETransferRB(T, primEkin, lEkin, t.imc, rng, t.isEl);
-  Sec<R>* s = sl.addGm();
+  Sec<R>* s = &sl.s0;
+  sl.ns = 1;
+  s->ptype = 2;
   BremSampleDirections(pE, eGamma, s->dir, t.dir, rng);
   t.setE(primEkin - (double)eGamma);
   s->ekin = (double)eGamma;
@@ -773,8 +789,10 @@ template <typename R> HD void AnnihilateAtRest(SecList<R>& sl, Rng& rng) {  // :
   const R cost = (R)2. * (R)rng.flat() - (R)1.;
   const R sint = dsqrt(((R)1. - cost) * ((R)1. + cost));
   const R phi = (R)k2Pi * (R)rng.flat();
-  Sec<R>* g1 = sl.addGm();
-  Sec<R>* g2 = sl.addGm();
+  Sec<R>* g1 = &sl.s0;
+  Sec<R>* g2 = &sl.s1;
+  sl.ns = 2;
+  g1->ptype = g2->ptype = 2;
   g1->dir[0] = sint * dcos(phi);
   g1->dir[1] = sint * dsin(phi);
   g1->dir[2] = cost;
@@ -787,8 +805,10 @@ template <typename R> HD void AnnihilateAtRest(SecList<R>& sl, Rng& rng) {  // :
 template <typename R> HD void AnnihilateInFlight(Trk<R>& t, SecList<R>& sl, Rng& rng) {  // :51-119
   const R primEkin = (R)t.ekin;
   const R* primDir = t.dir;
-  Sec<R>* g1 = sl.addGm();
-  Sec<R>* g2 = sl.addGm();
+  Sec<R>* g1 = &sl.s0;
+  Sec<R>* g2 = &sl.s1;
+  sl.ns = 2;
+  g1->ptype = g2->ptype = 2;
   const R me = (R)kElectronMassC2;
   const R tau = primEkin * (R)kInvElectronMassC2;
   const R gam = tau + (R)1.0;
@@ -1075,13 +1095,13 @@ template <typename R> HD bool CheckDelta(const DevTables<R>& T, Trk<R>& t, R rnd
   const R lekin = t.logE();
   const R mxsec = iDProc < 2 ? GetRestMacXSec(e, imc, ekin, lekin, iDProc == 0)
                 : (iDProc < 3 ? MacXsecAnnihilation((R)ekin, (R)T.matElDens[imat]) : GetMacXSecNuclear(e, imat, ekin, lekin));
-  return mxsec <= (R)0.0 || rnd > mxsec * t.mfp[iDProc];
+  return mxsec <= (R)0.0 || rnd > mxsec * PickMFP(t.mfp, iDProc);
 }
 
 template <typename R> HD void PerformDiscrete(const DevTables<R>& T, Trk<R>& t, SecList<R>& sl, Rng& rng) {  // :425-459
   const int iDProc = t.winner;
   if (iDProc < 0 || t.onBoundary) return;
-  t.nial[iDProc] = (R)-1.0;
+  ResetNumIALeft(t.nial, iDProc);
   if (CheckDelta(T, t, (R)rng.flat())) return;
   switch (iDProc) {
     case 0: IoniPerform(T, t, sl, rng); break;
@@ -1224,7 +1244,9 @@ template <typename R> HD void ComptonPerform(Trk<R>& t, SecList<R>& sl, Rng& rng
   const R secE = primE - postE;
   double edep = 0.0;
   if (secE > thr) {
-    Sec<R>* s = sl.addEl(0);
+    Sec<R>* s = &sl.s0;
+    sl.ns = 1;
+    s->ptype = 0;
     s->dir[0] = primE * org[0] - postE * t.dir[0];
     s->dir[1] = primE * org[1] - postE * t.dir[1];
     s->dir[2] = primE * org[2] - postE * t.dir[2];
@@ -1366,8 +1388,11 @@ template <typename R> HD void ConvPerform(const DevTables<R>& T, Trk<R>& t, SecL
   }
   const R eKin = tmax((R)0., eTot - me);
   const R pKin = tmax((R)0., pTot - me);
-  Sec<R>* se = sl.addEl(0);
-  Sec<R>* sp = sl.addEl(1);
+  Sec<R>* se = &sl.s0;
+  Sec<R>* sp = &sl.s1;
+  sl.ns = 2;
+  se->ptype = 0;
+  sp->ptype = 1;
   const R phi = (R)k2Pi * (R)rng.flat();
   const R cosPhi = dcos(phi), sinPhi = dsin(phi);
   const R costEl = SampleCostModifiedTsai(eKin, rng);
@@ -1417,7 +1442,9 @@ template <typename R> HD void PEPerform(const DevTables<R>& T, Trk<R>& t, R peMx
   const double be = T.elKBind[T.matElemZ[est + ielem]];
   const double peE = gEd - be;
   if (peE > 0.000001) {
-    Sec<R>* s = sl.addEl(0);
+    Sec<R>* s = &sl.s0;
+    sl.ns = 1;
+    s->ptype = 0;
     const R kinE = (R)peE;
     const R tau = kinE * (R)kInvElectronMassC2;
     const R gamma = (R)1.0 + tau;

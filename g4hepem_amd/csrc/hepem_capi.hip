@@ -355,7 +355,7 @@ extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, vo
   uint32_t* queue = counters + 64;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters, 64, 0u);
   if (n > 0) {
-    const int grid = (n + kBlock - 1) / kBlock;
+    const int grid = (n + kClassifyTile - 1) / kClassifyTile;
     hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kBlock), 0, s, n, in->ptype, queue, counters);
   }
   HIP_OK(hipGetLastError());

@@ -25,49 +25,80 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Each block compacts a tile of kClassifyTile consecutive slots in two
+// passes over the tile: count per wave (ballot + popcount), one atomic per
+// block and kind for the tile's base, then scatter. The tile keeps atomic
+// traffic on the two queue counters to a few hundred operations per launch.
+constexpr int kClassifyItems = 16;
+constexpr int kClassifyTile = kBlock * kClassifyItems;
+
 __global__ void __launch_bounds__(kBlock) k_classify(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
                                                      uint32_t* __restrict__ counters) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < n;
-  const int8_t pt = valid ? ptype[i] : (int8_t)-1;
-  const bool isE = valid && (pt == 0 || pt == 1);
-  const bool isG = valid && !isE;
-  const uint64_t mE = __ballot(isE);
-  const uint64_t mG = __ballot(isG);
+  __shared__ uint32_t sE[kBlock / 64], sG[kBlock / 64], sBase[2];
   const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t baseE = 0, baseG = 0;
-  if (lane == 0) {
-    if (mE) baseE = atomicAdd(&counters[0], (uint32_t)__popcll(mE));
-    if (mG) baseG = atomicAdd(&counters[1], (uint32_t)__popcll(mG));
+  const int tile0 = blockIdx.x * kClassifyTile;
+  uint32_t cE = 0, cG = 0;
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int i = tile0 + it * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    const int8_t pt = valid ? ptype[i] : (int8_t)-1;
+    const bool isE = valid && (pt == 0 || pt == 1);
+    cE += (uint32_t)__popcll(__ballot(isE));
+    cG += (uint32_t)__popcll(__ballot(valid && !isE));
   }
-  baseE = __shfl(baseE, 0);
-  baseG = __shfl(baseG, 0);
-  if (isE) queue[baseE + __popcll(mE & below)] = (uint32_t)i;
-  if (isG) queue[(uint32_t)n - 1u - (baseG + __popcll(mG & below))] = (uint32_t)i;
+  if (lane == 0) {
+    sE[wave] = cE;
+    sG[wave] = cG;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tE = 0, tG = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      const uint32_t e = sE[w], g = sG[w];
+      sE[w] = tE;
+      sG[w] = tG;
+      tE += e;
+      tG += g;
+    }
+    sBase[0] = tE ? atomicAdd(&counters[0], tE) : 0u;
+    sBase[1] = tG ? atomicAdd(&counters[1], tG) : 0u;
+  }
+  __syncthreads();
+  uint32_t pE = sBase[0] + sE[wave], pG = sBase[1] + sG[wave];
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int i = tile0 + it * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    const int8_t pt = valid ? ptype[i] : (int8_t)-1;
+    const bool isE = valid && (pt == 0 || pt == 1);
+    const bool isG = valid && !isE;
+    const uint64_t mE = __ballot(isE), mG = __ballot(isG);
+    if (isE) queue[pE + __popcll(mE & below)] = (uint32_t)i;
+    if (isG) queue[(uint32_t)n - 1u - (pG + __popcll(mG & below))] = (uint32_t)i;
+    pE += (uint32_t)__popcll(mE);
+    pG += (uint32_t)__popcll(mG);
+  }
 }
 
+template <typename R> HD void WriteSlot(const hepem_step_out& res, int n, int i, int s, const Sec<R>* sec) {
+  if (sec) {
+    res.sec_ptype[s * n + i] = (int8_t)sec->ptype;
+    res.sec_ekin[s * n + i] = sec->ekin;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = (double)sec->dir[c];
+  } else {
+    res.sec_ptype[s * n + i] = -1;
+    res.sec_ekin[s * n + i] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = 0.0;
+  }
+}
 template <typename R>
 HD void WriteSecondaries(const hepem_step_out& res, int n, int i, const SecList<R>& sl) {
-  int ns = 0;
-  const Sec<R>* list[4];
-  for (int k = 0; k < sl.nel; ++k) list[ns++] = &sl.el[k];
-  for (int k = 0; k < sl.ngm; ++k) list[ns++] = &sl.gm[k];
-  res.nsec[i] = ns;
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    if (s < ns) {
-      res.sec_ptype[s * n + i] = (int8_t)list[s]->ptype;
-      res.sec_ekin[s * n + i] = list[s]->ekin;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = (double)list[s]->dir[c];
-    } else {
-      res.sec_ptype[s * n + i] = -1;
-      res.sec_ekin[s * n + i] = 0.0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = 0.0;
-    }
-  }
+  res.nsec[i] = sl.ns;
+  WriteSlot<R>(res, n, i, 0, sl.ns > 0 ? &sl.s0 : nullptr);
+  WriteSlot<R>(res, n, i, 1, sl.ns > 1 ? &sl.s1 : nullptr);
 }
 
 // A track is stepped only with a valid couple index and a finite, positive
@@ -87,7 +118,7 @@ HD bool RejectTrack(const DevTables<R>& T, const hepem_tracks& in, hepem_tracks&
   res.winner[i] = -99;
   res.stopped[i] = 1;
   SecList<R> sl;
-  sl.nel = sl.ngm = 0;
+  sl.ns = 0;
   WriteSecondaries(res, n, i, sl);
   if (out.ekin != in.ekin) {
     out.ptype[i] = in.ptype[i];
@@ -147,7 +178,7 @@ HD void StepElectron(const DevTables<R>& T, uint64_t seed, const hepem_tracks& i
   Rng rng;
   rng.init(seed, (uint32_t)in.id[i], in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
   SecList<R> sl;
-  sl.nel = sl.ngm = 0;
+  sl.ns = 0;
 
   ElectronHowFar(T, el, rng);
   ElectronPerform(T, el, sl, rng);
@@ -195,7 +226,7 @@ HD void StepGamma(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, 
   Rng rng;
   rng.init(seed, (uint32_t)in.id[i], in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
   SecList<R> sl;
-  sl.nel = sl.ngm = 0;
+  sl.ns = 0;
   R pe = (R)0.0;
   // G4HepEmGammaManager::HowFar (G4HepEmGammaManager.icc:27-48)
   if (t.nial[0] <= (R)0.0) t.nial[0] = -vlog((R)rng.flat());
@@ -210,7 +241,7 @@ HD void StepGamma(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, 
     SampleInteraction(T, t, pe, (R)rng.flat());
     if (t.winner != 3) {
       t.nial[0] -= t.gstep / t.mfp[0];
-      t.nial[t.winner] = (R)-1.0;
+      ResetNumIALeft(t.nial, t.winner);
       switch (t.winner) {
         case 0: ConvPerform(T, t, sl, rng); break;
         case 1: ComptonPerform(t, sl, rng); break;
