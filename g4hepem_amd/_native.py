@@ -42,6 +42,13 @@ class StepOut(C.Structure):
                 ("stopped", P), ("nsec", P), ("sec_ptype", P), ("sec_ekin", P), ("sec_dir", P)]
 
 
+class HowFarState(C.Structure):
+    """hepem_howfar_state: HowFar -> Perform hand-over of one step."""
+    _fields_ = [("n", C.c_int32), ("geom_step", P), ("range", P), ("mfp", P), ("winner", P),
+                ("msc_lambtr1", P), ("msc_true_step", P), ("msc_zpath", P), ("msc_par1", P),
+                ("msc_par2", P), ("msc_par3", P), ("msc_active", P), ("msc_displace", P), ("pe_xsec", P)]
+
+
 class BatchDesc(C.Structure):
     _fields_ = [("inp", Tracks), ("out", Tracks), ("res", StepOut)]
 
@@ -59,6 +66,13 @@ OUT_FIELDS = {
     "true_step": ("float64", 1), "geom_step": ("float64", 1), "edep": ("float64", 1),
     "displacement": ("float64", 3), "winner": ("int32", 1), "stopped": ("uint8", 1),
     "nsec": ("int32", 1), "sec_ptype": ("int8", 2), "sec_ekin": ("float64", 2), "sec_dir": ("float64", 6),
+}
+
+HOWFAR_FIELDS = {
+    "geom_step": ("float64", 1), "range": ("float64", 1), "mfp": ("float64", 4), "winner": ("int32", 1),
+    "msc_lambtr1": ("float64", 1), "msc_true_step": ("float64", 1), "msc_zpath": ("float64", 1),
+    "msc_par1": ("float64", 1), "msc_par2": ("float64", 1), "msc_par3": ("float64", 1),
+    "msc_active": ("uint8", 1), "msc_displace": ("uint8", 1), "pe_xsec": ("float64", 1),
 }
 
 _lib = None
@@ -88,10 +102,16 @@ def lib() -> C.CDLL:
     L.hepem_step_rejected.argtypes = [vp, C.POINTER(C.c_uint32), vp]
     L.hepem_step_batched.argtypes = [vp, u64, C.POINTER(BatchDesc), i32, vp]
     L.hepem_tracks_reset_state.argtypes = [C.POINTER(Tracks), vp]
+    split = ("hepem_howfar", "hepem_electron_howfar", "hepem_gamma_howfar")
+    for name in split:
+        getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), vp, vp]
+    for name in ("hepem_perform", "hepem_electron_perform", "hepem_gamma_perform"):
+        getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), C.POINTER(StepOut), vp, vp]
     for name in ("hepem_tables_build", "hepem_tables_free", "hepem_tables_serialize", "hepem_ctx_create",
                  "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_step",
                  "hepem_step_prepare", "hepem_step_run", "hepem_step_rejected",
-                 "hepem_step_batched", "hepem_tracks_reset_state"):
+                 "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",
+                 "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 
 from . import _native as N
-from .tracks import StepOutSoA, TrackSoA
+from .tracks import HowFarSoA, StepOutSoA, TrackSoA
 
 
 def _stream_handle(stream=None) -> int:
@@ -59,6 +59,24 @@ class Context:
         si, so, rr = tin.struct(), tout.struct(), res.struct()
         N.check("hepem_step_run", N.lib().hepem_step_run(self.h, C.c_uint64(seed), C.byref(si), C.byref(so),
                                                           C.byref(rr), work.data_ptr(), _stream_handle(stream)))
+
+    def howfar(self, seed: int, t: TrackSoA, hs: HowFarSoA, work=None, stream=None, kind: str = "mixed") -> None:
+        """hepem_howfar (kind "mixed"), hepem_electron_howfar ("electron") or
+        hepem_gamma_howfar ("gamma"): updates `t` in place and fills `hs`."""
+        name = {"mixed": "hepem_howfar", "electron": "hepem_electron_howfar", "gamma": "hepem_gamma_howfar"}[kind]
+        st, sh = t.struct(), hs.struct()
+        wp = work.data_ptr() if work is not None else None
+        N.check(name, getattr(N.lib(), name)(self.h, C.c_uint64(seed), C.byref(st), C.byref(sh), wp,
+                                               _stream_handle(stream)))
+
+    def perform(self, seed: int, t: TrackSoA, hs: HowFarSoA, res: StepOutSoA, work=None, stream=None,
+                kind: str = "mixed") -> None:
+        """hepem_perform / hepem_electron_perform / hepem_gamma_perform."""
+        name = {"mixed": "hepem_perform", "electron": "hepem_electron_perform", "gamma": "hepem_gamma_perform"}[kind]
+        st, sh, rr = t.struct(), hs.struct(), res.struct()
+        wp = work.data_ptr() if work is not None else None
+        N.check(name, getattr(N.lib(), name)(self.h, C.c_uint64(seed), C.byref(st), C.byref(sh), C.byref(rr), wp,
+                                               _stream_handle(stream)))
 
     @staticmethod
     def rejected(work, stream=None) -> int:

@@ -1098,17 +1098,16 @@ template <typename R> HD bool CheckDelta(const DevTables<R>& T, Trk<R>& t, R rnd
   return mxsec <= (R)0.0 || rnd > mxsec * PickMFP(t.mfp, iDProc);
 }
 
-template <typename R> HD void PerformDiscrete(const DevTables<R>& T, Trk<R>& t, SecList<R>& sl, Rng& rng) {  // :425-459
+// PerformDiscrete (:425-441) up to the final-state sampling: the
+// interaction-length reset and the delta-interaction test. Returns the
+// process whose final state is to be sampled (0 ioni, 1 brem, 2 annihilation,
+// 3 nuclear: no final state) or -1.
+template <typename R> HD int ElectronSelectDiscrete(const DevTables<R>& T, Trk<R>& t, Rng& rng) {
   const int iDProc = t.winner;
-  if (iDProc < 0 || t.onBoundary) return;
+  if (iDProc < 0 || t.onBoundary) return -1;
   ResetNumIALeft(t.nial, iDProc);
-  if (CheckDelta(T, t, (R)rng.flat())) return;
-  switch (iDProc) {
-    case 0: IoniPerform(T, t, sl, rng); break;
-    case 1: BremPerform(T, t, sl, rng, t.ekin < T.bremModelLim); break;
-    case 2: AnnihilateInFlight(t, sl, rng); break;
-    default: break;
-  }
+  if (CheckDelta(T, t, (R)rng.flat())) return -1;
+  return iDProc;
 }
 
 // G4HepEmElectronManager::HowFar(data, pars, tlData) (:35-45, 166-169)
@@ -1120,13 +1119,15 @@ template <typename R> HD void ElectronHowFar(const DevTables<R>& T, ElState<R>& 
   HowFarToMSC(T, el, rng);
 }
 
-// G4HepEmElectronManager::Perform(data, pars, tlData) (:375-405, 461-483)
-template <typename R> HD void ElectronPerform(const DevTables<R>& T, ElState<R>& el, SecList<R>& sl, Rng& rng) {
+// G4HepEmElectronManager::Perform(data, pars, tlData) (:461-483) up to
+// PerformDiscrete: PerformContinuous (:375-405). Returns 1 if the track
+// stopped in the step, 0 if it goes on to PerformDiscrete and -1 for a zero
+// geometrical step (nothing else happens in the step).
+template <typename R> HD int ElectronPerformContinuous(const DevTables<R>& T, ElState<R>& el, Rng& rng) {
   Trk<R>& t = el.t;
   t.edep = 0;
   el.pstep = t.gstep;
-  if (t.gstep <= (R)0.) return;
-  // PerformContinuous
+  if (t.gstep <= (R)0.) return -1;
   el.preEkin = t.ekin;
   el.preLEkin = t.logE();
   UpdatePStepLength(el);
@@ -1139,11 +1140,7 @@ template <typename R> HD void ElectronPerform(const DevTables<R>& T, ElState<R>&
       stopped = SampleLossFluctuations(T, el, rng);
     }
   }
-  if (stopped) {
-    if (!t.isEl) AnnihilateAtRest(sl, rng);
-    return;
-  }
-  PerformDiscrete(T, t, sl, rng);
+  return stopped ? 1 : 0;
 }
 
 // ---- G4HepEmGammaManager.icc ------------------------------------------------------------------

@@ -345,6 +345,7 @@ bool ResOk(const hepem_step_out* r) {
 }
 }  // namespace
 
+// work layout: 64 counters (256 bytes) | queue[n]
 extern "C" uint64_t hepem_step_work_bytes(int32_t n) { return 256 + 4ull * (uint64_t)(n > 0 ? n : 0); }
 
 extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
@@ -369,7 +370,7 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
   if (n == 0) return HEPEM_OK;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   uint32_t* counters = static_cast<uint32_t*>(work);
-  uint32_t* queue = counters + 64;
+  const uint32_t* queue = counters + 64;
   const int grid = (n + kBlock - 1) / kBlock;
   if (c->precision == HEPEM_FP64)
     hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, queue, counters);
@@ -379,26 +380,101 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
   return HEPEM_OK;
 }
 
-extern "C" int hepem_step(const hepem_ctx* cc, uint64_t seed, const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res,
-                          void* work, void* stream) {
+namespace {
+// The context's own work buffer, grown to hold a queue of n slots.
+int EnsureWork(const hepem_ctx* cc, int n, void*& work) {
+  if (work) return HEPEM_OK;
   hepem_ctx* c = const_cast<hepem_ctx*>(cc);
+  const uint64_t need = hepem_step_work_bytes(n);
+  if (c->workBytes < need) {
+    if (c->work) HIP_OK(hipFree(c->work));
+    c->work = nullptr;
+    c->workBytes = 0;
+    if (hipMalloc(&c->work, need) != hipSuccess) return HEPEM_ERR_ALLOC;
+    c->workBytes = need;
+  }
+  work = c->work;
+  return HEPEM_OK;
+}
+}  // namespace
+
+extern "C" int hepem_step(const hepem_ctx* c, uint64_t seed, const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res,
+                          void* work, void* stream) {
   if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res) || out->n != in->n) return HEPEM_ERR_INVALID;
   const int n = in->n;
   if (n == 0) return HEPEM_OK;
-  if (!work) {
-    const uint64_t need = hepem_step_work_bytes(n);
-    if (c->workBytes < need) {
-      if (c->work) HIP_OK(hipFree(c->work));
-      c->work = nullptr;
-      c->workBytes = 0;
-      if (hipMalloc(&c->work, need) != hipSuccess) return HEPEM_ERR_ALLOC;
-      c->workBytes = need;
-    }
-    work = c->work;
-  }
-  const int rc = hepem_step_prepare(c, in, work, stream);
+  int rc = EnsureWork(c, n, work);
+  if (rc != HEPEM_OK) return rc;
+  rc = hepem_step_prepare(c, in, work, stream);
   if (rc != HEPEM_OK) return rc;
   return hepem_step_run(c, seed, in, out, res, work, stream);
+}
+
+namespace {
+bool HowFarOk(const hepem_howfar_state* h, int n) {
+  return h && h->n == n && h->geom_step && h->range && h->mfp && h->winner && h->msc_lambtr1 && h->msc_true_step &&
+         h->msc_zpath && h->msc_par1 && h->msc_par2 && h->msc_par3 && h->msc_active && h->msc_displace && h->pe_xsec;
+}
+
+// HowFar (perform == false) or Perform of a batch of slots of one KIND; a mixed
+// batch is ordered by the particle-type queue in `work` (built here for HowFar).
+template <int KIND>
+int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs, hepem_step_out* res,
+              void* work, void* stream, bool perform) {
+  if (!c || !TracksOk(t) || !HowFarOk(hs, t ? t->n : -1) || (perform && !ResOk(res))) return HEPEM_ERR_INVALID;
+  const int n = t->n;
+  if (n == 0) return HEPEM_OK;
+  int rc = EnsureWork(c, n, work);
+  if (rc != HEPEM_OK) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t* counters = static_cast<uint32_t*>(work);
+  const uint32_t* queue = KIND == kMixed ? counters + 64 : nullptr;
+  if (!perform) {
+    if (KIND == kMixed) {
+      rc = hepem_step_prepare(c, t, work, stream);
+      if (rc != HEPEM_OK) return rc;
+    } else {
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters, 64, 0u);
+    }
+  }
+  const int grid = (n + kBlock - 1) / kBlock;
+  const bool fp64 = c->precision == HEPEM_FP64;
+  if (!perform && fp64)
+    hipLaunchKernelGGL((k_howfar<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, queue, counters);
+  else if (!perform)
+    hipLaunchKernelGGL((k_howfar<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, queue, counters);
+  else if (fp64)
+    hipLaunchKernelGGL((k_perform<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue, counters);
+  else
+    hipLaunchKernelGGL((k_perform<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue, counters);
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+}  // namespace
+
+extern "C" int hepem_howfar(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs, void* work,
+                            void* stream) {
+  return SplitCall<kMixed>(c, seed, t, hs, nullptr, work, stream, false);
+}
+extern "C" int hepem_perform(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                             hepem_step_out* res, void* work, void* stream) {
+  return SplitCall<kMixed>(c, seed, t, hs, res, work, stream, true);
+}
+extern "C" int hepem_electron_howfar(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs, void* work,
+                                     void* stream) {
+  return SplitCall<kOnlyElectrons>(c, seed, t, hs, nullptr, work, stream, false);
+}
+extern "C" int hepem_electron_perform(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                                      hepem_step_out* res, void* work, void* stream) {
+  return SplitCall<kOnlyElectrons>(c, seed, t, hs, res, work, stream, true);
+}
+extern "C" int hepem_gamma_howfar(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs, void* work,
+                                  void* stream) {
+  return SplitCall<kOnlyGammas>(c, seed, t, hs, nullptr, work, stream, false);
+}
+extern "C" int hepem_gamma_perform(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                                   hepem_step_out* res, void* work, void* stream) {
+  return SplitCall<kOnlyGammas>(c, seed, t, hs, res, work, stream, true);
 }
 
 extern "C" int hepem_step_rejected(const void* work, uint32_t* nrej, void* stream) {

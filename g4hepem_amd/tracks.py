@@ -11,7 +11,8 @@ import math
 
 import numpy as np
 
-from ._native import OUT_FIELDS, TRACK_FIELDS, StepOut, Tracks, ELECTRON, POSITRON, GAMMA
+from ._native import (HOWFAR_FIELDS, OUT_FIELDS, TRACK_FIELDS, HowFarState, StepOut, Tracks, ELECTRON, POSITRON,
+                      GAMMA)
 
 
 def _alloc(kind: str, dtype: str, count: int, device=None):
@@ -78,6 +79,17 @@ class StepOutSoA(SoA):
         for name in self.FIELDS:
             setattr(r, name, _ptr(self.a[name]))
         return r
+
+
+class HowFarSoA(SoA):
+    FIELDS = HOWFAR_FIELDS
+
+    def struct(self) -> HowFarState:
+        h = HowFarState()
+        h.n = self.n
+        for name in self.FIELDS:
+            setattr(h, name, _ptr(self.a[name]))
+        return h
 
 
 def reset_state_numpy(t: dict, n: int) -> None:

@@ -106,6 +106,31 @@ typedef struct hepem_step_out {
   double*   sec_dir;           /* [3][2][n]: component c of slot s at (c*2+s)*n+i */
 } hepem_step_out;
 
+/* Step-local state that a HowFar call leaves for the Perform call of the same
+ * step: the fields of G4HepEmElectronTrack (fRange, fPStepLength:
+ * G4HepEmElectronTrack.hh:51-58), G4HepEmTrack (fGStepLength, fMFPs,
+ * fWinnerProcessIndex), G4HepEmMSCTrackData (G4HepEmMSCTrackData.hh:112-131)
+ * and G4HepEmGammaTrack (fPEmxSec, G4HepEmGammaTrack.hh:32-45) that HowFar
+ * computes and Perform reads. */
+typedef struct hepem_howfar_state {
+  int32_t   n;
+  double*   geom_step;         /* HowFar: proposed geometrical step [mm]; the
+                                  caller's geometry may shorten it before Perform */
+  double*   range;             /* e-/e+: restricted range at the pre-step point   */
+  double*   mfp;               /* [4][n] macroscopic mean free paths [mm]
+                                  (gamma: [0] = total)                            */
+  int32_t*  winner;            /* process that limited the physics step           */
+  double*   msc_lambtr1;       /* e-/e+ Urban MSC: first transport mfp            */
+  double*   msc_true_step;     /*   true step length                              */
+  double*   msc_zpath;         /*   geometrical (z) path length                   */
+  double*   msc_par1;          /*   true<->geometrical conversion parameters      */
+  double*   msc_par2;
+  double*   msc_par3;
+  uint8_t*  msc_active;        /*   MSC active in this step                       */
+  uint8_t*  msc_displace;      /*   lateral displacement to be sampled            */
+  double*   pe_xsec;           /* gamma: photoelectric macroscopic cross section  */
+} hepem_howfar_state;
+
 /* One batch of the batched launch. */
 typedef struct hepem_batch_desc {
   hepem_tracks   in;
@@ -121,6 +146,7 @@ const char* hepem_status_string(int status);
 /* Device bytes held by the context's table image. */
 int  hepem_ctx_table_bytes(const hepem_ctx* ctx, uint64_t* bytes);
 
+
 /* ---- config-1 style lookups ---------------------------------------------
  * For n queries (imc, E, lnE) of particle `ptype` (e- or e+) writes
  * out[0*n+i] restricted dE/dx, out[1*n+i] range, out[2*n+i] inverse range of
@@ -132,16 +158,17 @@ int  hepem_lookup_eloss_xsec(const hepem_ctx* ctx, int ptype, int32_t n,
 
 /* ---- stepping ------------------------------------------------------------
  * One step of every track of the batch. `out` may alias `in`. `work` must be
- * at least hepem_step_work_bytes(n) bytes of device memory (queues and
- * counters); pass NULL to let the context use its own growable buffer
- * (then calls on one context must not overlap). */
+ * at least hepem_step_work_bytes(n) bytes of device memory (counters and the
+ * particle-type queue);
+ * pass NULL to let the context use its own buffer (then calls on one context
+ * must not overlap). */
 int  hepem_step(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
                 hepem_tracks* out, hepem_step_out* res, void* work, void* hip_stream);
 uint64_t hepem_step_work_bytes(int32_t n);
 
-/* The two phases of hepem_step, for callers that time or reuse them:
- * hepem_step_prepare compacts the track slots of `in` into the per-particle
- * queue held in `work` (wave-level sort by particle type); hepem_step_run
+/* The two phases of hepem_step, for callers that time them separately:
+ * hepem_step_prepare clears the step counters and compacts the track slots
+ * by particle type into the queue held in `work` (e-/e+ first); hepem_step_run
  * performs the physics step of every queued slot. `work` is required here.
  * hepem_step == prepare + run on the same stream. */
 int  hepem_step_prepare(const hepem_ctx* ctx, const hepem_tracks* in, void* work, void* hip_stream);
@@ -151,6 +178,40 @@ int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
  * non-positive/non-finite energy; such tracks get winner -99 and are left
  * unchanged). Synchronises `hip_stream`. */
 int  hepem_step_rejected(const void* work, uint32_t* nrejected, void* hip_stream);
+
+/* ---- HowFar and Perform as separate calls, for callers with geometry -----
+ * hepem_howfar: G4HepEmElectronManager::HowFar (G4HepEmElectronManager.hh:71;
+ * .icc:35-45) for the e-/e+ and G4HepEmGammaManager::HowFar
+ * (G4HepEmGammaManager.hh:32; .icc:27-48) for the gammas of a mixed batch.
+ * It reads the pre-step `on_boundary` and `safety`, updates the persistent
+ * state of `t` in place (interaction lengths, MSC state, RNG) and fills `hs`.
+ * The caller then moves the tracks: it may shorten hs->geom_step and sets
+ * t->on_boundary to the post-step status.
+ * hepem_perform: G4HepEmElectronManager::Perform (.hh:222; .icc:461-483) and,
+ * for gammas, the boundary update or SelectInteraction + Perform of
+ * G4HepEmGammaManager (.hh:38-55; .icc:54-105,173-219), as driven by
+ * G4HepEmTrackingManager.cc:999-1000,1092-1108. Updates `t` in place and fills
+ * `res`. With hs->geom_step and on_boundary left unchanged, howfar + perform
+ * equals hepem_step bit for bit.
+ * `work` (hepem_step_work_bytes(n) bytes, or NULL for the context's own
+ * buffer) holds the particle-type queue built by hepem_howfar; pass the same
+ * buffer to the hepem_perform of the same step. Tracks that hepem_howfar
+ * rejects get hs->winner = -99 and hs->geom_step = 0. */
+int  hepem_howfar(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                  void* work, void* hip_stream);
+int  hepem_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                   hepem_step_out* res, void* work, void* hip_stream);
+/* The same for batches of one kind: every slot must hold an e-/e+
+ * (electron calls) or a gamma (gamma calls); other slots are rejected. No
+ * queue is built; `work` only holds the rejection counter (NULL allowed). */
+int  hepem_electron_howfar(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                           void* work, void* hip_stream);
+int  hepem_electron_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                            hepem_step_out* res, void* work, void* hip_stream);
+int  hepem_gamma_howfar(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                        void* work, void* hip_stream);
+int  hepem_gamma_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                         hepem_step_out* res, void* work, void* hip_stream);
 
 /* Many independent batches in one persistent launch. `descs` is a host array.
  * Results are bit-identical to calling hepem_step on each batch. */

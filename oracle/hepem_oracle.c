@@ -1799,8 +1799,22 @@ typedef struct step_job {
   const hepem_tracks* in;
   hepem_tracks* out;
   hepem_step_out* res;
+  const double* geom_limit;
   int begin, end;
 } step_job;
+
+/* The caller's geometry between HowFar and Perform (G4HepEmTrackingManager.cc:
+   262-300,1000-1092): the geometrical step is limited to geom_limit[i]; a
+   limited step ends on a boundary, any other step does not. */
+static void apply_geometry(const step_job* J, int i, otrack* t) {
+  if (!J->geom_limit) return;
+  if (J->geom_limit[i] < t->gstep) {
+    t->gstep = J->geom_limit[i];
+    t->on_boundary = 1;
+  } else {
+    t->on_boundary = 0;
+  }
+}
 
 static void step_one(const step_job* J, int i) {
   const struct G4HepEmData* d = J->d;
@@ -1882,6 +1896,7 @@ static void step_one(const step_job* J, int i) {
     /* G4HepEmTrackingManager.cc:999-1000,1092-1108: HowFar, (no geometry),
        then UpdateNumIALeft on a boundary or SelectInteraction + Perform */
     GammaHowFar(d, &tl);
+    apply_geometry(J, i, t);
     if (t->on_boundary) {
       t->nial[0] -= t->gstep / t->mfp[0];
       t->edep = 0.0;
@@ -1894,6 +1909,7 @@ static void step_one(const step_job* J, int i) {
     stopped = t->ekin <= 0.0;
   } else {
     ElectronHowFar(d, p, &tl);
+    apply_geometry(J, i, t);
     ElectronPerform(d, p, &tl);
     trueStep = tl.el.pstep;
     geomStep = t->gstep;
@@ -1920,8 +1936,8 @@ static void step_one(const step_job* J, int i) {
     out->id[i] = in->id[i];
     out->imc[i] = in->imc[i];
     out->safety[i] = in->safety[i];
-    out->on_boundary[i] = in->on_boundary[i];
   }
+  out->on_boundary[i] = (uint8_t)t->on_boundary;
   /* results */
   res->true_step[i] = trueStep;
   res->geom_step[i] = geomStep;
@@ -1958,6 +1974,11 @@ static void* step_worker(void* arg) {
 
 int oracle_step(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed, const hepem_tracks* in,
                 hepem_tracks* out, hepem_step_out* res, int nthreads) {
+  return oracle_step_geom(d, p, seed, in, out, res, NULL, nthreads);
+}
+
+int oracle_step_geom(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed, const hepem_tracks* in,
+                     hepem_tracks* out, hepem_step_out* res, const double* geom_limit, int nthreads) {
   if (!d || !p || !in || !out || !res || in->n < 0) return -1;
   const int n = in->n;
   if (nthreads < 1) nthreads = 1;
@@ -1968,7 +1989,7 @@ int oracle_step(const struct G4HepEmData* d, const struct G4HepEmParameters* p, 
   const int chunk = (n + nthreads - 1) / nthreads;
   for (int k = 0; k < nthreads; ++k) {
     jobs[k].d = d; jobs[k].p = p; jobs[k].seed = seed;
-    jobs[k].in = in; jobs[k].out = out; jobs[k].res = res;
+    jobs[k].in = in; jobs[k].out = out; jobs[k].res = res; jobs[k].geom_limit = geom_limit;
     jobs[k].begin = k * chunk < n ? k * chunk : n;
     jobs[k].end = (k + 1) * chunk < n ? (k + 1) * chunk : n;
   }

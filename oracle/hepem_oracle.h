@@ -47,6 +47,12 @@ int oracle_probe_gamma(const struct G4HepEmData* d, int n, const int32_t* imc, c
  * device API). out may alias in. Uses up to `nthreads` host threads. */
 int oracle_step(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed,
                 const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res, int nthreads);
+/* The same with a geometry between HowFar and Perform: track i's geometrical
+ * step is limited to geom_limit[i] (NULL: no limit); a limited step ends on a
+ * boundary (out->on_boundary = 1), any other step does not (0). */
+int oracle_step_geom(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed,
+                     const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res, const double* geom_limit,
+                     int nthreads);
 
 #ifdef __cplusplus
 }

@@ -63,8 +63,8 @@ class _Base:
         self._f_probe_sel.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         self._f_probe_gm = getattr(self.L, p + "probe_gamma")
         self._f_probe_gm.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
-        self._f_step = getattr(self.L, p + "step")
-        self._f_step.argtypes = [vp, vp, C.c_uint64, C.POINTER(_Tracks), C.POINTER(_Tracks), C.POINTER(_Out), C.c_int]
+        self._f_step = getattr(self.L, p + "step_geom")
+        self._f_step.argtypes = [vp, vp, C.c_uint64, C.POINTER(_Tracks), C.POINTER(_Tracks), C.POINTER(_Out), vp, C.c_int]
         for f in (self._f_lookup, self._f_probe_el, self._f_probe_sel, self._f_probe_gm, self._f_step):
             f.restype = C.c_int
 
@@ -101,9 +101,11 @@ class _Base:
         assert rc == 0
         return xs, pe, pid
 
-    def step(self, tables, seed, tracks: dict, nthreads: int = 1):
+    def step(self, tables, seed, tracks: dict, nthreads: int = 1, geom_limit=None):
         """One step of every track (host dict of planar arrays). Returns
-        (new track state dict, results dict)."""
+        (new track state dict, results dict). With `geom_limit` (n doubles) a
+        geometry between HowFar and Perform cuts each geometrical step there;
+        the post-step boundary flag is returned in on_boundary."""
         n = len(tracks["ekin"])
         tin = {k: np.ascontiguousarray(tracks[k], dtype=dt) for k, (dt, _) in TRACK_FIELDS.items()}
         tout = {k: np.zeros(c * n, dtype=dt) for k, (dt, c) in TRACK_FIELDS.items()}
@@ -115,7 +117,9 @@ class _Base:
             setattr(so, k, tout[k].ctypes.data)
         for k in OUT_FIELDS:
             setattr(rr, k, res[k].ctypes.data)
-        rc = self._f_step(tables.data, tables.pars, C.c_uint64(seed), C.byref(si), C.byref(so), C.byref(rr), int(nthreads))
+        gl = None if geom_limit is None else np.ascontiguousarray(geom_limit, dtype=np.float64)
+        rc = self._f_step(tables.data, tables.pars, C.c_uint64(seed), C.byref(si), C.byref(so), C.byref(rr),
+                          None if gl is None else gl.ctypes.data, int(nthreads))
         assert rc == 0
         # gamma tracks do not carry MSC state: keep the input values there
         g = tin["ptype"] == 2

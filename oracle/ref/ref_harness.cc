@@ -143,8 +143,20 @@ namespace {
 inline G4HepEmData* R(const mine::G4HepEmData* d) { return reinterpret_cast<G4HepEmData*>(const_cast<mine::G4HepEmData*>(d)); }
 inline G4HepEmParameters* R(const mine::G4HepEmParameters* p) { return reinterpret_cast<G4HepEmParameters*>(const_cast<mine::G4HepEmParameters*>(p)); }
 
+// the caller's geometry between HowFar and Perform: a step longer than
+// geom_limit[i] is cut there and ends on a boundary, any other does not
+void ApplyGeometry(const double* geom_limit, int i, G4HepEmTrack* t) {
+  if (!geom_limit) return;
+  if (geom_limit[i] < t->GetGStepLength()) {
+    t->SetGStepLength(geom_limit[i]);
+    t->SetOnBoundary(true);
+  } else {
+    t->SetOnBoundary(false);
+  }
+}
+
 void StepRange(G4HepEmData* d, G4HepEmParameters* p, uint64_t seed, const mine::hepem_tracks* in, mine::hepem_tracks* out,
-               mine::hepem_step_out* res, int begin, int end) {
+               mine::hepem_step_out* res, const double* geom_limit, int begin, int end) {
   const int n = in->n;
   G4HepEmTLData tl;
   for (int i = begin; i < end; ++i) {
@@ -184,6 +196,7 @@ void StepRange(G4HepEmData* d, G4HepEmParameters* p, uint64_t seed, const mine::
     for (int k = 0; k < 4; ++k) t->SetNumIALeft(in->num_ia_left[k * n + i], k);
     if (ptype == mine::HEPEM_GAMMA) {
       G4HepEmGammaManager::HowFar(d, p, &tl);
+      ApplyGeometry(geom_limit, i, t);
       if (t->GetOnBoundary()) {
         G4HepEmGammaManager::UpdateNumIALeft(t);
         t->SetEnergyDeposit(0.0);
@@ -195,6 +208,7 @@ void StepRange(G4HepEmData* d, G4HepEmParameters* p, uint64_t seed, const mine::
       trueStep = geomStep = t->GetGStepLength();
     } else {
       G4HepEmElectronManager::HowFar(d, p, &tl);
+      ApplyGeometry(geom_limit, i, t);
       G4HepEmElectronManager::Perform(d, p, &tl);
       G4HepEmElectronTrack* et = tl.GetPrimaryElectronTrack();
       G4HepEmMSCTrackData* m = et->GetMSCTrackData();
@@ -217,8 +231,8 @@ void StepRange(G4HepEmData* d, G4HepEmParameters* p, uint64_t seed, const mine::
       out->id[i] = in->id[i];
       out->imc[i] = in->imc[i];
       out->safety[i] = in->safety[i];
-      out->on_boundary[i] = in->on_boundary[i];
     }
+    out->on_boundary[i] = t->GetOnBoundary() ? 1 : 0;
     res->true_step[i] = trueStep;
     res->geom_step[i] = geomStep;
     res->edep[i] = t->GetEnergyDeposit();
@@ -318,8 +332,16 @@ int ref_probe_gamma(const mine::G4HepEmData* dd, int n, const int32_t* imc, cons
   return 0;
 }
 
+int ref_step_geom(const mine::G4HepEmData* dd, const mine::G4HepEmParameters* pp, uint64_t seed, const mine::hepem_tracks* in,
+                  mine::hepem_tracks* out, mine::hepem_step_out* res, const double* geom_limit, int nthreads);
+
 int ref_step(const mine::G4HepEmData* dd, const mine::G4HepEmParameters* pp, uint64_t seed, const mine::hepem_tracks* in,
              mine::hepem_tracks* out, mine::hepem_step_out* res, int nthreads) {
+  return ref_step_geom(dd, pp, seed, in, out, res, nullptr, nthreads);
+}
+
+int ref_step_geom(const mine::G4HepEmData* dd, const mine::G4HepEmParameters* pp, uint64_t seed, const mine::hepem_tracks* in,
+                  mine::hepem_tracks* out, mine::hepem_step_out* res, const double* geom_limit, int nthreads) {
   G4HepEmData* d = R(dd);
   G4HepEmParameters* p = R(pp);
   const int n = in->n;
@@ -329,7 +351,7 @@ int ref_step(const mine::G4HepEmData* dd, const mine::G4HepEmParameters* pp, uin
   const int chunk = (n + nthreads - 1) / nthreads;
   for (int k = 0; k < nthreads; ++k) {
     const int b = std::min(n, k * chunk), e = std::min(n, (k + 1) * chunk);
-    th.emplace_back(StepRange, d, p, seed, in, out, res, b, e);
+    th.emplace_back(StepRange, d, p, seed, in, out, res, geom_limit, b, e);
   }
   for (auto& t : th) t.join();
   return 0;

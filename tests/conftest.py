@@ -8,6 +8,7 @@ ROOT = Path(__file__).resolve().parents[1]
 sys.path.insert(0, str(ROOT))
 
 GOLDEN = ROOT / "tests" / "golden" / "golden_v1.npz"
+GOLDEN_GEOM = ROOT / "tests" / "golden" / "golden_geom_v1.npz"
 SET1 = ["G4_Pb", "G4_lAr"]
 SET2 = ["G4_PbWO4", "G4_Pb", "G4_lAr", "G4_Si"]
 CUT = 0.7
@@ -21,6 +22,11 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def golden():
     return np.load(GOLDEN, allow_pickle=False)
+
+
+@pytest.fixture(scope="session")
+def golden_geom():
+    return np.load(GOLDEN_GEOM, allow_pickle=False)
 
 
 @pytest.fixture(scope="session")

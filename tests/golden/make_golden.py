@@ -11,7 +11,10 @@ and the per-track Philox streams, and stores inputs and outputs:
   section + interaction sampling over all three energy windows;
 * one HowFar+Perform step of fresh e-/e+/gamma tracks in three energy regimes;
 * a 4-step chain (each step starting from the previous state) with tracks on
-  and off volume boundaries.
+  and off volume boundaries;
+* (golden_geom_v1.npz) a 3-step chain with a geometry between HowFar and
+  Perform: each track's geometrical step is cut at a given distance, a cut
+  step ends on a boundary and the next step starts from there.
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -31,6 +34,7 @@ from g4hepem_amd.tracks import lookup_queries, step_tracks  # noqa: E402
 from oracle.oracle_py import Ref  # noqa: E402
 
 OUT = Path(__file__).resolve().parent / "golden_v1.npz"
+OUT_GEOM = Path(__file__).resolve().parent / "golden_geom_v1.npz"
 SET1 = ["G4_Pb", "G4_lAr"]
 SET2 = ["G4_PbWO4", "G4_Pb", "G4_lAr", "G4_Si"]
 CUT = 0.7
@@ -100,6 +104,28 @@ def main():
 
     np.savez_compressed(OUT, **g)
     print(f"wrote {OUT} ({OUT.stat().st_size / 1024:.0f} KiB); set1 {t1.digest()[:16]} set2 {t2.digest()[:16]}")
+    geometry_vectors(ref, t2)
+
+
+def geometry_vectors(ref, t2, n=1024, nsteps=3):
+    """Steps cut by a geometry: distances log-uniform in [1 um, 5 mm]."""
+    g = {"digest_set2": np.array(t2.digest())}
+    tr = step_tracks(n, 700, len(SET2), lnemin=math.log(1e-2), lnemax=math.log(1e4), first_id=70000)
+    rng = np.random.Generator(np.random.Philox(701))
+    for k, v in tr.items():
+        g[f"gc_in_{k}"] = v
+    cur = tr
+    for s in range(nsteps):
+        lim = np.exp(rng.uniform(math.log(1e-3), math.log(5.0), n))
+        g[f"gc{s}_limit"] = lim
+        cur, res = ref.step(t2, STEP_SEED + 2, cur, 8, geom_limit=lim)
+        for k, v in res.items():
+            g[f"gc{s}_res_{k}"] = v
+        for k in ("ekin", "dir", "num_ia_left", "rng_ctr", "rng_gauss", "rng_has_gauss", "msc_init_range",
+                  "msc_dyn_range_factor", "msc_tlimit_min", "msc_first_step", "on_boundary"):
+            g[f"gc{s}_out_{k}"] = cur[k]
+    np.savez_compressed(OUT_GEOM, **g)
+    print(f"wrote {OUT_GEOM} ({OUT_GEOM.stat().st_size / 1024:.0f} KiB)")
 
 
 if __name__ == "__main__":

@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     from g4hepem_amd import lib
     L = lib()
     names = declared_functions()
-    assert len(names) >= 15, names
+    assert len(names) >= 21, names
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
@@ -45,6 +45,12 @@ def test_null_arguments_are_rejected():
     assert L.hepem_tracks_reset_state(C.byref(t), None) == -1  # null arrays
     assert L.hepem_step_batched(None, 1, None, 0, None) == -1
     assert L.hepem_step_rejected(None, None, None) == -1
+    from g4hepem_amd._native import HowFarState
+    hs = HowFarState()
+    for name in ("hepem_howfar", "hepem_electron_howfar", "hepem_gamma_howfar"):
+        assert getattr(L, name)(None, 1, C.byref(t), C.byref(hs), None, None) == -1, name
+    for name in ("hepem_perform", "hepem_electron_perform", "hepem_gamma_perform"):
+        assert getattr(L, name)(None, 1, C.byref(t), C.byref(hs), C.byref(r), None, None) == -1, name
     assert L.hepem_status_string(-3) == b"capacity exceeded"
     assert L.hepem_step_work_bytes(1000) >= 4000
 

@@ -80,6 +80,50 @@ def test_step_chain_golden(golden, tables2, oracle):
     assert golden["ch1_out_rng_has_gauss"].sum() > 0
 
 
+GEOM_STATE = ("ekin", "dir", "num_ia_left", "rng_ctr", "rng_gauss", "rng_has_gauss", "msc_init_range",
+              "msc_dyn_range_factor", "msc_tlimit_min", "msc_first_step", "on_boundary")
+
+
+def test_geometry_chain_golden(golden_geom, tables2, oracle):
+    """Steps cut by a geometry between HowFar and Perform, chained over 3 steps."""
+    assert tables2.digest() == str(golden_geom["digest_set2"])
+    cur = golden_tracks(golden_geom, "gc_in")
+    for s in range(3):
+        alive = cur["ekin"] > 0
+        lim = golden_geom[f"gc{s}_limit"]
+        cur, res = oracle.step(tables2, STEP_SEED + 2, cur, nthreads=4, geom_limit=lim)
+        for k, (dt, c) in OUT_FIELDS.items():
+            m = np.tile(alive, c)
+            assert same_bits(res[k][m], golden_geom[f"gc{s}_res_{k}"][m]), (s, k)
+        for k in GEOM_STATE:
+            m = np.tile(alive, TRACK_FIELDS[k][1])
+            if k.startswith("msc"):
+                m &= np.tile(cur["ptype"] != 2, TRACK_FIELDS[k][1])
+            assert same_bits(cur[k][m], golden_geom[f"gc{s}_out_{k}"][m]), (s, k)
+        # a cut step ends on the boundary at exactly the cut distance
+        onb = cur["on_boundary"][alive] == 1
+        assert 0.1 < onb.mean() < 0.9, onb.mean()
+        assert np.all(res["geom_step"][alive][onb] == lim[alive][onb])
+        assert np.all(res["geom_step"][alive] <= lim[alive])
+        # no discrete interaction and no secondaries on a boundary step
+        assert np.all(res["nsec"][alive][onb & (cur["ptype"][alive] == 2)] == 0)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference tree not built here")
+def test_oracle_equals_reference_live_geometry(tables2, oracle):
+    from g4hepem_amd.tracks import step_tracks
+    from oracle.oracle_py import Ref
+    ref = Ref()
+    tr = step_tracks(20000, 778, 4, lnemin=math.log(1e-3), lnemax=math.log(1e6))
+    lim = np.exp(np.random.Generator(np.random.Philox(779)).uniform(math.log(1e-4), math.log(20.0), 20000))
+    a_t, a_r = oracle.step(tables2, 4243, tr, 8, geom_limit=lim)
+    b_t, b_r = ref.step(tables2, 4243, tr, 8, geom_limit=lim)
+    for k in OUT_FIELDS:
+        assert same_bits(a_r[k], b_r[k]), k
+    for k in ("ekin", "dir", "num_ia_left", "rng_ctr", "rng_gauss", "on_boundary"):
+        assert same_bits(a_t[k], b_t[k]), k
+
+
 @pytest.mark.skipif(not ref_available(), reason="reference tree not built here")
 def test_oracle_equals_reference_live(tables2, oracle):
     from g4hepem_amd.tracks import step_tracks
