@@ -101,6 +101,12 @@ def lib() -> C.CDLL:
     L.hepem_step_run.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(Tracks), C.POINTER(StepOut), vp, vp]
     L.hepem_step_rejected.argtypes = [vp, C.POINTER(C.c_uint32), vp]
     L.hepem_step_batched.argtypes = [vp, u64, C.POINTER(BatchDesc), i32, vp]
+    L.hepem_batch_plan_create.argtypes = [vp, C.POINTER(BatchDesc), i32, C.POINTER(vp)]
+    L.hepem_batch_plan_destroy.argtypes = [vp]
+    L.hepem_batch_plan_run.argtypes = [vp, u64, vp]
+    L.hepem_batch_plan_tune.argtypes = [vp, u64, vp]
+    L.hepem_batch_plan_shape.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+    L.hepem_batch_plan_rejected.argtypes = [vp, i32, C.POINTER(C.c_uint32), vp]
     L.hepem_tracks_reset_state.argtypes = [C.POINTER(Tracks), vp]
     split = ("hepem_howfar", "hepem_electron_howfar", "hepem_gamma_howfar")
     for name in split:
@@ -111,7 +117,9 @@ def lib() -> C.CDLL:
                  "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_step",
                  "hepem_step_prepare", "hepem_step_run", "hepem_step_rejected",
                  "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",
-                 "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform"):
+                 "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform",
+                 "hepem_batch_plan_create", "hepem_batch_plan_destroy", "hepem_batch_plan_run", "hepem_batch_plan_tune",
+                 "hepem_batch_plan_shape", "hepem_batch_plan_rejected"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

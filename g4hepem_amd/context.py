@@ -99,3 +99,49 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class BatchPlan:
+    """hepem_batch_plan: many track batches of uneven size stepped by one
+    launch sequence (graph-capturable run)."""
+
+    def __init__(self, ctx: Context, batches):
+        """`batches`: sequence of (tin, tout, res) SoA triples on the device."""
+        self.ctx = ctx
+        self.batches = list(batches)  # keeps the arrays alive
+        descs = (N.BatchDesc * len(self.batches))()
+        for k, (tin, tout, res) in enumerate(self.batches):
+            descs[k].inp = tin.struct()
+            descs[k].out = tout.struct()
+            descs[k].res = res.struct()
+        h = C.c_void_p()
+        N.check("hepem_batch_plan_create", N.lib().hepem_batch_plan_create(ctx.h, descs, len(self.batches), C.byref(h)))
+        self.h = h
+
+    def run(self, seed: int, stream=None) -> None:
+        N.check("hepem_batch_plan_run", N.lib().hepem_batch_plan_run(self.h, C.c_uint64(seed), _stream_handle(stream)))
+
+    def tune(self, seed: int, stream=None) -> tuple:
+        N.check("hepem_batch_plan_tune", N.lib().hepem_batch_plan_tune(self.h, C.c_uint64(seed), _stream_handle(stream)))
+        return self.shape()
+
+    def shape(self) -> tuple:
+        b, g = C.c_int32(), C.c_int32()
+        N.check("hepem_batch_plan_shape", N.lib().hepem_batch_plan_shape(self.h, C.byref(b), C.byref(g)))
+        return b.value, g.value
+
+    def rejected(self, b: int, stream=None) -> int:
+        v = C.c_uint32()
+        N.check("hepem_batch_plan_rejected", N.lib().hepem_batch_plan_rejected(self.h, b, C.byref(v), _stream_handle(stream)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().hepem_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

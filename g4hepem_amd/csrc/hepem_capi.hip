@@ -334,14 +334,16 @@ extern "C" int hepem_lookup_eloss_xsec(const hepem_ctx* c, int ptype, int32_t n,
 }
 
 namespace {
+// every array of a non-empty batch must be given; an empty batch may pass nulls
 bool TracksOk(const hepem_tracks* t) {
-  return t && t->n >= 0 && t->ptype && t->id && t->imc && t->ekin && t->dir && t->safety && t->on_boundary && t->num_ia_left &&
-         t->msc_init_range && t->msc_dyn_range_factor && t->msc_tlimit_min && t->msc_first_step && t->rng_ctr && t->rng_gauss &&
-         t->rng_has_gauss;
+  return t && t->n >= 0 &&
+         (t->n == 0 || (t->ptype && t->id && t->imc && t->ekin && t->dir && t->safety && t->on_boundary && t->num_ia_left &&
+                        t->msc_init_range && t->msc_dyn_range_factor && t->msc_tlimit_min && t->msc_first_step && t->rng_ctr &&
+                        t->rng_gauss && t->rng_has_gauss));
 }
-bool ResOk(const hepem_step_out* r) {
-  return r && r->true_step && r->geom_step && r->edep && r->displacement && r->winner && r->stopped && r->nsec && r->sec_ptype &&
-         r->sec_ekin && r->sec_dir;
+bool ResOk(const hepem_step_out* r, int n) {
+  return r && (n == 0 || (r->true_step && r->geom_step && r->edep && r->displacement && r->winner && r->stopped && r->nsec &&
+                          r->sec_ptype && r->sec_ekin && r->sec_dir));
 }
 }  // namespace
 
@@ -365,7 +367,7 @@ extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, vo
 
 extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
                               hepem_step_out* res, void* work, void* stream) {
-  if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res) || out->n != in->n || !work) return HEPEM_ERR_INVALID;
+  if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res, in->n) || out->n != in->n || !work) return HEPEM_ERR_INVALID;
   const int n = in->n;
   if (n == 0) return HEPEM_OK;
   const hipStream_t s = static_cast<hipStream_t>(stream);
@@ -400,7 +402,7 @@ int EnsureWork(const hepem_ctx* cc, int n, void*& work) {
 
 extern "C" int hepem_step(const hepem_ctx* c, uint64_t seed, const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res,
                           void* work, void* stream) {
-  if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res) || out->n != in->n) return HEPEM_ERR_INVALID;
+  if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res, in->n) || out->n != in->n) return HEPEM_ERR_INVALID;
   const int n = in->n;
   if (n == 0) return HEPEM_OK;
   int rc = EnsureWork(c, n, work);
@@ -412,8 +414,9 @@ extern "C" int hepem_step(const hepem_ctx* c, uint64_t seed, const hepem_tracks*
 
 namespace {
 bool HowFarOk(const hepem_howfar_state* h, int n) {
-  return h && h->n == n && h->geom_step && h->range && h->mfp && h->winner && h->msc_lambtr1 && h->msc_true_step &&
-         h->msc_zpath && h->msc_par1 && h->msc_par2 && h->msc_par3 && h->msc_active && h->msc_displace && h->pe_xsec;
+  return h && h->n == n &&
+         (n == 0 || (h->geom_step && h->range && h->mfp && h->winner && h->msc_lambtr1 && h->msc_true_step && h->msc_zpath &&
+                     h->msc_par1 && h->msc_par2 && h->msc_par3 && h->msc_active && h->msc_displace && h->pe_xsec));
 }
 
 // HowFar (perform == false) or Perform of a batch of slots of one KIND; a mixed
@@ -421,7 +424,7 @@ bool HowFarOk(const hepem_howfar_state* h, int n) {
 template <int KIND>
 int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs, hepem_step_out* res,
               void* work, void* stream, bool perform) {
-  if (!c || !TracksOk(t) || !HowFarOk(hs, t ? t->n : -1) || (perform && !ResOk(res))) return HEPEM_ERR_INVALID;
+  if (!c || !TracksOk(t) || !HowFarOk(hs, t ? t->n : -1) || (perform && !ResOk(res, t->n))) return HEPEM_ERR_INVALID;
   const int n = t->n;
   if (n == 0) return HEPEM_OK;
   int rc = EnsureWork(c, n, work);
@@ -485,11 +488,177 @@ extern "C" int hepem_step_rejected(const void* work, uint32_t* nrej, void* strea
   return HEPEM_OK;
 }
 
-extern "C" int hepem_step_batched(const hepem_ctx* c, uint64_t seed, const hepem_batch_desc* descs, int32_t nbatch, void* stream) {
+// ---- batch plans ----------------------------------------------------------------------------------
+struct hepem_batch_plan {
+  const hepem_ctx* ctx = nullptr;
+  int nbatch = 0;
+  uint64_t total = 0;             // tracks over all batches
+  uint32_t ntiles = 0;            // classify tiles over all batches
+  uint32_t nchunks[kNumShapes] = {0, 0, 0};
+  BatchDev* dDescs = nullptr;
+  uint32_t* dQueue = nullptr;
+  uint32_t* dCounters = nullptr;  // 4 per batch
+  int shape = kNumShapes - 1;     // block size ShapeBlock(shape)
+  int grid = 0;                   // step-kernel blocks (nchunks[shape])
+};
+
+namespace {
+template <typename R, int S>
+void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
+  hipLaunchKernelGGL((k_step_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+                     p->nchunks[S], p->dQueue, p->dCounters);
+}
+template <typename R> void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
+  switch (p->shape) {
+    case 0: LaunchStepBatched<R, 0>(T, p, seed, s); break;
+    case 1: LaunchStepBatched<R, 1>(T, p, seed, s); break;
+    default: LaunchStepBatched<R, 2>(T, p, seed, s); break;
+  }
+}
+
+void FreePlan(hepem_batch_plan* p) {
+  if (p->dDescs) (void)hipFree(p->dDescs);
+  if (p->dQueue) (void)hipFree(p->dQueue);
+  if (p->dCounters) (void)hipFree(p->dCounters);
+  delete p;
+}
+}  // namespace
+
+extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_desc* descs, int32_t nbatch,
+                                       hepem_batch_plan** out) {
+  if (!out) return HEPEM_ERR_INVALID;
+  *out = nullptr;
+  if (!c || nbatch < 1 || !descs) return HEPEM_ERR_INVALID;
+  std::vector<BatchDev> h((size_t)nbatch + 1);
+  uint64_t q = 0, tiles = 0, chunks[kNumShapes] = {0, 0, 0};
+  for (int b = 0; b < nbatch; ++b) {
+    const hepem_batch_desc& d = descs[b];
+    if (!TracksOk(&d.in) || !TracksOk(&d.out) || !ResOk(&d.res, d.in.n) || d.out.n != d.in.n) return HEPEM_ERR_INVALID;
+    BatchDev& x = h[b];
+    x.in = d.in;
+    x.out = d.out;
+    x.res = d.res;
+    x.qoff = (uint32_t)q;
+    x.tile0 = (uint32_t)tiles;
+    for (int s = 0; s < kNumShapes; ++s) x.chunk0[s] = (uint32_t)chunks[s];
+    const uint64_t n = (uint64_t)d.in.n;
+    q += n;
+    tiles += (n + kClassifyTile - 1) / kClassifyTile;
+    for (int s = 0; s < kNumShapes; ++s) chunks[s] += (n + ShapeBlock(s) - 1) / ShapeBlock(s);
+  }
+  if (q >= (1ull << 31)) return HEPEM_ERR_CAPACITY;
+  BatchDev& last = h[nbatch];
+  std::memset(&last, 0, sizeof(last));
+  last.qoff = (uint32_t)q;
+  last.tile0 = (uint32_t)tiles;
+  for (int s = 0; s < kNumShapes; ++s) last.chunk0[s] = (uint32_t)chunks[s];
+
+  hepem_batch_plan* p = new (std::nothrow) hepem_batch_plan();
+  if (!p) return HEPEM_ERR_ALLOC;
+  p->ctx = c;
+  p->nbatch = nbatch;
+  p->total = q;
+  p->ntiles = (uint32_t)tiles;
+  for (int s = 0; s < kNumShapes; ++s) p->nchunks[s] = (uint32_t)chunks[s];
+  p->grid = (int)(p->nchunks[p->shape] > 0 ? p->nchunks[p->shape] : 1);
+  if (hipMalloc(&p->dDescs, h.size() * sizeof(BatchDev)) != hipSuccess ||
+      hipMalloc(&p->dQueue, (q > 0 ? q : 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&p->dCounters, (size_t)4 * nbatch * sizeof(uint32_t)) != hipSuccess) {
+    FreePlan(p);
+    return HEPEM_ERR_ALLOC;
+  }
+  if (hipMemcpy(p->dDescs, h.data(), h.size() * sizeof(BatchDev), hipMemcpyHostToDevice) != hipSuccess) {
+    FreePlan(p);
+    return HEPEM_ERR_HIP;
+  }
+  *out = p;
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_batch_plan_destroy(hepem_batch_plan* p) {
+  if (!p) return HEPEM_ERR_INVALID;
+  FreePlan(p);
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, void* stream) {
+  if (!p || !p->ctx) return HEPEM_ERR_INVALID;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nc = 4 * p->nbatch;
+  hipLaunchKernelGGL(k_fill_u32, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dCounters, nc, 0u);
+  if (p->total > 0) {
+    hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kBlock), 0, s, p->dDescs, p->nbatch, p->dQueue, p->dCounters);
+    if (p->ctx->precision == HEPEM_FP64) LaunchStepBatched(p->ctx->t64, p, seed, s);
+    else LaunchStepBatched(p->ctx->t32, p, seed, s);
+  }
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_batch_plan_tune(hepem_batch_plan* p, uint64_t seed, void* stream) {
+  if (!p || !p->ctx) return HEPEM_ERR_INVALID;
+  if (p->total == 0) return HEPEM_OK;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return HEPEM_ERR_HIP; }
+  float best = 1e30f;
+  int bestShape = p->shape, bestGrid = p->grid;
+  int rc = HEPEM_OK;
+  for (int sh = 0; sh < kNumShapes && rc == HEPEM_OK; ++sh) {
+    {
+      p->shape = sh;
+      p->grid = (int)p->nchunks[sh];
+      rc = hepem_batch_plan_run(p, seed, stream);  // warm-up
+      if (rc != HEPEM_OK) break;
+      if (hipEventRecord(e0, s) != hipSuccess) { rc = HEPEM_ERR_HIP; break; }
+      for (int r = 0; r < 3 && rc == HEPEM_OK; ++r) rc = hepem_batch_plan_run(p, seed, stream);
+      if (rc != HEPEM_OK || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = rc != HEPEM_OK ? rc : HEPEM_ERR_HIP;
+        break;
+      }
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = HEPEM_ERR_HIP; break; }
+      if (ms < best) {
+        best = ms;
+        bestShape = sh;
+        bestGrid = p->grid;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->shape = bestShape;
+  p->grid = bestGrid;
+  return rc;
+}
+
+extern "C" int hepem_batch_plan_shape(const hepem_batch_plan* p, int32_t* block, int32_t* grid) {
+  if (!p || !block || !grid) return HEPEM_ERR_INVALID;
+  *block = ShapeBlock(p->shape);
+  *grid = p->grid;
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_batch_plan_rejected(const hepem_batch_plan* p, int32_t b, uint32_t* nrej, void* stream) {
+  if (!p || !nrej || b < 0 || b >= p->nbatch) return HEPEM_ERR_INVALID;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_OK(hipMemcpyAsync(nrej, p->dCounters + 4 * b + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_step_batched(const hepem_ctx* c, uint64_t seed, const hepem_batch_desc* descs, int32_t nbatch,
+                                  void* stream) {
   if (!c || nbatch < 0 || (nbatch > 0 && !descs)) return HEPEM_ERR_INVALID;
-  (void)seed;
-  (void)stream;
-  return HEPEM_ERR_UNSUPPORTED;
+  if (nbatch == 0) return HEPEM_OK;
+  hepem_batch_plan* p = nullptr;
+  int rc = hepem_batch_plan_create(c, descs, nbatch, &p);
+  if (rc != HEPEM_OK) return rc;
+  rc = hepem_batch_plan_run(p, seed, stream);
+  if (rc == HEPEM_OK && hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess) rc = HEPEM_ERR_HIP;
+  FreePlan(p);
+  return rc;
 }
 
 extern "C" int hepem_tracks_reset_state(const hepem_tracks* t, void* stream) {

@@ -35,13 +35,15 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
 constexpr int kClassifyItems = 16;
 constexpr int kClassifyTile = kBlock * kClassifyItems;
 
-__global__ void __launch_bounds__(kBlock) k_classify(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
-                                                     uint32_t* __restrict__ counters) {
+// Compacts tile `tile` of a batch of n slots into its queue (e-/e+ from the
+// front, gammas from the back); counters[0..1] count e-/e+ and gammas.
+__device__ void ClassifyTile(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
+                             uint32_t* __restrict__ counters, int tile) {
   __shared__ uint32_t sE[kBlock / 64], sG[kBlock / 64], sBase[2];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int tile0 = blockIdx.x * kClassifyTile;
+  const int tile0 = tile * kClassifyTile;
   uint32_t cE = 0, cG = 0;
   for (int it = 0; it < kClassifyItems; ++it) {
     const int i = tile0 + it * kBlock + threadIdx.x;
@@ -82,6 +84,11 @@ __global__ void __launch_bounds__(kBlock) k_classify(int n, const int8_t* __rest
     pE += (uint32_t)__popcll(mE);
     pG += (uint32_t)__popcll(mG);
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_classify(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
+                                                     uint32_t* __restrict__ counters) {
+  ClassifyTile(n, ptype, queue, counters, blockIdx.x);
 }
 
 template <typename R> HD void WriteSlot(const hepem_step_out& res, int n, int i, int s, const Sec<R>* sec) {
@@ -492,6 +499,71 @@ __global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, 
   CopyIdentity(in, out, i);
   if ((uint32_t)j < nE) StepElectron(T, seed, in, out, res, n, i);
   else StepGamma(T, seed, in, out, res, n, i);
+}
+
+// ---- batched stepping ---------------------------------------------------------------------------
+// Block sizes of the batched step kernel, selected at run time.
+constexpr int kNumShapes = 3;
+constexpr int ShapeBlock(int s) { return 64 << s; }
+
+// Device descriptor of one batch of a plan. A plan holds nbatch + 1 of them;
+// the last carries only the totals of the prefix fields.
+struct BatchDev {
+  hepem_tracks in, out;
+  hepem_step_out res;
+  uint32_t qoff;                 // first entry of the batch's queue
+  uint32_t tile0;                // first classify tile of the batch
+  uint32_t chunk0[kNumShapes];   // first step chunk for block size ShapeBlock(s)
+};
+
+// Largest b in [0, nbatch) with key(d[b]) <= x (binary search over a prefix).
+template <typename Key> HD int FindBatch(const BatchDev* __restrict__ d, int nbatch, uint32_t x, Key key) {
+  int lo = 0, hi = nbatch - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (key(d[mid]) <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// bcounters[4b + 0..2]: e-/e+, gammas, rejected tracks of batch b
+__global__ void __launch_bounds__(kBlock) k_classify_batched(const BatchDev* __restrict__ d, int nbatch,
+                                                             uint32_t* __restrict__ queue, uint32_t* __restrict__ bcounters) {
+  const uint32_t t = blockIdx.x;
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
+  ClassifyTile(d[b].in.n, d[b].in.ptype, queue + d[b].qoff, bcounters + 4 * b, (int)(t - d[b].tile0));
+}
+
+// One thread per queue entry of every batch: block c takes chunk c of the
+// chunk sequence (all batches' chunks of BS entries, in batch order). A
+// grid-stride form keeps the descriptor and table state live across
+// iterations and costs a third of the occupancy, so the hardware dispatcher
+// does the load balancing instead.
+template <typename R, int S>
+__global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
+                                                                int nbatch, uint32_t nchunks, const uint32_t* __restrict__ queue,
+                                                                uint32_t* __restrict__ bcounters) {
+  constexpr int BS = ShapeBlock(S);
+  {
+    const uint32_t c = blockIdx.x;
+    if (c >= nchunks) return;
+    // the batch index is uniform over the block: keep it (and the descriptor
+    // loads) scalar
+    const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, c, [](const BatchDev& x) { return x.chunk0[S]; }));
+    const hepem_tracks in = d[b].in;
+    hepem_tracks out = d[b].out;
+    const hepem_step_out res = d[b].res;
+    const int n = in.n;
+    const int j = (int)(c - d[b].chunk0[S]) * BS + (int)threadIdx.x;
+    if (j >= n) return;
+    uint32_t* cnt = bcounters + 4 * b;
+    const int i = (int)queue[d[b].qoff + j];
+    if (RejectTrack(T, in, out, res, n, i, cnt)) return;
+    CopyIdentity(in, out, i);
+    if ((uint32_t)j < cnt[0]) StepElectron(T, seed, in, out, res, n, i);
+    else StepGamma(T, seed, in, out, res, n, i);
+  }
 }
 
 template <typename R>

@@ -29,7 +29,7 @@
  *       with no geometry between HowFar and Perform (geometric step = physics
  *       step, not on a boundary).
  *
- *   hepem_step_batched
+ *   hepem_batch_plan_* / hepem_step_batched
  *       many independent track batches of uneven size in one launch (no
  *       reference counterpart: the reference steps one track per thread,
  *       G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:346,895).
@@ -213,8 +213,32 @@ int  hepem_gamma_howfar(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, he
 int  hepem_gamma_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
                          hepem_step_out* res, void* work, void* hip_stream);
 
-/* Many independent batches in one persistent launch. `descs` is a host array.
- * Results are bit-identical to calling hepem_step on each batch. */
+/* ---- many independent batches in one launch -------------------------------
+ * A batch plan holds, in device memory, the descriptors of nbatch batches of
+ * uneven size, their particle-type queues and per-batch counters. Creating it
+ * allocates and uploads (synchronously); running it only enqueues kernels
+ * (counter fill, one batched classify, one batched step) -- no allocation, no
+ * host copies -- so a run can be captured into a hipGraph and replayed.
+ * Results are bit-identical to calling hepem_step on each batch with the same
+ * seed. The batches' arrays must stay valid while the plan is used. */
+typedef struct hepem_batch_plan hepem_batch_plan;
+int  hepem_batch_plan_create(const hepem_ctx* ctx, const hepem_batch_desc* descs, int32_t nbatch,
+                             hepem_batch_plan** out_plan);
+int  hepem_batch_plan_destroy(hepem_batch_plan* plan);
+int  hepem_batch_plan_run(const hepem_batch_plan* plan, uint64_t seed, void* hip_stream);
+/* Times each launch shape of the batched step (block size 64, 128 or 256
+ * threads) on the plan's own batches and keeps the fastest. This runs the
+ * step several times: batches stepped in place advance. Synchronises the
+ * stream. */
+int  hepem_batch_plan_tune(hepem_batch_plan* plan, uint64_t seed, void* hip_stream);
+/* Launch shape in use: block size and grid blocks of the step kernel. */
+int  hepem_batch_plan_shape(const hepem_batch_plan* plan, int32_t* block, int32_t* grid);
+/* Tracks of batch `b` the last run rejected (see hepem_step_rejected).
+ * Synchronises the stream. */
+int  hepem_batch_plan_rejected(const hepem_batch_plan* plan, int32_t b, uint32_t* nrejected, void* hip_stream);
+
+/* Convenience form: create a plan for `descs` (a host array), run it once,
+ * wait for it and free it. Allocates; not for graph capture. */
 int  hepem_step_batched(const hepem_ctx* ctx, uint64_t seed, const hepem_batch_desc* descs,
                         int32_t nbatch, void* hip_stream);
 

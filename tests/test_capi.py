@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     from g4hepem_amd import lib
     L = lib()
     names = declared_functions()
-    assert len(names) >= 21, names
+    assert len(names) >= 27, names
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
@@ -42,7 +42,10 @@ def test_null_arguments_are_rejected():
     assert L.hepem_step_prepare(None, C.byref(t), None, None) == -1
     assert L.hepem_step_run(None, 1, C.byref(t), C.byref(t), C.byref(r), None, None) == -1
     assert L.hepem_tracks_reset_state(None, None) == -1
-    assert L.hepem_tracks_reset_state(C.byref(t), None) == -1  # null arrays
+    assert L.hepem_tracks_reset_state(C.byref(t), None) == 0   # empty batch: nothing to do
+    t5 = Tracks()
+    t5.n = 5
+    assert L.hepem_tracks_reset_state(C.byref(t5), None) == -1  # null arrays
     assert L.hepem_step_batched(None, 1, None, 0, None) == -1
     assert L.hepem_step_rejected(None, None, None) == -1
     from g4hepem_amd._native import HowFarState
@@ -51,6 +54,10 @@ def test_null_arguments_are_rejected():
         assert getattr(L, name)(None, 1, C.byref(t), C.byref(hs), None, None) == -1, name
     for name in ("hepem_perform", "hepem_electron_perform", "hepem_gamma_perform"):
         assert getattr(L, name)(None, 1, C.byref(t), C.byref(hs), C.byref(r), None, None) == -1, name
+    assert L.hepem_batch_plan_create(None, None, 1, None) == -1
+    assert L.hepem_batch_plan_run(None, 1, None) == -1
+    assert L.hepem_batch_plan_tune(None, 1, None) == -1
+    assert L.hepem_batch_plan_destroy(None) == -1
     assert L.hepem_status_string(-3) == b"capacity exceeded"
     assert L.hepem_step_work_bytes(1000) >= 4000
 

@@ -1,0 +1,122 @@
+"""Many uneven batches stepped by one batch plan (hepem_batch_plan_*):
+bit-identical to hepem_step on each batch, before and after tuning the
+launch shape, and when the run is captured into a hipGraph and replayed."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import same_bits
+from oracle.oracle_py import OUT_FIELDS, TRACK_FIELDS
+from test_gpu_parity import gpu_step
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+SEED = 0xBA7C4
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU test run without a visible GPU"
+    return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def ctx64(tables2, dev):
+    from g4hepem_amd import HEPEM_FP64, Context
+    return Context(tables2, HEPEM_FP64)
+
+
+def make_batches(sizes, dev, seed0=900):
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_tracks
+    host, dev_batches = [], []
+    first = 0
+    for k, n in enumerate(sizes):
+        tr = step_tracks(n, seed0 + k, 4, lnemin=math.log(1e-2), lnemax=math.log(1e5), first_id=first)
+        first += n
+        if n > 10:
+            tr["imc"][3] = 77     # one rejected track per batch
+        host.append(tr)
+        dev_batches.append((TrackSoA.from_numpy(tr, "torch", dev), TrackSoA(n, "torch", dev), StepOutSoA(n, "torch", dev)))
+    return host, dev_batches
+
+
+def check_against_unbatched(ctx, host, dev_batches, dev, label):
+    for k, (tr, (tin, tout, res)) in enumerate(zip(host, dev_batches)):
+        ref_t, ref_r, ref_rej = gpu_step(ctx, SEED, tr, dev)
+        got_t, got_r = tout.to_numpy(), res.to_numpy()
+        for f in OUT_FIELDS:
+            assert same_bits(got_r[f], ref_r[f]), (label, k, f)
+        for f in TRACK_FIELDS:
+            if f.startswith("msc"):
+                m = np.tile(tr["ptype"] != 2, TRACK_FIELDS[f][1])
+                assert same_bits(got_t[f][m], ref_t[f][m]), (label, k, f)
+            else:
+                assert same_bits(got_t[f], ref_t[f]), (label, k, f)
+
+
+def test_batched_equals_unbatched(ctx64, dev):
+    from g4hepem_amd import BatchPlan
+    sizes = [1, 63, 64, 65, 1000, 4095, 4097, 30000, 0, 100000, 257]
+    host, batches = make_batches(sizes, dev)
+    plan = BatchPlan(ctx64, batches)
+    _print("batched run, default shape", plan.shape())
+    plan.run(SEED)
+    torch.cuda.synchronize()
+    for k, n in enumerate(sizes):
+        assert plan.rejected(k) == (1 if n > 10 else 0), k
+    check_against_unbatched(ctx64, host, batches, dev, "default")
+
+
+def test_batched_tune_then_run(ctx64, dev):
+    from g4hepem_amd import BatchPlan
+    rng = np.random.Generator(np.random.Philox(5))
+    sizes = np.exp(rng.uniform(math.log(1e2), math.log(3e4), 24)).astype(int).tolist()
+    host, batches = make_batches(sizes, dev, seed0=950)
+    plan = BatchPlan(ctx64, batches)
+    block, grid = plan.tune(SEED)
+    _print("tuned shape", block, grid)
+    assert block in (64, 128, 256) and grid == sum((n + block - 1) // block for n in sizes)
+    for _, tout, res in batches:  # results must come from the run below
+        res["edep"].fill_(-1.0)
+    plan.run(SEED)
+    torch.cuda.synchronize()
+    check_against_unbatched(ctx64, host, batches, dev, "tuned")
+
+
+def test_batched_graph_replay(ctx64, dev):
+    """The plan run captured into a graph; every replay, with eager work and
+    synchronisation in between, reproduces the eager result."""
+    from g4hepem_amd import BatchPlan
+    sizes = [5000, 123, 70000, 9999, 1]
+    host, batches = make_batches(sizes, dev, seed0=990)
+    plan = BatchPlan(ctx64, batches)
+    plan.run(SEED)
+    torch.cuda.synchronize()
+    eager = [(tout.to_numpy(), res.to_numpy()) for _, tout, res in batches]
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            plan.run(SEED, stream=s)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        _print(f"graph replay {rep}")
+        for _, tout, res in batches:
+            tout["ekin"].zero_()
+            res["edep"].fill_(-1.0)
+            res["winner"].fill_(-7)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k, (_, tout, res) in enumerate(batches):
+            got_t, got_r = tout.to_numpy(), res.to_numpy()
+            for f in OUT_FIELDS:
+                assert same_bits(got_r[f], eager[k][1][f]), (rep, k, f)
+            assert same_bits(got_t["ekin"], eager[k][0]["ekin"]), (rep, k)
+        assert plan.rejected(2) == 1
