@@ -12,8 +12,10 @@ before the timed region; every step starts from the same input batch and writes
 a separate output batch.
 
 Also reported: the FP32-table form of the same step, config-1 style lookups
-(100k and 10M e- queries over G4_Pb/G4_lAr) and the oracle's CPU throughput on
-the same workload on the host cores.
+(100k and 10M e- queries over G4_Pb/G4_lAr), 64 uneven batches (sizes
+log-uniform in [1e3, 1e6], ~9.3M tracks) stepped by one tuned batch plan
+replayed from a hipGraph in both forms, and the oracle's CPU throughput on the
+headline workload on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -118,6 +120,48 @@ def time_steps(ctx, tin, tout, res, work, steps, warmup, seed):
     return total_ms / steps, float(np.mean(run_ms)), wall
 
 
+def config3_batches(seed: int, nbatch: int = 64):
+    """64 batches, sizes log-uniform in [1e3, 1e6], per-track distributions
+    as the headline workload; track ids run on across batches."""
+    from g4hepem_amd.tracks import step_tracks
+    rng = np.random.Generator(np.random.Philox(seed ^ 0x3C3))
+    sizes = np.exp(rng.uniform(math.log(1e3), math.log(1e6), nbatch)).astype(np.int64)
+    out, first = [], 0
+    for k, n in enumerate(sizes):
+        out.append(step_tracks(int(n), seed + 17 * k, len(SET2), first_id=first))
+        first += int(n)
+    return out
+
+
+def time_batched(ctx, batches, seed, reps):
+    """A batch plan over `batches` (device SoA triples): tune the launch shape,
+    capture one run into a hipGraph and time `reps` replays with HIP events on
+    the replay stream. Returns (ms per run, (block, grid))."""
+    import torch
+    from g4hepem_amd import BatchPlan
+    plan = BatchPlan(ctx, batches)
+    shape = plan.tune(seed)
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            plan.run(seed, stream=s)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(s):
+        a.record(s)
+        for _ in range(reps):
+            g.replay()
+        b.record(s)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    plan.close()
+    return ms, shape
+
+
 def time_lookups(ctx, n, reps, seed=2026):
     import torch
     from g4hepem_amd.tracks import lookup_queries
@@ -165,6 +209,10 @@ def cpu_baseline(tables, tracks: dict, min_seconds: float, seed: int):
                       f"({done} track-steps, {dt:.1f} s, {cores} threads)"}
 
 
+def log(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +253,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    log("headline FP64 step")
     ctx64 = Context(tables2, HEPEM_FP64)
     ms64, run64, wall64 = time_steps(ctx64, tin, tout, res, work, args.steps, args.warmup, args.seed)
     value = n / (ms64 * 1e-3)
@@ -236,10 +285,12 @@ def main():
         },
     }
     if not args.no_extras:
+        log("FP32-table step")
         ctx32 = Context(tables2, HEPEM_FP32T)
         ms32, run32, _ = time_steps(ctx32, tin, tout, res, work, args.steps, args.warmup, args.seed)
         line["fp32_table_form"] = {"value": n / (ms32 * 1e-3), "unit": "track-steps/s", "ms_per_step": ms32,
                                    "kernel_ms": run32, "speedup_vs_fp64": ms64 / ms32}
+        log("config-1 lookups")
         tables1 = Tables(SET1, 0.7)
         ctx1 = Context(tables1, HEPEM_FP64)
         q100k, ms100k = time_lookups(ctx1, 100_000, 200)
@@ -247,7 +298,27 @@ def main():
         line["config1_lookups"] = {"queries_per_s_100k": q100k, "ms_100k": ms100k,
                                    "queries_per_s_10M": q10m, "ms_10M": ms10m,
                                    "achieved_GBs_10M": 60.0 * 10_000_000 / (ms10m * 1e-3) / 1e9}
+        del tin, tout, res
+        torch.cuda.empty_cache()
+        log("config-3 batches")
+        host_batches = config3_batches(args.seed)
+        ntot = sum(len(b["ekin"]) for b in host_batches)
+        dev_batches = [(TrackSoA.from_numpy(b, "torch", dev), TrackSoA(len(b["ekin"]), "torch", dev),
+                        StepOutSoA(len(b["ekin"]), "torch", dev)) for b in host_batches]
+        del host_batches
+        ms_b64, shape64 = time_batched(ctx64, dev_batches, args.seed, max(3, args.steps // 4))
+        ms_b32, shape32 = time_batched(ctx32, dev_batches, args.seed, max(3, args.steps // 4))
+        line["config3_batched"] = {
+            "batches": len(dev_batches), "tracks": ntot, "launch": "hipGraph replay of a tuned batch plan",
+            "fp64": {"value": ntot / (ms_b64 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b64,
+                     "block": shape64[0]},
+            "fp32_table_form": {"value": ntot / (ms_b32 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b32,
+                                "block": shape32[0]},
+            "speedup_fp32_vs_fp64": ms_b64 / ms_b32,
+        }
+        del dev_batches
     if not args.no_cpu:
+        log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(tables2, tracks, args.cpu_seconds, args.seed)
     else:
         line["cpu_baseline"] = None
