@@ -33,7 +33,7 @@ class Tracks(C.Structure):
     _fields_ = [("n", C.c_int32), ("ptype", P), ("id", P), ("imc", P), ("ekin", P), ("dir", P),
                 ("safety", P), ("on_boundary", P), ("num_ia_left", P), ("msc_init_range", P),
                 ("msc_dyn_range_factor", P), ("msc_tlimit_min", P), ("msc_first_step", P),
-                ("rng_ctr", P), ("rng_gauss", P), ("rng_has_gauss", P)]
+                ("rng_ctr", P), ("rng_gauss", P), ("rng_has_gauss", P), ("rng_stream", P)]
 
 
 class StepOut(C.Structure):
@@ -49,6 +49,27 @@ class HowFarState(C.Structure):
                 ("msc_par2", P), ("msc_par3", P), ("msc_active", P), ("msc_displace", P), ("pe_xsec", P)]
 
 
+class SlabSetup(C.Structure):
+    """hepem_slab_setup (include/g4hepem_amd_shower.h)."""
+    _fields_ = [("nlayers", C.c_int32), ("nabs", C.c_int32), ("thickness", C.c_double * 4), ("imc", C.c_int32 * 4),
+                ("primary", C.c_int32), ("ekin0", C.c_double), ("capacity", C.c_int32), ("events_per_chunk", C.c_int32)]
+
+
+class SlabResult(C.Structure):
+    _fields_ = [("edep_sum", P), ("edep_sum2", P), ("edep_event", P), ("leak_event", P), ("leak_sum", C.c_double), ("edep_total", C.c_double),
+                ("steps", C.c_int64), ("tracks", C.c_int64), ("iterations", C.c_int32), ("max_in_flight", C.c_int32)]
+
+
+def slab_setup(nlayers: int, thickness, imc, primary: int, ekin0: float, capacity: int = 1 << 22,
+               events_per_chunk: int = 256) -> SlabSetup:
+    s = SlabSetup()
+    s.nlayers, s.nabs = nlayers, len(thickness)
+    for k, (t, m) in enumerate(zip(thickness, imc)):
+        s.thickness[k], s.imc[k] = t, m
+    s.primary, s.ekin0, s.capacity, s.events_per_chunk = primary, ekin0, capacity, events_per_chunk
+    return s
+
+
 class BatchDesc(C.Structure):
     _fields_ = [("inp", Tracks), ("out", Tracks), ("res", StepOut)]
 
@@ -60,7 +81,7 @@ TRACK_FIELDS = {
     "num_ia_left": ("float64", 4), "msc_init_range": ("float64", 1),
     "msc_dyn_range_factor": ("float64", 1), "msc_tlimit_min": ("float64", 1),
     "msc_first_step": ("uint8", 1), "rng_ctr": ("uint32", 1), "rng_gauss": ("float64", 1),
-    "rng_has_gauss": ("uint8", 1),
+    "rng_has_gauss": ("uint8", 1), "rng_stream": ("uint32", 1),
 }
 OUT_FIELDS = {
     "true_step": ("float64", 1), "geom_step": ("float64", 1), "edep": ("float64", 1),
@@ -101,6 +122,7 @@ def lib() -> C.CDLL:
     L.hepem_step_run.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(Tracks), C.POINTER(StepOut), vp, vp]
     L.hepem_step_rejected.argtypes = [vp, C.POINTER(C.c_uint32), vp]
     L.hepem_step_batched.argtypes = [vp, u64, C.POINTER(BatchDesc), i32, vp]
+    L.hepem_slab_run.argtypes = [vp, C.POINTER(SlabSetup), i32, u64, C.POINTER(SlabResult), vp]
     L.hepem_batch_plan_create.argtypes = [vp, C.POINTER(BatchDesc), i32, C.POINTER(vp)]
     L.hepem_batch_plan_destroy.argtypes = [vp]
     L.hepem_batch_plan_run.argtypes = [vp, u64, vp]
@@ -119,7 +141,7 @@ def lib() -> C.CDLL:
                  "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",
                  "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform",
                  "hepem_batch_plan_create", "hepem_batch_plan_destroy", "hepem_batch_plan_run", "hepem_batch_plan_tune",
-                 "hepem_batch_plan_shape", "hepem_batch_plan_rejected"):
+                 "hepem_batch_plan_shape", "hepem_batch_plan_rejected", "hepem_slab_run"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

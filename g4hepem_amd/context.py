@@ -78,6 +78,27 @@ class Context:
         N.check(name, getattr(N.lib(), name)(self.h, C.c_uint64(seed), C.byref(st), C.byref(sh), C.byref(rr), wp,
                                                _stream_handle(stream)))
 
+    def slab_run(self, setup: N.SlabSetup, nevents: int, seed: int, stream=None, per_event: bool = False) -> dict:
+        """hepem_slab_run: showers in a stack of slabs. Returns per-slab sums
+        over events of the deposit and its square, run totals and, with
+        per_event, each event's deposits and leakage (1e-9 MeV units)."""
+        import numpy as np
+        nb = setup.nlayers * setup.nabs
+        s1, s2 = np.zeros(nb), np.zeros(nb)
+        r = N.SlabResult()
+        r.edep_sum, r.edep_sum2 = s1.ctypes.data, s2.ctypes.data
+        ev = lk = None
+        if per_event:
+            ev, lk = np.zeros(nevents * nb, dtype=np.int64), np.zeros(nevents, dtype=np.int64)
+            r.edep_event, r.leak_event = ev.ctypes.data, lk.ctypes.data
+        N.check("hepem_slab_run", N.lib().hepem_slab_run(self.h, C.byref(setup), nevents, C.c_uint64(seed), C.byref(r),
+                                                         _stream_handle(stream)))
+        out = {"edep_sum": s1, "edep_sum2": s2, "leak_sum": r.leak_sum, "edep_total": r.edep_total,
+               "steps": r.steps, "tracks": r.tracks, "iterations": r.iterations, "max_in_flight": r.max_in_flight}
+        if per_event:
+            out["edep_event"], out["leak_event"] = ev.reshape(nevents, nb), lk
+        return out
+
     @staticmethod
     def rejected(work, stream=None) -> int:
         v = C.c_uint32()

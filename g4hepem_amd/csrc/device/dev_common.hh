@@ -133,16 +133,17 @@ HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 }
 
 struct Rng {
-  uint32_t k0, k1, id, ctr;
+  uint32_t k0, k1, id, hi, ctr;
   uint32_t blk;          // block held in buf (0xFFFFFFFF: none)
   double buf[2];
   bool has_gauss;        // G4HepEmRandomEngine::fIsGauss
   double gauss;          // G4HepEmRandomEngine::fGauss
 
-  HD void init(uint64_t seed, uint32_t tid, uint32_t c, bool hg, double g) {
+  HD void init(uint64_t seed, uint32_t tid, uint32_t thi, uint32_t c, bool hg, double g) {
     k0 = (uint32_t)seed;
     k1 = (uint32_t)(seed >> 32);
     id = tid;
+    hi = thi;
     ctr = c;
     blk = 0xFFFFFFFFu;
     has_gauss = hg;
@@ -152,7 +153,7 @@ struct Rng {
     const uint32_t d = ctr++;
     const uint32_t b = d >> 1;
     if (b != blk) {
-      uint32_t c[4] = {b, id, 0u, 0u};
+      uint32_t c[4] = {b, id, hi, 0u};
       philox4x32_10(c, k0, k1);
       const uint64_t w0 = ((uint64_t)c[1] << 32) | c[0];
       const uint64_t w1 = ((uint64_t)c[3] << 32) | c[2];

@@ -134,6 +134,7 @@ HD void WriteRejected(const hepem_tracks& in, hepem_tracks& out, const hepem_ste
   if (out.ekin != in.ekin) {
     out.ptype[i] = in.ptype[i];
     out.id[i] = in.id[i];
+    if (out.rng_stream) out.rng_stream[i] = in.rng_stream ? in.rng_stream[i] : 0u;
     out.imc[i] = in.imc[i];
     out.ekin[i] = in.ekin[i];
     for (int c = 0; c < 3; ++c) out.dir[c * n + i] = in.dir[c * n + i];
@@ -161,11 +162,15 @@ HD bool RejectTrack(const DevTables<R>& T, const hepem_tracks& in, hepem_tracks&
   return true;
 }
 
+// High word of the stream selector of slot i (0 without the optional array).
+HD uint32_t StreamHi(const hepem_tracks& t, int i) { return t.rng_stream ? t.rng_stream[i] : 0u; }
+
 // Copies the identity fields of slot i to a separate output batch.
 HD void CopyIdentity(const hepem_tracks& in, hepem_tracks& out, int i) {
   if (out.ekin == in.ekin) return;
   out.ptype[i] = in.ptype[i];
   out.id[i] = in.id[i];
+  if (out.rng_stream) out.rng_stream[i] = StreamHi(in, i);
   out.imc[i] = in.imc[i];
   out.safety[i] = in.safety[i];
   out.on_boundary[i] = in.on_boundary[i];
@@ -215,7 +220,7 @@ HD void InitElectron(const hepem_tracks& in, int n, int i, uint64_t seed, ElStat
   m.noScat = m.isDisplace = m.active = false;
   m.firstStep = in.msc_first_step[i] != 0;
   el.range = el.pstep = (R)0.;
-  rng.init(seed, (uint32_t)in.id[i], in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
+  rng.init(seed, (uint32_t)in.id[i], StreamHi(in, i), in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
 }
 
 // G4HepEmElectronManager::Perform (G4HepEmElectronManager.icc:461-483).
@@ -383,7 +388,7 @@ HD void StepGamma(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, 
   t.isEl = false;
   LoadTrack(in, n, i, t);
   Rng rng;
-  rng.init(seed, (uint32_t)in.id[i], in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
+  rng.init(seed, (uint32_t)in.id[i], StreamHi(in, i), in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
   SecList<R> sl;
   sl.ns = 0;
   const R pe = GammaHowFar(T, t, rng);
@@ -437,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) k_howfar(DevTables<R> T, uint64_t seed
     g.isEl = false;
     LoadTrack(t, n, i, g);
     Rng rng;
-    rng.init(seed, (uint32_t)t.id[i], t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
+    rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
     const R pe = GammaHowFar(T, g, rng);
 #pragma unroll
     for (int k = 0; k < 4; ++k) t.num_ia_left[k * n + i] = (double)g.nial[k];
@@ -480,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_perform(DevTables<R> T, uint64_t see
     g.gstep = (R)hs.geom_step[i];
     g.mfp[0] = (R)hs.mfp[i];
     Rng rng;
-    rng.init(seed, (uint32_t)t.id[i], t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
+    rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
     GammaPerform(T, g, (R)hs.pe_xsec[i], sl, rng);
     StoreGamma(t, res, n, i, g, sl, rng);
   }

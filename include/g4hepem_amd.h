@@ -35,7 +35,8 @@
  *       G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:346,895).
  *
  * Random numbers: every track owns a counter-based Philox4x32-10 stream keyed
- * by (seed, track id); its draw counter and the cached second Gaussian
+ * by (seed, track id) -- the 64-bit pair (rng_stream, id) when the optional
+ * rng_stream array is given; its draw counter and the cached second Gaussian
  * deviate of G4HepEmRandomEngine::Gauss (G4HepEmRandomEngine.hh:50-70) are
  * part of the track state, so a batch can be re-run or resumed bit-exactly.
  *
@@ -90,6 +91,8 @@ typedef struct hepem_tracks {
   uint32_t* rng_ctr;           /* Philox draw counter                           */
   double*   rng_gauss;         /* cached second Gaussian deviate                */
   uint8_t*  rng_has_gauss;
+  uint32_t* rng_stream;        /* optional (NULL: 0): high word of the 64-bit
+                                  stream selector, stream = rng_stream:id   */
 } hepem_tracks;
 
 /* Per-step results (write only). Up to 2 secondaries per track and step. */

@@ -136,15 +136,16 @@ void oracle_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], ui
 typedef struct orng {
   uint32_t key[2];
   uint32_t id;
+  uint32_t hi;        /* high word of the stream selector */
   uint32_t ctr;       /* draw index */
   int      has_gauss; /* G4HepEmRandomEngine::fIsGauss */
   double   gauss;     /* G4HepEmRandomEngine::fGauss   */
 } orng;
 
-/* draw d of stream id: Philox block d/2, counter {d/2, id, 0, 0}; double j of a
-   block from words (2j, 2j+1): 52 high bits k -> (k + 0.5) 2^-52 in (0,1) */
-static inline double rng_draw(const uint32_t key[2], uint32_t id, uint32_t d) {
-  const uint32_t c[4] = {d >> 1, id, 0u, 0u};
+/* draw d of stream (hi:id): Philox block d/2, counter {d/2, id, hi, 0}; double j
+   of a block from words (2j, 2j+1): 52 high bits k -> (k + 0.5) 2^-52 in (0,1) */
+static inline double rng_draw(const uint32_t key[2], uint32_t id, uint32_t hi, uint32_t d) {
+  const uint32_t c[4] = {d >> 1, id, hi, 0u};
   uint32_t o[4];
   oracle_philox4x32_10(c, key, o);
   const int j = (int)(d & 1u);
@@ -154,10 +155,10 @@ static inline double rng_draw(const uint32_t key[2], uint32_t id, uint32_t d) {
 
 void oracle_rng_uniforms(uint64_t seed, uint32_t id, uint32_t first, int n, double* out) {
   const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-  for (int i = 0; i < n; ++i) out[i] = rng_draw(key, id, first + (uint32_t)i);
+  for (int i = 0; i < n; ++i) out[i] = rng_draw(key, id, 0u, first + (uint32_t)i);
 }
 
-static inline double flat(orng* r) { return rng_draw(r->key, r->id, r->ctr++); }
+static inline double flat(orng* r) { return rng_draw(r->key, r->id, r->hi, r->ctr++); }
 static inline void flatArray(orng* r, int n, double* v) { for (int i = 0; i < n; ++i) v[i] = flat(r); }
 
 static double Gauss(orng* r, double mean, double stDev) {
@@ -1841,6 +1842,7 @@ static void step_one(const step_job* J, int i) {
       }
       if (out != in) {
         out->ptype[i] = in->ptype[i]; out->id[i] = in->id[i]; out->imc[i] = imc; out->ekin[i] = e;
+        if (out->rng_stream) out->rng_stream[i] = in->rng_stream ? in->rng_stream[i] : 0u;
         for (int c = 0; c < 3; ++c) out->dir[c * n + i] = in->dir[c * n + i];
         out->safety[i] = in->safety[i]; out->on_boundary[i] = in->on_boundary[i];
         for (int k = 0; k < 4; ++k) out->num_ia_left[k * n + i] = in->num_ia_left[k * n + i];
@@ -1857,6 +1859,7 @@ static void step_one(const step_job* J, int i) {
   rng.key[0] = (uint32_t)J->seed;
   rng.key[1] = (uint32_t)(J->seed >> 32);
   rng.id = (uint32_t)in->id[i];
+  rng.hi = in->rng_stream ? in->rng_stream[i] : 0u;
   rng.ctr = in->rng_ctr[i];
   rng.has_gauss = in->rng_has_gauss[i];
   rng.gauss = in->rng_gauss[i];
@@ -1934,6 +1937,7 @@ static void step_one(const step_job* J, int i) {
   if (out != in) {
     out->ptype[i] = in->ptype[i];
     out->id[i] = in->id[i];
+    if (out->rng_stream) out->rng_stream[i] = in->rng_stream ? in->rng_stream[i] : 0u;
     out->imc[i] = in->imc[i];
     out->safety[i] = in->safety[i];
   }
@@ -2000,4 +2004,196 @@ int oracle_step_geom(const struct G4HepEmData* d, const struct G4HepEmParameters
   for (int k = 0; k < nthreads; ++k) pthread_create(&th[k], NULL, step_worker, &jobs[k]);
   for (int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
   return 0;
+}
+
+/* ---- slab shower (CPU harness of hepem_slab_run) ------------------------------------
+   The same tracking loop as the device driver (g4hepem_amd_shower.h): safety, HowFar,
+   geometry cut at the next plane, Perform, MSC displacement with the safety rule of
+   G4HepEmTrackingManager.cc:527-566, slab crossing, fixed-point scoring, and the same
+   64-bit stream ids; tracks of an event are followed depth-first from a stack. */
+
+#define SH_FIX 1.0e9
+#define SH_GEOM_MIN 5.0e-8
+
+static uint64_t sh_mix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static uint64_t sh_primary_stream(int event) { return sh_mix64(0x5EED000000000000ull ^ (uint64_t)event); }
+static uint64_t sh_child_stream(uint64_t parent, int nstep, int slot) {
+  return sh_mix64(parent ^ sh_mix64(((uint64_t)(uint32_t)nstep << 2) | (uint64_t)(slot + 1)));
+}
+
+typedef struct sh_track {
+  int8_t ptype; int32_t id; uint32_t hi; int32_t imc; double ekin, dir[3], safety; uint8_t onb;
+  double nial[4], mir, mdrf, mtm; uint8_t mfs; uint32_t ctr; double gauss; uint8_t hg;
+  double x; int layer, nstep;
+} sh_track;
+
+typedef struct sh_job {
+  const struct G4HepEmData* d; const struct G4HepEmParameters* p; uint64_t seed;
+  const hepem_slab_setup* su; const double* plane; int nbins;
+  int e_begin, e_end;
+  int64_t* dep;        /* [nevents][nbins] fixed point */
+  int64_t* leak;       /* [nevents] */
+  int64_t steps, tracks;
+  int status;
+} sh_job;
+
+static void sh_event(sh_job* J, int ev) {
+  const int nbins = J->nbins, nabs = J->su->nabs;
+  const double* P = J->plane;
+  int cap = 4096, top = 0;
+  sh_track* stk = (sh_track*)malloc((size_t)cap * sizeof(sh_track));
+  if (!stk) { J->status = -4; return; }
+  {
+    sh_track t;
+    memset(&t, 0, sizeof(t));
+    const uint64_t sid = sh_primary_stream(ev);
+    t.ptype = (int8_t)J->su->primary; t.id = (int32_t)(uint32_t)sid; t.hi = (uint32_t)(sid >> 32);
+    t.imc = J->su->imc[0]; t.ekin = J->su->ekin0; t.dir[0] = 1.0; t.onb = 1;
+    for (int k = 0; k < 4; ++k) t.nial[k] = -1.0;
+    t.mir = 1.0e+21; t.mdrf = 0.04; t.mtm = 1.0E-7; t.mfs = 1;
+    stk[top++] = t;
+    J->tracks++;
+  }
+  /* one-track SoA views for step_one */
+  int8_t a_pt; int32_t a_id, a_imc; uint32_t a_hi, a_ctr; double a_e, a_dir[3], a_saf, a_nial[4], a_mir, a_mdrf, a_mtm, a_g;
+  uint8_t a_onb, a_mfs, a_hg;
+  double r_ts, r_gs, r_ed, r_disp[3], r_sek[2], r_sdir[6]; int32_t r_w, r_ns; uint8_t r_st; int8_t r_spt[2];
+  hepem_tracks T = {1, &a_pt, &a_id, &a_imc, &a_e, a_dir, &a_saf, &a_onb, a_nial, &a_mir, &a_mdrf, &a_mtm, &a_mfs,
+                    &a_ctr, &a_g, &a_hg, &a_hi};
+  hepem_step_out R = {&r_ts, &r_gs, &r_ed, r_disp, &r_w, &r_st, &r_ns, r_spt, r_sek, r_sdir};
+  double glim;
+  step_job SJ;
+  SJ.d = J->d; SJ.p = J->p; SJ.seed = J->seed; SJ.in = &T; SJ.out = &T; SJ.res = &R; SJ.geom_limit = &glim;
+  SJ.begin = 0; SJ.end = 1;
+  while (top > 0) {
+    sh_track t = stk[--top];
+    for (;;) {
+      int l = t.layer;
+      const double lo = P[l], hi = P[l + 1];
+      double saf = t.x - lo < hi - t.x ? t.x - lo : hi - t.x;
+      t.safety = saf > 0.0 ? saf : 0.0;
+      a_pt = t.ptype; a_id = t.id; a_hi = t.hi; a_imc = t.imc; a_e = t.ekin;
+      a_dir[0] = t.dir[0]; a_dir[1] = t.dir[1]; a_dir[2] = t.dir[2]; a_saf = t.safety; a_onb = t.onb;
+      for (int k = 0; k < 4; ++k) a_nial[k] = t.nial[k];
+      a_mir = t.mir; a_mdrf = t.mdrf; a_mtm = t.mtm; a_mfs = t.mfs; a_ctr = t.ctr; a_g = t.gauss; a_hg = t.hg;
+      /* geometry: distance to the next plane along the pre-step direction */
+      const double ux = t.dir[0];
+      int sd = 0;
+      glim = 1.0e300;
+      if (ux > 0.0) { glim = (hi - t.x) / ux; sd = 1; }
+      else if (ux < 0.0) { glim = (lo - t.x) / ux; sd = -1; }
+      step_one(&SJ, 0);
+      J->steps++;
+      const int onb = a_onb;
+      /* scoring */
+      if (r_ed > 0.0) J->dep[(size_t)ev * nbins + l] += (int64_t)llrint(r_ed * SH_FIX);
+      double x;
+      if (onb) {
+        x = sd > 0 ? hi : lo;
+      } else {
+        x = t.x + r_gs * ux;
+        x = x < lo ? lo : (x > hi ? hi : x);
+        if (t.ptype != 2) {
+          const double d0 = r_disp[0], d1 = r_disp[1], d2 = r_disp[2];
+          const double dl2 = d0 * d0 + d1 * d1 + d2 * d2;
+          if (dl2 > SH_GEOM_MIN * SH_GEOM_MIN) {
+            const double dispR = sqrt(dl2);
+            double ps = x - lo < hi - x ? x - lo : hi - x;
+            ps = 0.99 * (ps > 0.0 ? ps : 0.0);
+            if (ps > 0.0 && dispR <= ps) x += d0;
+            else if (dispR < ps) x += d0;
+            else if (ps > SH_GEOM_MIN) x += d0 * (ps / dispR);
+          }
+        }
+      }
+      const double ekin = a_e;
+      int alive = ekin > 0.0, left = 0;
+      if (alive && onb) {
+        l += sd;
+        if (l < 0 || l >= nbins) {
+          const double e = ekin + (t.ptype == 1 ? 2.0 * 0.51099895000 : 0.0);
+          J->leak[ev] += (int64_t)llrint(e * SH_FIX);
+          alive = 0; left = 1; l -= sd;
+        }
+      }
+      const uint64_t sid = ((uint64_t)t.hi << 32) | (uint32_t)t.id;
+      const int ns1 = t.nstep + 1;
+      const int ns = left ? 0 : r_ns;
+      /* secondaries: pushed in reverse so that slot 0 is followed first */
+      if (top + ns + 1 > cap) {
+        cap *= 2;
+        sh_track* nstk = (sh_track*)realloc(stk, (size_t)cap * sizeof(sh_track));
+        if (!nstk) { free(stk); J->status = -4; return; }
+        stk = nstk;
+      }
+      for (int s = ns - 1; s >= 0; --s) {
+        sh_track c;
+        memset(&c, 0, sizeof(c));
+        const uint64_t cid = sh_child_stream(sid, ns1, s);
+        c.ptype = r_spt[s]; c.id = (int32_t)(uint32_t)cid; c.hi = (uint32_t)(cid >> 32);
+        c.imc = J->su->imc[l % nabs]; c.ekin = r_sek[s];
+        for (int k = 0; k < 3; ++k) c.dir[k] = r_sdir[k * 2 + s];
+        for (int k = 0; k < 4; ++k) c.nial[k] = -1.0;
+        c.mir = 1.0e+21; c.mdrf = 0.04; c.mtm = 1.0E-7; c.mfs = 1;
+        c.x = x; c.layer = l; c.nstep = 0;
+        stk[top++] = c;
+        J->tracks++;
+      }
+      if (!alive) break;
+      /* the track goes on */
+      t.ekin = ekin; t.dir[0] = a_dir[0]; t.dir[1] = a_dir[1]; t.dir[2] = a_dir[2]; t.onb = (uint8_t)onb;
+      for (int k = 0; k < 4; ++k) t.nial[k] = a_nial[k];
+      t.mir = a_mir; t.mdrf = a_mdrf; t.mtm = a_mtm; t.mfs = a_mfs; t.ctr = a_ctr; t.gauss = a_g; t.hg = a_hg;
+      t.x = x; t.layer = l; t.imc = J->su->imc[l % nabs]; t.nstep = ns1;
+    }
+  }
+  free(stk);
+}
+
+static void* sh_worker(void* arg) {
+  sh_job* J = (sh_job*)arg;
+  for (int e = J->e_begin; e < J->e_end && J->status == 0; ++e) sh_event(J, e);
+  return NULL;
+}
+
+int oracle_slab_shower(const struct G4HepEmData* d, const struct G4HepEmParameters* p, const hepem_slab_setup* su,
+                       int nevents, uint64_t seed, int64_t* dep, int64_t* leak, int64_t* steps_tracks, int nthreads) {
+  if (!d || !p || !su || !dep || !leak || nevents < 0 || su->nabs < 1 || su->nabs > HEPEM_SLAB_MAX_ABS || su->nlayers < 1)
+    return -1;
+  const int nbins = su->nlayers * su->nabs;
+  double* plane = (double*)malloc((size_t)(nbins + 1) * sizeof(double));
+  if (!plane) return -4;
+  plane[0] = 0.0;
+  for (int k = 0; k < nbins; ++k) plane[k + 1] = plane[k] + su->thickness[k % su->nabs];
+  memset(dep, 0, (size_t)nevents * nbins * sizeof(int64_t));
+  memset(leak, 0, (size_t)nevents * sizeof(int64_t));
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if (nthreads > nevents) nthreads = nevents > 0 ? nevents : 1;
+  sh_job jobs[256];
+  pthread_t th[256];
+  for (int k = 0; k < nthreads; ++k) {
+    sh_job* J = &jobs[k];
+    J->d = d; J->p = p; J->seed = seed; J->su = su; J->plane = plane; J->nbins = nbins;
+    J->e_begin = (int)((int64_t)nevents * k / nthreads);
+    J->e_end = (int)((int64_t)nevents * (k + 1) / nthreads);
+    J->dep = dep; J->leak = leak; J->steps = 0; J->tracks = 0; J->status = 0;
+  }
+  for (int k = 0; k < nthreads; ++k) pthread_create(&th[k], NULL, sh_worker, &jobs[k]);
+  int status = 0;
+  int64_t steps = 0, tracks = 0;
+  for (int k = 0; k < nthreads; ++k) {
+    pthread_join(th[k], NULL);
+    if (jobs[k].status) status = jobs[k].status;
+    steps += jobs[k].steps;
+    tracks += jobs[k].tracks;
+  }
+  free(plane);
+  if (steps_tracks) { steps_tracks[0] = steps; steps_tracks[1] = tracks; }
+  return status;
 }

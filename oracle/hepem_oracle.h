@@ -12,6 +12,7 @@
 #include <stdint.h>
 #include "g4hepem_amd_data.h"
 #include "g4hepem_amd.h"
+#include "g4hepem_amd_shower.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -53,6 +54,12 @@ int oracle_step(const struct G4HepEmData* d, const struct G4HepEmParameters* p, 
 int oracle_step_geom(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed,
                      const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res, const double* geom_limit,
                      int nthreads);
+
+/* CPU harness of hepem_slab_run: the same tracking loop, per event on up to
+ * `nthreads` threads. dep: [nevents][nlayers*nabs] and leak: [nevents] in
+ * 1e-9 MeV units; steps_tracks (optional): {track-steps, tracks}. */
+int oracle_slab_shower(const struct G4HepEmData* d, const struct G4HepEmParameters* p, const hepem_slab_setup* su,
+                       int nevents, uint64_t seed, int64_t* dep, int64_t* leak, int64_t* steps_tracks, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,7 @@ TRACK_FIELDS = {
     "num_ia_left": ("float64", 4), "msc_init_range": ("float64", 1),
     "msc_dyn_range_factor": ("float64", 1), "msc_tlimit_min": ("float64", 1),
     "msc_first_step": ("uint8", 1), "rng_ctr": ("uint32", 1), "rng_gauss": ("float64", 1),
-    "rng_has_gauss": ("uint8", 1),
+    "rng_has_gauss": ("uint8", 1), "rng_stream": ("uint32", 1),
 }
 OUT_FIELDS = {
     "true_step": ("float64", 1), "geom_step": ("float64", 1), "edep": ("float64", 1),
@@ -153,6 +153,21 @@ class Oracle(_Base):
         o = np.zeros(n)
         self.L.oracle_rng_uniforms(C.c_uint64(seed), tid, first, n, o.ctypes.data)
         return o
+
+    def slab_shower(self, tables, setup, nevents: int, seed: int, nthreads: int = 1):
+        """CPU harness of hepem_slab_run: per-event per-slab deposits and leakage
+        in MeV (from 1e-9 MeV fixed point), plus (steps, tracks)."""
+        nb = setup.nlayers * setup.nabs
+        dep = np.zeros(nevents * nb, dtype=np.int64)
+        leak = np.zeros(nevents, dtype=np.int64)
+        st = np.zeros(2, dtype=np.int64)
+        f = self.L.oracle_slab_shower
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        f.restype = C.c_int
+        rc = f(tables.data, tables.pars, C.byref(setup), nevents, C.c_uint64(seed), dep.ctypes.data, leak.ctypes.data,
+               st.ctypes.data, nthreads)
+        assert rc == 0, rc
+        return dep.reshape(nevents, nb), leak, int(st[0]), int(st[1])
 
     def vdt_exp(self, x):
         return self.L.oracle_vdt_exp(float(x))

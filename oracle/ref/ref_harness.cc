@@ -107,7 +107,7 @@ SAME(G4HepEmData, fTheGammaData);
 
 // ---- random engine binding ------------------------------------------------------------
 namespace {
-struct Stream { uint32_t key[2]; uint32_t id; uint32_t ctr; };
+struct Stream { uint32_t key[2]; uint32_t id; uint32_t hi; uint32_t ctr; };
 
 inline void Philox(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
   uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3], k0 = key_in[0], k1 = key_in[1];
@@ -121,7 +121,7 @@ inline void Philox(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t 
 }
 inline double Draw(Stream* s) {
   const uint32_t d = s->ctr++;
-  const uint32_t c[4] = {d >> 1, s->id, 0u, 0u};
+  const uint32_t c[4] = {d >> 1, s->id, s->hi, 0u};
   uint32_t o[4];
   Philox(c, s->key, o);
   const int j = int(d & 1u);
@@ -160,7 +160,8 @@ void StepRange(G4HepEmData* d, G4HepEmParameters* p, uint64_t seed, const mine::
   const int n = in->n;
   G4HepEmTLData tl;
   for (int i = begin; i < end; ++i) {
-    Stream st{{uint32_t(seed), uint32_t(seed >> 32)}, uint32_t(in->id[i]), in->rng_ctr[i]};
+    Stream st{{uint32_t(seed), uint32_t(seed >> 32)}, uint32_t(in->id[i]), in->rng_stream ? in->rng_stream[i] : 0u,
+              in->rng_ctr[i]};
     G4HepEmRandomEngine eng(&st);
     EngineMirror* em = reinterpret_cast<EngineMirror*>(&eng);
     em->isGauss = in->rng_has_gauss[i] != 0;
@@ -229,6 +230,7 @@ void StepRange(G4HepEmData* d, G4HepEmParameters* p, uint64_t seed, const mine::
     if (out != in) {
       out->ptype[i] = in->ptype[i];
       out->id[i] = in->id[i];
+      if (out->rng_stream) out->rng_stream[i] = in->rng_stream ? in->rng_stream[i] : 0u;
       out->imc[i] = in->imc[i];
       out->safety[i] = in->safety[i];
     }

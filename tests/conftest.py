@@ -48,8 +48,15 @@ def oracle():
 
 
 def golden_tracks(g, prefix):
+    """Track state stored in a golden file; fields added after it was written
+    (rng_stream: 0) get their neutral value."""
     from oracle.oracle_py import TRACK_FIELDS
-    return {k: g[f"{prefix}_{k}"] for k in TRACK_FIELDS}
+    n = len(g[f"{prefix}_ekin"])
+    out = {}
+    for k, (dt, c) in TRACK_FIELDS.items():
+        key = f"{prefix}_{k}"
+        out[k] = g[key] if key in g.files else np.zeros(c * n, dtype=dt)
+    return out
 
 
 def same_bits(a, b):

@@ -12,6 +12,7 @@ and the per-track Philox streams, and stores inputs and outputs:
 * one HowFar+Perform step of fresh e-/e+/gamma tracks in three energy regimes;
 * a 4-step chain (each step starting from the previous state) with tracks on
   and off volume boundaries;
+* one step of tracks with 64-bit stream selectors (nonzero rng_stream);
 * (golden_geom_v1.npz) a 3-step chain with a geometry between HowFar and
   Perform: each track's geometrical step is cut at a given distance, a cut
   step ends on a boundary and the next step starts from there.
@@ -101,6 +102,17 @@ def main():
         for k in ("ekin", "dir", "num_ia_left", "rng_ctr", "rng_gauss", "rng_has_gauss", "msc_init_range",
                   "msc_dyn_range_factor", "msc_tlimit_min", "msc_first_step"):
             g[f"ch{s}_out_{k}"] = cur[k]
+
+    # one step with 64-bit stream selectors (rng_stream:id)
+    tr = step_tracks(512, 650, len(SET2), lnemin=math.log(1e-2), lnemax=math.log(1e5), first_id=60000)
+    tr["rng_stream"][:] = np.random.Generator(np.random.Philox(651)).integers(0, 2**32, 512, dtype=np.uint32)
+    to, res = ref.step(t2, STEP_SEED + 3, tr, 8)
+    for k, v in tr.items():
+        g[f"hs_in_{k}"] = v
+    for k, v in res.items():
+        g[f"hs_res_{k}"] = v
+    for k, v in to.items():
+        g[f"hs_out_{k}"] = v
 
     np.savez_compressed(OUT, **g)
     print(f"wrote {OUT} ({OUT.stat().st_size / 1024:.0f} KiB); set1 {t1.digest()[:16]} set2 {t2.digest()[:16]}")

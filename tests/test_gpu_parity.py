@@ -160,6 +160,16 @@ def test_step_chain_golden(golden, ctx2, dev):
             cur[k] = ref_t[k]
 
 
+def test_step_stream_high_word_golden(golden, ctx2, dev):
+    _print("step golden with 64-bit stream selectors")
+    tin = golden_tracks(golden, "hs_in")
+    t, res, nrej = gpu_step(ctx2, STEP_SEED + 3, tin, dev)
+    assert nrej == 0
+    ref_t = {k: golden[f"hs_out_{k}"] for k in TRACK_FIELDS}
+    ref_r = {k: golden[f"hs_res_{k}"] for k in OUT_FIELDS}
+    compare_step(tin, t, res, ref_t, ref_r, label="stream-hi")
+
+
 def test_step_parity_1M(tables2, ctx2, oracle, dev):
     """The full config-2 shape: 1M tracks, e-:e+:gamma = 4:1:5, 100 keV-10 GeV."""
     from g4hepem_amd.tracks import step_tracks

@@ -1,0 +1,73 @@
+"""Showers in the 50 x (2.3 mm Pb + 5.7 mm lAr) slab stack (hepem_slab_run)
+against the oracle's CPU harness of the same tracking loop, event by event:
+deposits are per-track deterministic (per-track streams, integer scoring), so
+most events agree exactly and the per-slab means agree to statistics (chi2)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+LAYERS, THICK = 50, (2.3, 5.7)
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx1(tables1):
+    assert torch.cuda.is_available(), "GPU test run without a visible GPU"
+    from g4hepem_amd import HEPEM_FP64, Context
+    return Context(tables1, HEPEM_FP64)
+
+
+def chi2_means(a, b):
+    """chi2 of the per-slab means of two event samples (rows = events)."""
+    ma, mb = a.mean(0), b.mean(0)
+    va, vb = a.var(0, ddof=1) / len(a), b.var(0, ddof=1) / len(b)
+    v = va + vb
+    m = v > 0
+    return float((((ma - mb) ** 2)[m] / v[m]).sum()), int(m.sum())
+
+
+def test_shower_matches_cpu_harness(tables1, ctx1, oracle):
+    from g4hepem_amd._native import slab_setup
+    nev, e0 = 200, 1000.0
+    su = slab_setup(LAYERS, THICK, [0, 1], 0, e0, capacity=1 << 21, events_per_chunk=64)
+    _print("shower 1 GeV e-: gpu")
+    g = ctx1.slab_run(su, nev, 77, per_event=True)
+    _print(f"  steps {g['steps']} tracks {g['tracks']} iterations {g['iterations']} max in flight {g['max_in_flight']}")
+    _print("shower 1 GeV e-: cpu harness")
+    dep, leak, steps, tracks = oracle.slab_shower(tables1, su, nev, 77, nthreads=16)
+    # energy balance of every event on both sides (fixed-point rounding only)
+    for d, lk in ((g["edep_event"], g["leak_event"]), (dep, leak)):
+        bal = (d.sum(1) + lk) / 1e9
+        assert np.abs(bal - e0).max() < 1e-4
+    same = np.all(g["edep_event"] == dep, axis=1) & (g["leak_event"] == leak)
+    _print(f"  events identical to the CPU harness: {same.mean():.3f}")
+    assert same.mean() > 0.5
+    chi2, ndf = chi2_means(g["edep_event"] / 1e9, dep / 1e9)
+    _print(f"  chi2/ndf of per-slab means: {chi2:.1f}/{ndf}")
+    assert chi2 < ndf + 5 * np.sqrt(2 * ndf)
+    # sums reported by the run equal the per-event records
+    assert np.allclose(g["edep_sum"], (g["edep_event"] / 1e9).sum(0), rtol=1e-12, atol=1e-9)
+    assert abs(g["leak_sum"] - g["leak_event"].sum() / 1e9) < 1e-6
+    assert abs(steps - g["steps"]) < 0.02 * steps
+
+
+def test_shower_independent_of_chunking(ctx1):
+    from g4hepem_amd._native import slab_setup
+    a = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 2, 500.0, capacity=1 << 20, events_per_chunk=7), 40, 5,
+                      per_event=True)
+    b = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 2, 500.0, capacity=1 << 20, events_per_chunk=40), 40, 5,
+                      per_event=True)
+    assert np.array_equal(a["edep_event"], b["edep_event"]) and np.array_equal(a["leak_event"], b["leak_event"])
+    assert a["steps"] == b["steps"] and a["tracks"] == b["tracks"]
+
+
+def test_shower_capacity_error(ctx1):
+    from g4hepem_amd import HepEmError
+    from g4hepem_amd._native import slab_setup
+    with pytest.raises(HepEmError):
+        ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 0, 10000.0, capacity=64, events_per_chunk=4), 4, 1)

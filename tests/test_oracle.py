@@ -80,6 +80,22 @@ def test_step_chain_golden(golden, tables2, oracle):
     assert golden["ch1_out_rng_has_gauss"].sum() > 0
 
 
+def test_stream_high_word_golden(golden, tables2, oracle):
+    """Tracks selecting their Philox stream by the 64-bit pair (rng_stream, id)."""
+    tin = golden_tracks(golden, "hs_in")
+    assert np.count_nonzero(tin["rng_stream"]) > 500
+    tout, res = oracle.step(tables2, STEP_SEED + 3, tin, nthreads=4)
+    for k in OUT_FIELDS:
+        assert same_bits(res[k], golden[f"hs_res_{k}"]), k
+    for k in ("ekin", "dir", "num_ia_left", "rng_ctr", "rng_gauss", "rng_stream"):
+        assert same_bits(tout[k], golden[f"hs_out_{k}"]), k
+    # the high word changes the stream: the same tracks with rng_stream = 0 differ
+    t0 = dict(tin)
+    t0["rng_stream"] = np.zeros_like(tin["rng_stream"])
+    _, res0 = oracle.step(tables2, STEP_SEED + 3, t0, nthreads=4)
+    assert not same_bits(res0["edep"], res["edep"])
+
+
 GEOM_STATE = ("ekin", "dir", "num_ia_left", "rng_ctr", "rng_gauss", "rng_has_gauss", "msc_init_range",
               "msc_dyn_range_factor", "msc_tlimit_min", "msc_first_step", "on_boundary")
 
