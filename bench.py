@@ -14,8 +14,9 @@ a separate output batch.
 Also reported: the FP32-table form of the same step, config-1 style lookups
 (100k and 10M e- queries over G4_Pb/G4_lAr), 64 uneven batches (sizes
 log-uniform in [1e3, 1e6], ~9.3M tracks) stepped by one tuned batch plan
-replayed from a hipGraph in both forms, and the oracle's CPU throughput on the
-headline workload on the host cores.
+replayed from a hipGraph in both forms, 10 GeV e- showers in 50 x (2.3 mm Pb +
+5.7 mm lAr) (10k events, with the CPU harness on a sample), and the oracle's
+CPU throughput on the headline workload on the host cores.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -162,6 +163,38 @@ def time_batched(ctx, batches, seed, reps):
     return ms, shape
 
 
+def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int):
+    """10 GeV e- into 50 x (2.3 mm Pb + 5.7 mm lAr) at 0.7 mm: GPU showers of
+    `nevents` events (wall clock of the whole driver, host syncs included), and
+    the oracle's CPU harness on the first `cpu_events` events for the per-slab
+    chi2 and the fraction of events identical to the GPU's."""
+    from g4hepem_amd._native import slab_setup
+    su = slab_setup(50, [2.3, 5.7], [0, 1], 0, 1.0e4, capacity=1 << 23, events_per_chunk=1024)
+    t0 = time.perf_counter()
+    g = ctx.slab_run(su, nevents, seed, per_event=True)
+    wall = time.perf_counter() - t0
+    ev = g["edep_event"] / 1e9
+    mean, rms = ev.mean(0), ev.std(0)
+    out = {"events": nevents, "wall_s": wall, "events_per_s": nevents / wall,
+           "track_steps_per_s": g["steps"] / wall, "steps": g["steps"], "tracks": g["tracks"],
+           "iterations": g["iterations"], "max_in_flight": g["max_in_flight"],
+           "mean_edep_total_MeV": float(ev.sum(1).mean()), "mean_leak_MeV": float(g["leak_event"].mean() / 1e9),
+           "layer_mean_MeV": [round(float(v), 4) for v in mean], "layer_rms_MeV": [round(float(v), 4) for v in rms]}
+    if cpu_events > 0:
+        from oracle.oracle_py import Oracle
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        t0 = time.perf_counter()
+        dep, leak, steps, _ = Oracle().slab_shower(tables, su, cpu_events, seed, nthreads=max(1, min(int(cores), 64)))
+        cwall = time.perf_counter() - t0
+        a, b = g["edep_event"][:cpu_events] / 1e9, dep / 1e9
+        v = a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b)
+        m = v > 0
+        out["cpu_harness"] = {"events": cpu_events, "wall_s": cwall, "track_steps_per_s": steps / cwall,
+                              "chi2": float((((a.mean(0) - b.mean(0)) ** 2)[m] / v[m]).sum()), "ndf": int(m.sum()),
+                              "identical_events": float(np.all(g["edep_event"][:cpu_events] == dep, axis=1).mean())}
+    return out
+
+
 def time_lookups(ctx, n, reps, seed=2026):
     import torch
     from g4hepem_amd.tracks import lookup_queries
@@ -223,6 +256,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline FP64 step only (for profiling)")
+    ap.add_argument("--shower-events", type=int, default=10000, help="config-5 events (0: skip)")
+    ap.add_argument("--shower-cpu-events", type=int, default=100, help="config-5 events for the CPU harness")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="before the steps, copy a buffer of this many MiB device-to-device "
                          "(reference kernel of known traffic for hardware byte counters)")
@@ -317,6 +352,11 @@ def main():
             "speedup_fp32_vs_fp64": ms_b64 / ms_b32,
         }
         del dev_batches
+    if not args.no_extras and args.shower_events > 0:
+        log("config-5 slab showers")
+        tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr
+        line["config5_slab_shower"] = run_config5(Context(tables_slab, HEPEM_FP64), tables_slab, args.seed,
+                                                  args.shower_events, 0 if args.no_cpu else args.shower_cpu_events)
     if not args.no_cpu:
         log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(tables2, tracks, args.cpu_seconds, args.seed)

@@ -58,6 +58,22 @@ def build_native() -> Path:
     return LIB
 
 
+CPP_TEST_SRC = ROOT / "tests" / "cpp" / "test_managers.cpp"
+CPP_TEST_BIN = ROOT / "tests" / "cpp" / "test_managers"
+
+
+def build_cpp_tests() -> Path:
+    """The C++ host-layer test program (include/g4hepem_amd.hpp), linked
+    against the in-tree library."""
+    tmp = CPP_TEST_BIN.with_suffix(".tmp")
+    cmd = [_hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", f"-I{ROOT / 'include'}", "-o", tmp, CPP_TEST_SRC, f"-L{PKG}", "-lg4hepem_amd",
+           "-Wl,-rpath,$ORIGIN/../../g4hepem_amd"]
+    _run(cmd)
+    os.replace(tmp, CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 def build_oracle() -> None:
     make = shutil.which("make") or "make"
     _run([make, "-s", "-C", ROOT / "oracle", "all"])
@@ -67,9 +83,10 @@ def build_oracle() -> None:
 
 def build_all() -> None:
     build_native()
+    build_cpp_tests()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"all": build_all, "native": build_native, "oracle": build_oracle}[what]()
+    {"all": build_all, "native": build_native, "oracle": build_oracle, "cpp": build_cpp_tests}[what]()
