@@ -29,9 +29,14 @@ template <typename R> HD void Rotate(R* dir, const R* ref) {  // RotateToReferen
 }
 // Interpolation: abscissas x, x1, x2 are double in both forms (see DevEl),
 // ordinates and second derivatives have the form's precision R.
+// Fraction b = (x - x1) / (x2 - x1) clamped to [0, 1]: both differences in
+// double (they cancel near thresholds); the quotient in the form's precision
+// (the FP64 form evaluates the reference's expression as written).
+HD double Frac(double dx, double dl, double) { return tmax(0., tmin(1., dx / dl)); }
+HD float Frac(double dx, double dl, float) { return tmax(0.f, tmin(1.f, (float)dx / (float)dl)); }
 template <typename R> HD R Spline(double x1, double x2, R y1, R y2, R s1, R s2, double x) {  // GetSpline :49-61
   const double dl = x2 - x1;
-  const R b = (R)tmax(0., tmin(1., (x - x1) / dl));
+  const R b = Frac(x - x1, dl, R());
   const R os = (R)0.166666666667;
   const R c0 = ((R)2.0 - b) * s1;
   const R c1 = ((R)1.0 + b) * s2;
@@ -39,7 +44,7 @@ template <typename R> HD R Spline(double x1, double x2, R y1, R y2, R s1, R s2, 
 }
 template <typename R> HD R Linear(double x1, double x2, R y1, R y2, double x) {  // GetLinear :63-71
   const double dl = x2 - x1;
-  const R b = (R)tmax(0., tmin(1., (x - x1) / dl));
+  const R b = Frac(x - x1, dl, R());
   return y1 + b * (y2 - y1);
 }
 template <typename R> HD int LogBin(R logx, R logxmin, R invLD, int ndata) {
@@ -69,7 +74,7 @@ HD void LinearLog2All(int nd, const double* dx, const R* d, double x, R logx, R 
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i0 = 3 * idx, i1 = i0 + 3;
   const double dl = dx[i1] - dx[i0];
-  const R b = (R)tmax(0., tmin(1., (x - dx[i0]) / dl));
+  const R b = Frac(x - dx[i0], dl, R());
 #pragma unroll
   for (int i = 1; i < 3; ++i) {
     const R y1 = d[i0 + i], y2 = d[i1 + i];
@@ -600,7 +605,7 @@ HD int BremSelectTargetAtom(const DevEl<R>& e, int imc, double ekin, R lekin, R 
   int i0 = idxEkin * numElem, i1 = i0 + numElem;
   const double x1 = xd64[i0++], x2 = xd64[i1++];
   const double dl = x2 - x1;
-  const R b = (R)tmax(0., tmin(1., (xv - x1) / dl));
+  const R b = Frac(xv - x1, dl, R());
   int ie = 0;
   while (ie < numElem - 1 && urndn > xd[i0 + ie] + b * (xd[i1 + ie] - xd[i0 + ie])) ++ie;
   return ie;
@@ -1274,7 +1279,7 @@ HD int ConvSelectTargetAtom(const DevTables<R>& T, int imat, double ekin, R leki
   const int idx = (int)tmax((R)0.0, tmin((lekin - T.convSelLogMin) * T.convSelILD, (R)(numData - 2)));
   const double x1 = xd[idx], x2 = xd[idx + 1];
   const double dl = x2 - x1;
-  const R b = (R)tmax(0., tmin(1., (xv - x1) / dl));
+  const R b = Frac(xv - x1, dl, R());
   const int i0 = idx * (numElem - 1) + 1;
   const int i1 = i0 + (numElem - 1);
   int ie = 0;

@@ -263,6 +263,7 @@ struct hepem_ctx {
   DevTables<float> t32;
   void* work = nullptr;       // internal work buffer for callers passing work == NULL
   uint64_t workBytes = 0;
+  int processSort = 1;        // hepem_step: sort pending interactions by process (two stages)
 };
 
 extern "C" const char* hepem_status_string(int s) {
@@ -347,19 +348,54 @@ bool ResOk(const hepem_step_out* r, int n) {
 }
 }  // namespace
 
-// work layout: 64 counters (256 bytes) | queue[n]
-extern "C" uint64_t hepem_step_work_bytes(int32_t n) { return 256 + 4ull * (uint64_t)(n > 0 ? n : 0); }
+// work layout (each part at 256-byte alignment): 64 counters | particle queue
+// (n x u32) | process code per slot (n x u8) | PE cross section per slot
+// (n x f64) | process queue (kNumProc regions of n x u32)
+namespace {
+constexpr uint64_t kCountersBytes = 256;
+uint64_t Al(uint64_t b) { return (b + 255) & ~255ull; }
+struct WorkLayout {
+  uint32_t* counters;
+  uint32_t* queue;
+  uint8_t* code;
+  double* pe;
+  uint32_t* pqueue;
+  WorkLayout(void* w, int32_t n) {
+    uint8_t* p = static_cast<uint8_t*>(w);
+    const uint64_t m = (uint64_t)(n > 0 ? n : 0);
+    counters = reinterpret_cast<uint32_t*>(p);
+    p += kCountersBytes;
+    queue = reinterpret_cast<uint32_t*>(p);
+    p += Al(4 * m);
+    code = p;
+    p += Al(m);
+    pe = reinterpret_cast<double*>(p);
+    p += Al(8 * m);
+    pqueue = reinterpret_cast<uint32_t*>(p);
+  }
+};
+}  // namespace
+
+extern "C" uint64_t hepem_step_work_bytes(int32_t n) {
+  const uint64_t m = (uint64_t)(n > 0 ? n : 0);
+  return kCountersBytes + Al(4 * m) + Al(m) + Al(8 * m) + Al(4 * m * kNumProc);
+}
+
+extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return HEPEM_ERR_INVALID;
+  c->processSort = on;
+  return HEPEM_OK;
+}
 
 extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
   if (!c || !TracksOk(in) || !work) return HEPEM_ERR_INVALID;
   const int n = in->n;
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  uint32_t* counters = static_cast<uint32_t*>(work);
-  uint32_t* queue = counters + 64;
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters, 64, 0u);
+  const WorkLayout w(work, n);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, w.counters, 64, 0u);
   if (n > 0) {
     const int grid = (n + kClassifyTile - 1) / kClassifyTile;
-    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kBlock), 0, s, n, in->ptype, queue, counters);
+    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kBlock), 0, s, n, in->ptype, w.queue, w.counters);
   }
   HIP_OK(hipGetLastError());
   return HEPEM_OK;
@@ -371,13 +407,30 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
   const int n = in->n;
   if (n == 0) return HEPEM_OK;
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  uint32_t* counters = static_cast<uint32_t*>(work);
-  const uint32_t* queue = counters + 64;
+  const WorkLayout w(work, n);
   const int grid = (n + kBlock - 1) / kBlock;
-  if (c->precision == HEPEM_FP64)
-    hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, queue, counters);
-  else
-    hipLaunchKernelGGL(k_step<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, queue, counters);
+  const bool fp64 = c->precision == HEPEM_FP64;
+  if (!c->processSort) {
+    if (fp64)
+      hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue, w.counters);
+    else
+      hipLaunchKernelGGL(k_step<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue, w.counters);
+  } else {
+    const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
+    if (fp64) {
+      hipLaunchKernelGGL(k_step_select<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue,
+                         w.counters, w.code, w.pe);
+      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, w.code, w.pqueue, w.counters);
+      hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *out, *res, w.code, w.pe, w.pqueue,
+                         w.counters);
+    } else {
+      hipLaunchKernelGGL(k_step_select<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue,
+                         w.counters, w.code, w.pe);
+      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, w.code, w.pqueue, w.counters);
+      hipLaunchKernelGGL(k_interact<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *out, *res, w.code, w.pe, w.pqueue,
+                         w.counters);
+    }
+  }
   HIP_OK(hipGetLastError());
   return HEPEM_OK;
 }
@@ -430,8 +483,9 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
   int rc = EnsureWork(c, n, work);
   if (rc != HEPEM_OK) return rc;
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  uint32_t* counters = static_cast<uint32_t*>(work);
-  const uint32_t* queue = KIND == kMixed ? counters + 64 : nullptr;
+  const WorkLayout wl(work, n);
+  uint32_t* counters = wl.counters;
+  const uint32_t* queue = KIND == kMixed ? wl.queue : nullptr;
   if (!perform) {
     if (KIND == kMixed) {
       rc = hepem_step_prepare(c, t, work, stream);

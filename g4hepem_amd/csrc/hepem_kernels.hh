@@ -20,6 +20,7 @@
 namespace hepem {
 
 constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
 
 // counters[0]: e-/e+ tracks, counters[1]: gammas, counters[2]: rejected tracks
 // (counters[3..63] reserved)
@@ -504,6 +505,279 @@ __global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, 
   CopyIdentity(in, out, i);
   if ((uint32_t)j < nE) StepElectron(T, seed, in, out, res, n, i);
   else StepGamma(T, seed, in, out, res, n, i);
+}
+
+// ---- two-stage step: particle sort, then process sort ----------------------------------------
+// k_step_select does HowFar, the continuous step and the choice of the discrete
+// interaction (one thread per particle-queue entry) and records the chosen
+// process per slot; k_classify_proc orders the pending slots by process into
+// one region per process; k_interact samples the final states, so each wave
+// runs one sampler (only the waves that straddle two regions diverge).
+// Between the stages the state travels through the output arrays.
+
+// processes of the second stage (0: none pending)
+enum ProcCode : uint8_t {
+  kPNone = 0,
+  kPIoniEl,    // Moller
+  kPIoniPos,   // Bhabha
+  kPBremSB,    // Seltzer-Berger brem (E < model limit)
+  kPBremRB,    // relativistic brem
+  kPAnnih,     // e+ annihilation in flight
+  kPConv,      // gamma conversion
+  kPCompton,   // Klein-Nishina
+  kPPE,        // photoelectric
+};
+constexpr int kNumProc = 8;          // codes 1..8
+constexpr int kProcCountBase = 16;   // counters[16 + c]: slots pending process c
+
+// Stage 1 of an e-/e+ slot: as StepElectron up to PerformDiscrete's choice
+// (G4HepEmElectronManager.icc:425-441). Returns the process still to sample;
+// a complete step (kPNone) also gets its secondaries and stopped flag here.
+template <typename R>
+HD uint8_t StepElectronSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
+                              const hepem_step_out& res, int n, int i) {
+  ElState<R> el;
+  Rng rng;
+  InitElectron(in, n, i, seed, el, rng);
+  Trk<R>& t = el.t;
+  SecList<R> sl;
+  sl.ns = 0;
+  uint8_t code = kPNone;
+  ElectronHowFar(T, el, rng);
+  const int cont = ElectronPerformContinuous(T, el, rng);
+  if (cont == 1) {
+    if (!t.isEl) AnnihilateAtRest(sl, rng);
+  } else if (cont == 0) {
+    const int ip = ElectronSelectDiscrete(T, t, rng);
+    code = ip == 0   ? (t.isEl ? kPIoniEl : kPIoniPos)
+           : ip == 1 ? (t.ekin < T.bremModelLim ? kPBremSB : kPBremRB)
+           : ip == 2 ? kPAnnih
+                     : kPNone;  // -1, or e-/e+ nuclear (no final state)
+  }
+  StoreElectronState(out, n, i, el, rng);
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+  res.true_step[i] = (double)el.pstep;
+  res.geom_step[i] = (double)t.gstep;
+  res.edep[i] = t.edep;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = (double)el.msc.displ[c];
+  res.winner[i] = t.winner;
+  if (code == kPNone) {
+    res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+    WriteSecondaries(res, n, i, sl);
+  }
+  return code;
+}
+
+// Stage 1 of a gamma slot: HowFar, SelectInteraction and the interaction-
+// length update of Perform (G4HepEmGammaManager.icc:27-62,173-219).
+template <typename R>
+HD uint8_t StepGammaSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
+                           const hepem_step_out& res, int n, int i, double& peOut) {
+  Trk<R> t;
+  t.isEl = false;
+  LoadTrack(in, n, i, t);
+  Rng rng;
+  rng.init(seed, (uint32_t)in.id[i], StreamHi(in, i), in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
+  R pe = GammaHowFar(T, t, rng);
+  uint8_t code = kPNone;
+  if (t.onBoundary) {
+    t.nial[0] -= t.gstep / t.mfp[0];
+  } else {
+    SampleInteraction(T, t, pe, (R)rng.flat());
+    if (t.winner != 3) {
+      t.nial[0] -= t.gstep / t.mfp[0];
+      ResetNumIALeft(t.nial, t.winner);
+      code = t.winner == 0 ? kPConv : t.winner == 1 ? kPCompton : kPPE;
+    }
+  }
+  peOut = (double)pe;
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out.num_ia_left[k * n + i] = (double)t.nial[k];
+  StoreRng(out, i, rng);
+  res.true_step[i] = (double)t.gstep;
+  res.geom_step[i] = (double)t.gstep;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = 0.0;
+  res.winner[i] = t.winner;
+  if (code == kPNone) {
+    res.edep[i] = t.edep;
+    res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+    SecList<R> sl;
+    sl.ns = 0;
+    WriteSecondaries(res, n, i, sl);
+  }
+  return code;
+}
+
+// Stage 2 of an e-/e+ slot: the final state of the chosen interaction
+// (PerformDiscrete, G4HepEmElectronManager.icc:443-458) from the post-step
+// state stage 1 left in `out`.
+template <typename R>
+HD void ElectronInteract(const DevTables<R>& T, uint64_t seed, hepem_tracks& out, const hepem_step_out& res, int n, int i,
+                         uint8_t code) {
+  Trk<R> t;
+  t.isEl = out.ptype[i] == 0;
+  t.setE(out.ekin[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c] = (R)out.dir[c * n + i];
+  t.imc = out.imc[i];
+  Rng rng;
+  rng.init(seed, (uint32_t)out.id[i], StreamHi(out, i), out.rng_ctr[i], out.rng_has_gauss[i] != 0, out.rng_gauss[i]);
+  SecList<R> sl;
+  sl.ns = 0;
+  switch (code) {
+    case kPIoniEl:
+    case kPIoniPos: IoniPerform(T, t, sl, rng); break;
+    case kPBremSB: BremPerform(T, t, sl, rng, true); break;
+    case kPBremRB: BremPerform(T, t, sl, rng, false); break;
+    default: AnnihilateInFlight(t, sl, rng); break;
+  }
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+  StoreRng(out, i, rng);
+  res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+  WriteSecondaries(res, n, i, sl);
+}
+
+// Stage 2 of a gamma slot: the interaction and the gamma tracking cut of
+// G4HepEmGammaManager::Perform (G4HepEmGammaManager.icc:63-93).
+template <typename R>
+HD void GammaInteract(const DevTables<R>& T, uint64_t seed, hepem_tracks& out, const hepem_step_out& res, int n, int i,
+                      uint8_t code, double peD) {
+  Trk<R> t;
+  t.isEl = false;
+  t.setE(out.ekin[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c] = (R)out.dir[c * n + i];
+  t.imc = out.imc[i];
+  t.edep = 0.0;
+  Rng rng;
+  rng.init(seed, (uint32_t)out.id[i], StreamHi(out, i), out.rng_ctr[i], out.rng_has_gauss[i] != 0, out.rng_gauss[i]);
+  SecList<R> sl;
+  sl.ns = 0;
+  switch (code) {
+    case kPConv: ConvPerform(T, t, sl, rng); break;
+    case kPCompton: ComptonPerform(t, sl, rng); break;
+    default: PEPerform(T, t, (R)peD, sl, rng); break;
+  }
+  const double finalEkin = t.ekin;
+  if (finalEkin > 0.0 && finalEkin <= T.gamTrackCut) {
+    t.setE(0.0);
+    t.edep += finalEkin;
+  }
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+  StoreRng(out, i, rng);
+  res.edep[i] = t.edep;
+  res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+  WriteSecondaries(res, n, i, sl);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
+                                                        hepem_step_out res, const uint32_t* __restrict__ queue,
+                                                        uint32_t* __restrict__ counters, uint8_t* __restrict__ code,
+                                                        double* __restrict__ pe) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = in.n;
+  if (j >= n) return;
+  const uint32_t nE = counters[0];
+  const int i = (int)queue[j];
+  if (RejectTrack(T, in, out, res, n, i, counters)) {
+    code[i] = kPNone;
+    return;
+  }
+  CopyIdentity(in, out, i);
+  if ((uint32_t)j < nE) {
+    code[i] = StepElectronSelect(T, seed, in, out, res, n, i);
+  } else {
+    double p = 0.0;
+    const uint8_t c = StepGammaSelect(T, seed, in, out, res, n, i, p);
+    code[i] = c;
+    if (c == kPPE) pe[i] = p;
+  }
+}
+
+// Orders the pending slots by process: region c-1 of pqueue (n entries each)
+// receives the slots with code c; counters[kProcCountBase + c] counts them.
+// Tiles of kClassifyTile slots: wave ballots per process, an LDS scan, and one
+// atomic per tile and process.
+__global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* __restrict__ code,
+                                                          uint32_t* __restrict__ pqueue, uint32_t* __restrict__ counters) {
+  __shared__ uint32_t sW[kNumProc][kWaves], sBase[kNumProc];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int tile0 = blockIdx.x * kClassifyTile;
+  uint32_t cnt[kNumProc];
+#pragma unroll
+  for (int c = 0; c < kNumProc; ++c) cnt[c] = 0u;
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int i = tile0 + it * kBlock + threadIdx.x;
+    const int c = i < n ? (int)code[i] : 0;
+#pragma unroll
+    for (int p = 0; p < kNumProc; ++p) cnt[p] += (uint32_t)__popcll(__ballot(c == p + 1));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int p = 0; p < kNumProc; ++p) sW[p][wave] = cnt[p];
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumProc) {
+    const int p = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t v = sW[p][w];
+      sW[p][w] = tot;
+      tot += v;
+    }
+    sBase[p] = tot ? atomicAdd(&counters[kProcCountBase + p + 1], tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos[kNumProc];
+#pragma unroll
+  for (int p = 0; p < kNumProc; ++p) pos[p] = sBase[p] + sW[p][wave];
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int i = tile0 + it * kBlock + threadIdx.x;
+    const int c = i < n ? (int)code[i] : 0;
+#pragma unroll
+    for (int p = 0; p < kNumProc; ++p) {
+      const uint64_t m = __ballot(c == p + 1);
+      if (c == p + 1) pqueue[(size_t)p * n + pos[p] + __popcll(m & below)] = (uint32_t)i;
+      pos[p] += (uint32_t)__popcll(m);
+    }
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_interact(DevTables<R> T, uint64_t seed, hepem_tracks out, hepem_step_out res,
+                                                     const uint8_t* __restrict__ code, const double* __restrict__ pe,
+                                                     const uint32_t* __restrict__ pqueue,
+                                                     const uint32_t* __restrict__ counters) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = out.n;
+  // entry j of the concatenation of the process regions
+  uint32_t first = 0;
+  int p = -1;
+#pragma unroll
+  for (int q = 0; q < kNumProc; ++q) {
+    const uint32_t c = counters[kProcCountBase + q + 1];
+    if (p < 0 && (uint32_t)j < first + c) p = q;
+    if (p < 0) first += c;
+  }
+  if (p < 0) return;
+  const int i = (int)pqueue[(size_t)p * n + (j - first)];
+  const uint8_t c = (uint8_t)(p + 1);
+  if (c < kPConv) ElectronInteract(T, seed, out, res, n, i, c);
+  else GammaInteract(T, seed, out, res, n, i, c, pe[i]);
 }
 
 // ---- batched stepping ---------------------------------------------------------------------------

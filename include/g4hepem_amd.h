@@ -172,11 +172,18 @@ uint64_t hepem_step_work_bytes(int32_t n);
 /* The two phases of hepem_step, for callers that time them separately:
  * hepem_step_prepare clears the step counters and compacts the track slots
  * by particle type into the queue held in `work` (e-/e+ first); hepem_step_run
- * performs the physics step of every queued slot. `work` is required here.
- * hepem_step == prepare + run on the same stream. */
+ * performs the physics step of every queued slot: HowFar, the continuous step
+ * and the choice of the discrete interaction, then (by default) a second sort
+ * of the pending interactions by process and the final-state sampling with
+ * whole waves per process. `work` is required here. hepem_step == prepare +
+ * run on the same stream. */
 int  hepem_step_prepare(const hepem_ctx* ctx, const hepem_tracks* in, void* work, void* hip_stream);
 int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
                     hepem_step_out* res, void* work, void* hip_stream);
+/* hepem_step_run in one fused kernel (on = 0) or with the process sort
+ * (on = 1, the default). Results are identical; only the speed differs. */
+int  hepem_ctx_set_process_sort(hepem_ctx* ctx, int on);
+
 /* Number of tracks the last step on `work` rejected (invalid couple index or
  * non-positive/non-finite energy; such tracks get winner -99 and are left
  * unchanged). Synchronises `hip_stream`. */

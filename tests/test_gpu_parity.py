@@ -197,6 +197,29 @@ def test_step_wide_energy_parity(tables2, ctx2, oracle, dev):
         compare_step(tr, t, res, ref_t, ref_r, label=f"wide{k}")
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_process_sort_equals_fused(tables2, dev, form):
+    """The two-stage step (particle sort, then process sort of the pending
+    interactions) equals the single fused kernel bit for bit."""
+    from g4hepem_amd import HEPEM_FP32T, HEPEM_FP64, Context
+    from g4hepem_amd.tracks import step_tracks
+    ctx = Context(tables2, HEPEM_FP64 if form == "fp64" else HEPEM_FP32T)
+    tr = step_tracks(300000, 71, 4, lnemin=math.log(1e-3), lnemax=math.log(1e7))
+    tr["on_boundary"][:] = (np.arange(300000) % 9 == 4).astype(np.uint8)
+    tr["imc"][17] = 9  # one rejected track
+    ctx.set_process_sort(True)
+    a_t, a_r, a_rej = gpu_step(ctx, 41, tr, dev)
+    ctx.set_process_sort(False)
+    b_t, b_r, b_rej = gpu_step(ctx, 41, tr, dev)
+    assert a_rej == b_rej == 1
+    for k in OUT_FIELDS:
+        assert same_bits(a_r[k], b_r[k]), k
+    el = tr["ptype"] != 2
+    for k, (dt, c) in TRACK_FIELDS.items():
+        m = np.tile(el, c) if k.startswith("msc") else np.ones(c * len(el), bool)
+        assert same_bits(a_t[k][m], b_t[k][m]), k
+
+
 def test_step_inplace_equals_out_of_place(ctx2, dev):
     from g4hepem_amd.tracks import step_tracks
     tr = step_tracks(50000, 5, 4)

@@ -71,3 +71,17 @@ def test_shower_capacity_error(ctx1):
     from g4hepem_amd._native import slab_setup
     with pytest.raises(HepEmError):
         ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 0, 10000.0, capacity=64, events_per_chunk=4), 4, 1)
+
+
+def test_shower_photon_transmission_closed_form(tables1, ctx1, oracle):
+    """1 MeV photons on one 2 mm Pb slab: the fraction of events without any
+    deposit is the no-interaction probability exp(-d Sigma_tot)."""
+    import math
+    from g4hepem_amd._native import slab_setup
+    d, e0, nev = 2.0, 1.0, 50000
+    g = ctx1.slab_run(slab_setup(1, (d,), (0,), 2, e0, capacity=1 << 20, events_per_chunk=50000), nev, 9, per_event=True)
+    zero = (g["edep_event"].sum(1) == 0).mean()
+    xs, _, _ = oracle.probe_gamma(tables1, np.array([0], np.int32), np.array([e0]), np.array([math.log(e0)]),
+                                  np.array([0.5]))
+    p = math.exp(-d * xs[0])
+    assert abs(zero - p) < 4 * math.sqrt(p * (1 - p) / nev), (zero, p)
