@@ -24,6 +24,7 @@ constexpr double kMigdalConst = 5.2804955733859579e-30;
 constexpr double kLPMconstant = 7.6843819381368661e+05;
 constexpr double kALargeValue = 1.0E+20;
 constexpr int kPoissonMaxTerms = 1000;
+constexpr int kSBGuide = 32;           // buckets of the SB search guide
 
 template <typename T> HD T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> HD T tmin(T a, T b) { return a < b ? a : b; }
@@ -302,6 +303,12 @@ struct DevTables {
   const R* sbLKappa;             // [54]
   double sbLogMinE, sbILDE;
   int sbNumKappa;
+  // search guide of the SB sampling tables (derived at context creation):
+  // row r = sbGuideStartZ[Z] + (energy index - first energy index of Z) holds
+  // kSBGuide entries; entry g = largest kappa index k with cumulative value
+  // cum[k] <= g / kSBGuide, in the form's precision
+  const uint8_t* sbGuide;
+  const int* sbGuideStartZ;      // [121], -1 for elements without a table
   // gamma macroscopic cross sections (3 windows) and the conversion selector
   const R* gmData;
   const double* gmData64;

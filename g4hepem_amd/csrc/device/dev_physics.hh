@@ -610,8 +610,11 @@ HD int BremSelectTargetAtom(const DevEl<R>& e, int imc, double ekin, R lekin, R 
   while (ie < numElem - 1 && urndn > xd[i0 + ie] + b * (xd[i1 + ie] - xd[i0 + ie])) ++ie;
   return ie;
 }
-template <typename R> HD int BremLinSearch(const R* v, int size, R val) {  // :328-344
-  int i = 0;
+// First i (a multiple of 3, i < 3*size) with v[i] > val, or 3*size: the
+// reference's forward scan (:328-344) started at i0, which must satisfy
+// v[j] <= val for every j < i0 so that the result is the scan's from 0.
+template <typename R> HD int BremLinSearch(const R* v, int size, R val, int i0 = 0) {
+  int i = i0;
   const int size3 = 3 * size;
   while (i + 9 < size3) {
     if (v[i + 0] > val) return i + 0;
@@ -656,6 +659,7 @@ HD R SampleETransferSB(const DevTables<R>& T, double primEkinD, R lEkin, int imc
   const int iSTStart = iStart + 4 + (eIdx - minEIndx) * sizeOneE;
   const R minV = T.sbData[iSTStart + iGamCut];
   const R* stData = T.sbData + iSTStart + numGamCuts;
+  const uint8_t* guide = T.sbGuide + (size_t)(T.sbGuideStartZ[iZet] + (eIdx - minEIndx)) * kSBGuide;
   const R lKTrans = (logGamCut - lEkin) / (logGamCut - (R)T.sbLElE[eIdx]);
   const R primETot = primEkin + (R)kElectronMassC2;
   const R dielSupConst = (R)T.matDensCor[imat] * primETot * primETot;
@@ -665,7 +669,10 @@ HD R SampleETransferSB(const DevTables<R>& T, double primEkinD, R lEkin, int imc
     R kappa;
     if (!isSimply) {
       const R cumRV = r[0] * ((R)1.0 - minV) + minV;
-      const int cumLIndx3 = BremLinSearch(stData, T.sbNumKappa, cumRV) - 3;
+      // start one guide bucket below the sample: every cumulative value
+      // before the start is <= cumRV, so the scan finds the same index
+      const int g = tmax(0, tmin(kSBGuide - 1, (int)(cumRV * (R)kSBGuide) - 1));
+      const int cumLIndx3 = BremLinSearch(stData, T.sbNumKappa, cumRV, 3 * (int)guide[g]) - 3;
       const int cumLIndx = cumLIndx3 / 3;
       const R cumL = stData[cumLIndx3];
       const R pA = stData[cumLIndx3 + 1];
@@ -1073,9 +1080,10 @@ template <typename R> HD bool SampleLossFluctuations(const DevTables<R>& T, ElSt
     const double meanExE = T.matMeanExc[T.mcMat[imc]];
     const double tmx = t.isEl ? 0.5 * preEkin : preEkin;
     const double tcut = tmin(elCut, tmx);
-    // fluctuation sampling is an FP64 island in both forms: it subtracts large
-    // nearly equal numbers of collisions (a3 - namean)
-    eloss = SampleELossFluctuation(tcut, meanExE, eloss, rng);
+    // in the FP32 form the collision numbers are float: the remainder
+    // a3 - namean then carries an absolute error of ~6e-8 a3 (< 1e-2 of a
+    // Poisson mean of ~8 up to a3 ~ 1e5); the energies around it stay double
+    eloss = (double)SampleELossFluctuation((R)tcut, (R)meanExE, (R)eloss, rng);
     eloss = tmax(eloss, 0.0);
     finalEkin = preEkin - eloss;
   }

@@ -173,15 +173,16 @@ uint64_t hepem_step_work_bytes(int32_t n);
  * hepem_step_prepare clears the step counters and compacts the track slots
  * by particle type into the queue held in `work` (e-/e+ first); hepem_step_run
  * performs the physics step of every queued slot: HowFar, the continuous step
- * and the choice of the discrete interaction, then (by default) a second sort
- * of the pending interactions by process and the final-state sampling with
- * whole waves per process. `work` is required here. hepem_step == prepare +
+ * and the choice of the discrete interaction, then (see
+ * hepem_ctx_set_process_sort) a second sort of the pending interactions by
+ * process and the final-state sampling with whole waves per process. `work` is required here. hepem_step == prepare +
  * run on the same stream. */
 int  hepem_step_prepare(const hepem_ctx* ctx, const hepem_tracks* in, void* work, void* hip_stream);
 int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
                     hepem_step_out* res, void* work, void* hip_stream);
 /* hepem_step_run in one fused kernel (on = 0) or with the process sort
- * (on = 1, the default). Results are identical; only the speed differs. */
+ * (on = 1). Results are identical; only the speed differs. Default: on for
+ * HEPEM_FP64, off for HEPEM_FP32T (the faster of the two on MI355X). */
 int  hepem_ctx_set_process_sort(hepem_ctx* ctx, int on);
 
 /* Number of tracks the last step on `work` rejected (invalid couple index or

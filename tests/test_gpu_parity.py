@@ -294,4 +294,9 @@ def test_fp32_step_statistics(tables2, ctx2_32, oracle, dev):
     agree = (res["winner"] == ref_r["winner"]).mean()
     _print(f"  fp32 winner agreement {agree:.5f}")
     assert agree > 0.99
+    # energy deposited by all e-/e+ (continuous loss + fluctuation) agrees with FP64
+    el = tr["ptype"] != 2
+    rel = abs(res["edep"][el].sum() - ref_r["edep"][el].sum()) / ref_r["edep"][el].sum()
+    _print(f"  fp32 total e-/e+ deposit rel. difference {rel:.2e}")
+    assert rel < 1e-3
     check_step_invariants(tr, t, res, tol=2e-5, dir_tol=2e-6)
