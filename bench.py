@@ -39,23 +39,21 @@ SET2 = ["G4_PbWO4", "G4_Pb", "G4_lAr", "G4_Si"]
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(ptype: np.ndarray, out_of_place: bool = True) -> int:
-    """HBM bytes the step kernel must move per launch: the per-track state it
-    reads and writes, the queue entry, the step results and both secondary
-    slots (SoA field sizes of include/g4hepem_amd.h)."""
-    from g4hepem_amd._native import OUT_FIELDS
-    sz = lambda dt, c: np.dtype(dt).itemsize * c  # noqa: E731
-    common_r = sz("int8", 1) + sz("int32", 1) * 2 + sz("float64", 1) + sz("float64", 3) + sz("float64", 1) + 1 \
-        + sz("float64", 4) + 4 + 8 + 1  # ptype id imc ekin dir safety onb nial rng(ctr gauss flag)
+def algorithmic_bytes(ptype: np.ndarray, nsec_total: int, out_of_place: bool = True) -> int:
+    """HBM bytes a step must move: the per-track state it reads and writes
+    (rng_stream included), the queue entry, the per-track results and the
+    secondaries actually created (33 B each: type, energy, direction); SoA
+    field sizes of include/g4hepem_amd.h."""
+    common_r = 1 + 4 + 4 + 8 + 24 + 8 + 1 + 32 + 4 + 8 + 1 + 4  # ptype id imc ekin dir safety onb nial rng(4)
     msc = 3 * 8 + 1
     queue = 4
     state_w = 8 + 24 + 32 + 4 + 8 + 1  # ekin dir nial rng
-    ident_w = (1 + 4 + 4 + 8 + 1) if out_of_place else 0
-    res_w = sum(sz(dt, c) for dt, c in OUT_FIELDS.values())
+    ident_w = (1 + 4 + 4 + 8 + 1 + 4) if out_of_place else 0
+    res_w = 8 + 8 + 8 + 24 + 4 + 1 + 4  # true/geom step, edep, displacement, winner, stopped, nsec
     e_bytes = common_r + msc + queue + state_w + msc + ident_w + res_w
     g_bytes = common_r + queue + state_w + ident_w + res_w
     ne = int(np.count_nonzero(ptype != 2))
-    return ne * e_bytes + (len(ptype) - ne) * g_bytes
+    return ne * e_bytes + (len(ptype) - ne) * g_bytes + 33 * int(nsec_total)
 
 
 def measured_traffic(kernel_profile: str):
@@ -75,8 +73,8 @@ def measured_traffic(kernel_profile: str):
 
 def time_steps(ctx, tin, tout, res, work, steps, warmup, seed):
     """Times `steps` back-to-back steps (prepare + run) with HIP events on the
-    launch stream; the run (physics kernel) of each step is bracketed by its
-    own event pair. Argument structs are built once so the host enqueues ahead
+    launch stream; the run (the physics kernels: selection, process sort and
+    final states) of each step is bracketed by its own event pair. Argument structs are built once so the host enqueues ahead
     of the GPU."""
     import ctypes as C
 
@@ -292,9 +290,10 @@ def main():
     ctx64 = Context(tables2, HEPEM_FP64)
     ms64, run64, wall64 = time_steps(ctx64, tin, tout, res, work, args.steps, args.warmup, args.seed)
     value = n / (ms64 * 1e-3)
-    abytes = algorithmic_bytes(tracks["ptype"])
+    nsec_total = int(res["nsec"].sum().item())
+    abytes = algorithmic_bytes(tracks["ptype"], nsec_total)
     achieved = abytes / (run64 * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic("k_step_fp64_1M_pmc.md")
+    traffic, traffic_src = measured_traffic("step_run_fp64_1M_pmc.md")
     line = {
         "metric": "track-steps/s on 1 MI355X, FP64 vs FP32-table form, + HBM GB/s vs peak",
         "value": value,
@@ -314,7 +313,8 @@ def main():
             "tracks": n, "wall_s": round(wall64, 4),
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_step<double>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "bound": "hbm", "kernel": "hepem_step_run = k_step_select<double> + k_classify_proc + k_interact<double>",
+            "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel_ms": run64, "algorithmic_bytes_per_launch": abytes,
         },

@@ -92,24 +92,19 @@ __global__ void __launch_bounds__(kBlock) k_classify(int n, const int8_t* __rest
   ClassifyTile(n, ptype, queue, counters, blockIdx.x);
 }
 
-template <typename R> HD void WriteSlot(const hepem_step_out& res, int n, int i, int s, const Sec<R>* sec) {
-  if (sec) {
-    res.sec_ptype[s * n + i] = (int8_t)sec->ptype;
-    res.sec_ekin[s * n + i] = sec->ekin;
+// Secondary slots beyond nsec[i] are left unwritten (the reference's
+// G4HepEmTLData likewise holds only the secondaries created).
+template <typename R> HD void WriteSlot(const hepem_step_out& res, int n, int i, int s, const Sec<R>& sec) {
+  res.sec_ptype[s * n + i] = (int8_t)sec.ptype;
+  res.sec_ekin[s * n + i] = sec.ekin;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = (double)sec->dir[c];
-  } else {
-    res.sec_ptype[s * n + i] = -1;
-    res.sec_ekin[s * n + i] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = 0.0;
-  }
+  for (int c = 0; c < 3; ++c) res.sec_dir[(c * 2 + s) * n + i] = (double)sec.dir[c];
 }
 template <typename R>
 HD void WriteSecondaries(const hepem_step_out& res, int n, int i, const SecList<R>& sl) {
   res.nsec[i] = sl.ns;
-  WriteSlot<R>(res, n, i, 0, sl.ns > 0 ? &sl.s0 : nullptr);
-  WriteSlot<R>(res, n, i, 1, sl.ns > 1 ? &sl.s1 : nullptr);
+  if (sl.ns > 0) WriteSlot<R>(res, n, i, 0, sl.s0);
+  if (sl.ns > 1) WriteSlot<R>(res, n, i, 1, sl.s1);
 }
 
 // A track is stepped only with a valid couple index and a finite, positive

@@ -95,7 +95,9 @@ typedef struct hepem_tracks {
                                   stream selector, stream = rng_stream:id   */
 } hepem_tracks;
 
-/* Per-step results (write only). Up to 2 secondaries per track and step. */
+/* Per-step results (write only). Up to 2 secondaries per track and step:
+ * slot s < nsec[i] holds one (electrons first, then gammas); slots at or
+ * beyond nsec[i] are not written. */
 typedef struct hepem_step_out {
   double*   true_step;         /* physical (true) step length [mm]               */
   double*   geom_step;         /* geometrical step length [mm]                   */

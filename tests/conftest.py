@@ -59,6 +59,24 @@ def golden_tracks(g, prefix):
     return out
 
 
+def valid_secondaries(res: dict) -> dict:
+    """Results with the secondary slots at or beyond nsec cleared: those slots
+    are not part of the result (the kernels leave them unwritten)."""
+    out = dict(res)
+    nsec = np.asarray(res["nsec"])
+    n = len(nsec)
+    if n == 0:
+        return out
+    for k, fill in (("sec_ptype", -1), ("sec_ekin", 0.0), ("sec_dir", 0.0)):
+        a = np.array(res[k], copy=True)
+        comps = len(a) // (2 * n)
+        v = a.reshape(comps, 2, n)
+        for s in range(2):
+            v[:, s, nsec <= s] = fill
+        out[k] = v.reshape(-1)
+    return out
+
+
 def same_bits(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

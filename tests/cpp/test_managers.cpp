@@ -94,12 +94,26 @@ template <typename T> bool SameBytes(const T* a, const T* b, size_t n) {
   return std::memcmp(x.data(), y.data(), n * sizeof(T)) == 0;
 }
 
+// secondary slots at or beyond nsec are not written, so only valid ones compare
+bool SameSecondaries(hepem_step_out& x, hepem_step_out& y, size_t n) {
+  const std::vector<int32_t> ns = Download(x.nsec, n);
+  const std::vector<int8_t> pa = Download(x.sec_ptype, 2 * n), pb = Download(y.sec_ptype, 2 * n);
+  const std::vector<double> ea = Download(x.sec_ekin, 2 * n), eb = Download(y.sec_ekin, 2 * n);
+  const std::vector<double> da = Download(x.sec_dir, 6 * n), db = Download(y.sec_dir, 6 * n);
+  for (size_t i = 0; i < n; ++i)
+    for (int s = 0; s < ns[i] && s < 2; ++s) {
+      if (pa[s * n + i] != pb[s * n + i] || std::memcmp(&ea[s * n + i], &eb[s * n + i], sizeof(double)) != 0) return false;
+      for (int c = 0; c < 3; ++c)
+        if (std::memcmp(&da[(c * 2 + s) * n + i], &db[(c * 2 + s) * n + i], sizeof(double)) != 0) return false;
+    }
+  return true;
+}
+
 bool SameResults(StepResults& a, StepResults& b, size_t n) {
   hepem_step_out &x = a.View(), &y = b.View();
   return SameBytes(x.true_step, y.true_step, n) && SameBytes(x.geom_step, y.geom_step, n) && SameBytes(x.edep, y.edep, n) &&
          SameBytes(x.displacement, y.displacement, 3 * n) && SameBytes(x.winner, y.winner, n) &&
-         SameBytes(x.stopped, y.stopped, n) && SameBytes(x.nsec, y.nsec, n) && SameBytes(x.sec_ptype, y.sec_ptype, 2 * n) &&
-         SameBytes(x.sec_ekin, y.sec_ekin, 2 * n) && SameBytes(x.sec_dir, y.sec_dir, 6 * n);
+         SameBytes(x.stopped, y.stopped, n) && SameBytes(x.nsec, y.nsec, n) && SameSecondaries(x, y, n);
 }
 bool SameState(TrackBatch& a, TrackBatch& b, size_t n) {
   hepem_tracks &x = a.View(), &y = b.View();

@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import STEP_SEED, golden_tracks, same_bits
+from conftest import STEP_SEED, golden_tracks, same_bits, valid_secondaries
 from oracle.oracle_py import OUT_FIELDS, TRACK_FIELDS
 
 pytestmark = pytest.mark.gpu
@@ -65,7 +65,7 @@ def gpu_step(ctx, seed, tracks: dict, dev, inplace=False):
     work = ctx.work_buffer(n, dev)
     ctx.step(seed, tin, tout, res, work)
     torch.cuda.synchronize()
-    return tout.to_numpy(), res.to_numpy(), ctx.rejected(work)
+    return tout.to_numpy(), valid_secondaries(res.to_numpy()), ctx.rejected(work)
 
 
 def compare_step(tin, t_gpu, r_gpu, t_ref, r_ref, alive=None, label=""):
@@ -73,6 +73,7 @@ def compare_step(tin, t_gpu, r_gpu, t_ref, r_ref, alive=None, label=""):
     n = len(tin["ekin"])
     alive = np.ones(n, bool) if alive is None else alive
     el = (tin["ptype"] != 2) & alive
+    r_gpu, r_ref = valid_secondaries(r_gpu), valid_secondaries(r_ref)
     exact, total = 0, 0
     for k, (dt, c) in OUT_FIELDS.items():
         m = np.tile(alive, c)

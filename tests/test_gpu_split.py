@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import STEP_SEED, golden_tracks, same_bits
+from conftest import STEP_SEED, golden_tracks, same_bits, valid_secondaries
 from oracle.oracle_py import OUT_FIELDS, TRACK_FIELDS
 from test_gpu_parity import compare_step, gpu_step
 
@@ -58,7 +58,7 @@ def gpu_split_step(ctx, seed, tracks: dict, dev, geom_limit=None, kind="mixed"):
         t["on_boundary"].copy_(cut.to(torch.uint8))
     ctx.perform(seed, t, hs, res, work, kind=kind)
     torch.cuda.synchronize()
-    return t.to_numpy(), res.to_numpy(), ctx.rejected(work), proposed.cpu().numpy()
+    return t.to_numpy(), valid_secondaries(res.to_numpy()), ctx.rejected(work), proposed.cpu().numpy()
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32"])

@@ -245,8 +245,9 @@ def check_step_invariants(tr, out, res, tol=1e-9, dir_tol=1e-12):
     me = 0.51099891
     pt = tr["ptype"]
     e_in = tr["ekin"]
-    secE = res["sec_ekin"].reshape(2, n).sum(0)
-    sec_pt = res["sec_ptype"].reshape(2, n)
+    valid = np.stack([res["nsec"] > 0, res["nsec"] > 1])
+    secE = np.where(valid, res["sec_ekin"].reshape(2, n), 0.0).sum(0)
+    sec_pt = np.where(valid, res["sec_ptype"].reshape(2, n), -1)
     # e+ annihilation creates 2 m c^2 of photon energy
     ann = (pt == 1) & (res["nsec"] == 2) & (sec_pt[0] == 2) & (sec_pt[1] == 2)
     # pair production turns 2 m c^2 of photon energy into rest mass
