@@ -382,8 +382,8 @@ bool ResOk(const hepem_step_out* r, int n) {
 }  // namespace
 
 // work layout (each part at 256-byte alignment): 64 counters | particle queue
-// (n x u32) | process code per slot (n x u8) | PE cross section per slot
-// (n x f64) | process queue (kNumProc regions of n x u32)
+// (n x u32) | process code per slot (n x u8) | staged record per slot
+// (n x 64 bytes) | process queue (kNumProc regions of n x u32)
 namespace {
 constexpr uint64_t kCountersBytes = 256;
 uint64_t Al(uint64_t b) { return (b + 255) & ~255ull; }
@@ -391,7 +391,7 @@ struct WorkLayout {
   uint32_t* counters;
   uint32_t* queue;
   uint8_t* code;
-  double* pe;
+  Staged* st;
   uint32_t* pqueue;
   WorkLayout(void* w, int32_t n) {
     uint8_t* p = static_cast<uint8_t*>(w);
@@ -402,8 +402,8 @@ struct WorkLayout {
     p += Al(4 * m);
     code = p;
     p += Al(m);
-    pe = reinterpret_cast<double*>(p);
-    p += Al(8 * m);
+    st = reinterpret_cast<Staged*>(p);
+    p += Al(sizeof(Staged) * m);
     pqueue = reinterpret_cast<uint32_t*>(p);
   }
 };
@@ -411,7 +411,7 @@ struct WorkLayout {
 
 extern "C" uint64_t hepem_step_work_bytes(int32_t n) {
   const uint64_t m = (uint64_t)(n > 0 ? n : 0);
-  return kCountersBytes + Al(4 * m) + Al(m) + Al(8 * m) + Al(4 * m * kNumProc);
+  return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(4 * m * kNumProc);
 }
 
 extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int on) {
@@ -452,15 +452,15 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
     if (fp64) {
       hipLaunchKernelGGL(k_step_select<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue,
-                         w.counters, w.code, w.pe);
+                         w.counters, w.code, w.st);
       hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, w.code, w.pqueue, w.counters);
-      hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *out, *res, w.code, w.pe, w.pqueue,
+      hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *out, *res, w.code, w.st, w.pqueue,
                          w.counters);
     } else {
       hipLaunchKernelGGL(k_step_select<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue,
-                         w.counters, w.code, w.pe);
+                         w.counters, w.code, w.st);
       hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, w.code, w.pqueue, w.counters);
-      hipLaunchKernelGGL(k_interact<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *out, *res, w.code, w.pe, w.pqueue,
+      hipLaunchKernelGGL(k_interact<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *out, *res, w.code, w.st, w.pqueue,
                          w.counters);
     }
   }

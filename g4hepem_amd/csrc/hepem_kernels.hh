@@ -504,13 +504,15 @@ __global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, 
 
 // ---- two-stage step: particle sort, then process sort ----------------------------------------
 // k_step_select does HowFar, the continuous step and the choice of the discrete
-// interaction (one thread per particle-queue entry) and records the chosen
-// process per slot; k_classify_proc orders the pending slots by process into
+// interaction (one thread per particle-queue entry), records the chosen process
+// per slot and, for a pending interaction, the 64-byte record of what the
+// final-state sampling needs (at the slot: one line per track instead of ten
+// scattered fields); k_classify_proc orders the pending slots by process into
 // one region per process; k_interact samples the final states, so each wave
 // runs one sampler (only the waves that straddle two regions diverge).
-// Between the stages the state travels through the output arrays.
 
-// processes of the second stage (0: none pending)
+// processes of the second stage (0: none pending), in the low 4 bits of the
+// per-slot code; bit 4: the track is an e-, bit 5: a Gaussian deviate is cached
 enum ProcCode : uint8_t {
   kPNone = 0,
   kPIoniEl,    // Moller
@@ -524,60 +526,96 @@ enum ProcCode : uint8_t {
 };
 constexpr int kNumProc = 8;          // codes 1..8
 constexpr int kProcCountBase = 16;   // counters[16 + c]: slots pending process c
+constexpr uint8_t kCodeProc = 0x0F, kCodeElectron = 0x10, kCodeGauss = 0x20;
+
+// state handed from the selection to the final-state sampling of one slot
+struct alignas(16) Staged {
+  double ekin, dir[3], gauss, pe;
+  int32_t imc, id;
+  uint32_t stream, ctr;
+};
+static_assert(sizeof(Staged) == 64, "one 64-byte record per pending slot");
+
+HD uint8_t StageTrack(Staged* st, int i, uint8_t proc, bool isEl, int imc, int32_t id, uint32_t stream, double ekin,
+                      const double* dir, const Rng& rng, double pe) {
+  Staged r;
+  r.ekin = ekin;
+  r.dir[0] = dir[0];
+  r.dir[1] = dir[1];
+  r.dir[2] = dir[2];
+  r.gauss = rng.gauss;
+  r.pe = pe;
+  r.imc = imc;
+  r.id = id;
+  r.stream = stream;
+  r.ctr = rng.ctr;
+  st[i] = r;
+  return (uint8_t)(proc | (isEl ? kCodeElectron : 0) | (rng.has_gauss ? kCodeGauss : 0));
+}
 
 // Stage 1 of an e-/e+ slot: as StepElectron up to PerformDiscrete's choice
-// (G4HepEmElectronManager.icc:425-441). Returns the process still to sample;
-// a complete step (kPNone) also gets its secondaries and stopped flag here.
+// (G4HepEmElectronManager.icc:425-441). Returns the slot code; a complete step
+// (kPNone) also gets its post-step kinematics, secondaries and stopped flag
+// here, a pending one its staged record.
 template <typename R>
 HD uint8_t StepElectronSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
-                              const hepem_step_out& res, int n, int i) {
+                              const hepem_step_out& res, Staged* st, int n, int i) {
   ElState<R> el;
   Rng rng;
   InitElectron(in, n, i, seed, el, rng);
   Trk<R>& t = el.t;
   SecList<R> sl;
   sl.ns = 0;
-  uint8_t code = kPNone;
+  uint8_t proc = kPNone;
   ElectronHowFar(T, el, rng);
   const int cont = ElectronPerformContinuous(T, el, rng);
   if (cont == 1) {
     if (!t.isEl) AnnihilateAtRest(sl, rng);
   } else if (cont == 0) {
     const int ip = ElectronSelectDiscrete(T, t, rng);
-    code = ip == 0   ? (t.isEl ? kPIoniEl : kPIoniPos)
+    proc = ip == 0   ? (t.isEl ? kPIoniEl : kPIoniPos)
            : ip == 1 ? (t.ekin < T.bremModelLim ? kPBremSB : kPBremRB)
            : ip == 2 ? kPAnnih
                      : kPNone;  // -1, or e-/e+ nuclear (no final state)
   }
-  StoreElectronState(out, n, i, el, rng);
-  out.ekin[i] = t.ekin;
+  const Msc<R>& m = el.msc;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+  for (int k = 0; k < 4; ++k) out.num_ia_left[k * n + i] = (double)t.nial[k];
+  out.msc_init_range[i] = (double)m.initRange;
+  out.msc_dyn_range_factor[i] = (double)m.dynRangeFactor;
+  out.msc_tlimit_min[i] = (double)m.tlimitMin;
+  out.msc_first_step[i] = m.firstStep ? 1 : 0;
   res.true_step[i] = (double)el.pstep;
   res.geom_step[i] = (double)t.gstep;
   res.edep[i] = t.edep;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = (double)el.msc.displ[c];
+  for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = (double)m.displ[c];
   res.winner[i] = t.winner;
-  if (code == kPNone) {
-    res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
-    WriteSecondaries(res, n, i, sl);
+  if (proc != kPNone) {
+    const double d[3] = {(double)t.dir[0], (double)t.dir[1], (double)t.dir[2]};
+    return StageTrack(st, i, proc, t.isEl, t.imc, in.id[i], StreamHi(in, i), t.ekin, d, rng, 0.0);
   }
-  return code;
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+  StoreRng(out, i, rng);
+  res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+  WriteSecondaries(res, n, i, sl);
+  return kPNone;
 }
 
 // Stage 1 of a gamma slot: HowFar, SelectInteraction and the interaction-
 // length update of Perform (G4HepEmGammaManager.icc:27-62,173-219).
 template <typename R>
 HD uint8_t StepGammaSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
-                           const hepem_step_out& res, int n, int i, double& peOut) {
+                           const hepem_step_out& res, Staged* st, int n, int i) {
   Trk<R> t;
   t.isEl = false;
   LoadTrack(in, n, i, t);
   Rng rng;
   rng.init(seed, (uint32_t)in.id[i], StreamHi(in, i), in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
   R pe = GammaHowFar(T, t, rng);
-  uint8_t code = kPNone;
+  uint8_t proc = kPNone;
   if (t.onBoundary) {
     t.nial[0] -= t.gstep / t.mfp[0];
   } else {
@@ -585,48 +623,54 @@ HD uint8_t StepGammaSelect(const DevTables<R>& T, uint64_t seed, const hepem_tra
     if (t.winner != 3) {
       t.nial[0] -= t.gstep / t.mfp[0];
       ResetNumIALeft(t.nial, t.winner);
-      code = t.winner == 0 ? kPConv : t.winner == 1 ? kPCompton : kPPE;
+      proc = t.winner == 0 ? kPConv : t.winner == 1 ? kPCompton : kPPE;
     }
   }
-  peOut = (double)pe;
-  out.ekin[i] = t.ekin;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
 #pragma unroll
   for (int k = 0; k < 4; ++k) out.num_ia_left[k * n + i] = (double)t.nial[k];
-  StoreRng(out, i, rng);
   res.true_step[i] = (double)t.gstep;
   res.geom_step[i] = (double)t.gstep;
 #pragma unroll
   for (int c = 0; c < 3; ++c) res.displacement[c * n + i] = 0.0;
   res.winner[i] = t.winner;
-  if (code == kPNone) {
-    res.edep[i] = t.edep;
-    res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
-    SecList<R> sl;
-    sl.ns = 0;
-    WriteSecondaries(res, n, i, sl);
+  if (proc != kPNone) {
+    const double d[3] = {(double)t.dir[0], (double)t.dir[1], (double)t.dir[2]};
+    return StageTrack(st, i, proc, false, t.imc, in.id[i], StreamHi(in, i), t.ekin, d, rng, (double)pe);
   }
-  return code;
+  out.ekin[i] = t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out.dir[c * n + i] = (double)t.dir[c];
+  StoreRng(out, i, rng);
+  res.edep[i] = t.edep;
+  res.stopped[i] = t.ekin <= 0.0 ? 1 : 0;
+  SecList<R> sl;
+  sl.ns = 0;
+  WriteSecondaries(res, n, i, sl);
+  return kPNone;
+}
+
+// Stage 2: track state of a pending slot from its staged record.
+template <typename R> HD void Unstage(const Staged& r, uint8_t code, uint64_t seed, Trk<R>& t, Rng& rng) {
+  t.isEl = (code & kCodeElectron) != 0;
+  t.setE(r.ekin);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c] = (R)r.dir[c];
+  t.imc = r.imc;
+  t.edep = 0.0;
+  rng.init(seed, (uint32_t)r.id, r.stream, r.ctr, (code & kCodeGauss) != 0, r.gauss);
 }
 
 // Stage 2 of an e-/e+ slot: the final state of the chosen interaction
-// (PerformDiscrete, G4HepEmElectronManager.icc:443-458) from the post-step
-// state stage 1 left in `out`.
+// (PerformDiscrete, G4HepEmElectronManager.icc:443-458).
 template <typename R>
 HD void ElectronInteract(const DevTables<R>& T, uint64_t seed, hepem_tracks& out, const hepem_step_out& res, int n, int i,
-                         uint8_t code) {
+                         const Staged& r, uint8_t code) {
   Trk<R> t;
-  t.isEl = out.ptype[i] == 0;
-  t.setE(out.ekin[i]);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) t.dir[c] = (R)out.dir[c * n + i];
-  t.imc = out.imc[i];
   Rng rng;
-  rng.init(seed, (uint32_t)out.id[i], StreamHi(out, i), out.rng_ctr[i], out.rng_has_gauss[i] != 0, out.rng_gauss[i]);
+  Unstage(r, code, seed, t, rng);
   SecList<R> sl;
   sl.ns = 0;
-  switch (code) {
+  switch (code & kCodeProc) {
     case kPIoniEl:
     case kPIoniPos: IoniPerform(T, t, sl, rng); break;
     case kPBremSB: BremPerform(T, t, sl, rng, true); break;
@@ -645,22 +689,16 @@ HD void ElectronInteract(const DevTables<R>& T, uint64_t seed, hepem_tracks& out
 // G4HepEmGammaManager::Perform (G4HepEmGammaManager.icc:63-93).
 template <typename R>
 HD void GammaInteract(const DevTables<R>& T, uint64_t seed, hepem_tracks& out, const hepem_step_out& res, int n, int i,
-                      uint8_t code, double peD) {
+                      const Staged& r, uint8_t code) {
   Trk<R> t;
-  t.isEl = false;
-  t.setE(out.ekin[i]);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) t.dir[c] = (R)out.dir[c * n + i];
-  t.imc = out.imc[i];
-  t.edep = 0.0;
   Rng rng;
-  rng.init(seed, (uint32_t)out.id[i], StreamHi(out, i), out.rng_ctr[i], out.rng_has_gauss[i] != 0, out.rng_gauss[i]);
+  Unstage(r, code, seed, t, rng);
   SecList<R> sl;
   sl.ns = 0;
-  switch (code) {
+  switch (code & kCodeProc) {
     case kPConv: ConvPerform(T, t, sl, rng); break;
     case kPCompton: ComptonPerform(t, sl, rng); break;
-    default: PEPerform(T, t, (R)peD, sl, rng); break;
+    default: PEPerform(T, t, (R)r.pe, sl, rng); break;
   }
   const double finalEkin = t.ekin;
   if (finalEkin > 0.0 && finalEkin <= T.gamTrackCut) {
@@ -680,7 +718,7 @@ template <typename R>
 __global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
                                                         hepem_step_out res, const uint32_t* __restrict__ queue,
                                                         uint32_t* __restrict__ counters, uint8_t* __restrict__ code,
-                                                        double* __restrict__ pe) {
+                                                        Staged* __restrict__ st) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = in.n;
   if (j >= n) return;
@@ -691,18 +729,12 @@ __global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t
     return;
   }
   CopyIdentity(in, out, i);
-  if ((uint32_t)j < nE) {
-    code[i] = StepElectronSelect(T, seed, in, out, res, n, i);
-  } else {
-    double p = 0.0;
-    const uint8_t c = StepGammaSelect(T, seed, in, out, res, n, i, p);
-    code[i] = c;
-    if (c == kPPE) pe[i] = p;
-  }
+  code[i] = (uint32_t)j < nE ? StepElectronSelect(T, seed, in, out, res, st, n, i)
+                             : StepGammaSelect(T, seed, in, out, res, st, n, i);
 }
 
 // Orders the pending slots by process: region c-1 of pqueue (n entries each)
-// receives the slots with code c; counters[kProcCountBase + c] counts them.
+// receives the slots with process c; counters[kProcCountBase + c] counts them.
 // Tiles of kClassifyTile slots: wave ballots per process, an LDS scan, and one
 // atomic per tile and process.
 __global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* __restrict__ code,
@@ -717,7 +749,7 @@ __global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* 
   for (int c = 0; c < kNumProc; ++c) cnt[c] = 0u;
   for (int it = 0; it < kClassifyItems; ++it) {
     const int i = tile0 + it * kBlock + threadIdx.x;
-    const int c = i < n ? (int)code[i] : 0;
+    const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
 #pragma unroll
     for (int p = 0; p < kNumProc; ++p) cnt[p] += (uint32_t)__popcll(__ballot(c == p + 1));
   }
@@ -742,7 +774,7 @@ __global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* 
   for (int p = 0; p < kNumProc; ++p) pos[p] = sBase[p] + sW[p][wave];
   for (int it = 0; it < kClassifyItems; ++it) {
     const int i = tile0 + it * kBlock + threadIdx.x;
-    const int c = i < n ? (int)code[i] : 0;
+    const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
 #pragma unroll
     for (int p = 0; p < kNumProc; ++p) {
       const uint64_t m = __ballot(c == p + 1);
@@ -754,7 +786,7 @@ __global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* 
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_interact(DevTables<R> T, uint64_t seed, hepem_tracks out, hepem_step_out res,
-                                                     const uint8_t* __restrict__ code, const double* __restrict__ pe,
+                                                     const uint8_t* __restrict__ code, const Staged* __restrict__ st,
                                                      const uint32_t* __restrict__ pqueue,
                                                      const uint32_t* __restrict__ counters) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -770,9 +802,10 @@ __global__ void __launch_bounds__(kBlock) k_interact(DevTables<R> T, uint64_t se
   }
   if (p < 0) return;
   const int i = (int)pqueue[(size_t)p * n + (j - first)];
-  const uint8_t c = (uint8_t)(p + 1);
-  if (c < kPConv) ElectronInteract(T, seed, out, res, n, i, c);
-  else GammaInteract(T, seed, out, res, n, i, c, pe[i]);
+  const uint8_t c = code[i];
+  const Staged r = st[i];
+  if (p + 1 < kPConv) ElectronInteract<R>(T, seed, out, res, n, i, r, c);
+  else GammaInteract<R>(T, seed, out, res, n, i, r, c);
 }
 
 // ---- batched stepping ---------------------------------------------------------------------------
