@@ -1303,12 +1303,6 @@ template <typename R> HD void ConvScreen12(R delta, R& f1, R& f2) {  // Conversi
     f2 = (R)41.326 - delta * ((R)5.848 - (R)0.902 * delta);
   }
 }
-template <typename R> HD R ConvScreen1(R delta) {
-  return (delta > (R)1.4) ? (R)42.038 - (R)8.29 * vlog(delta + (R)0.958) : (R)42.184 - delta * ((R)7.444 - (R)1.623 * delta);
-}
-template <typename R> HD R ConvScreen2(R delta) {
-  return (delta > (R)1.4) ? (R)42.038 - (R)8.29 * vlog(delta + (R)0.958) : (R)41.326 - delta * ((R)5.848 - (R)0.902 * delta);
-}
 template <typename R> HD void ConvPhi12(R delta, R& phi1, R& phi2) {  // Conversion.icc:240-248
   if (delta > (R)1.4) {
     phi1 = (R)21.0190 - (R)4.145 * vlog(delta + (R)0.958);
@@ -1355,17 +1349,19 @@ template <typename R> HD void ConvPerform(const DevTables<R>& T, Trk<R>& t, SecL
     const R invF10 = (R)1. / F10, invF20 = (R)1. / F20;
     R r[3], greject;
     if (primE < (R)100000.0) {  // SampleEnergyRateNoLPM :179-200
+      // the two branches share delta and, above delta = 1.4, the screening
+      // function: written once, so a wave with lanes in both branches
+      // evaluates them once (same operations per lane as the branches)
       do {
         rng.flatArray(3, r);
-        if (NormCond > r[0]) {
-          eps = (R)0.5 - epsRange * cbrt13(r[1]);
-          const R delta = deltaFactor / (eps * ((R)1. - eps));
-          greject = (ConvScreen1(delta) - FZ) * invF10;
-        } else {
-          eps = epsMin + epsRange * r[1];
-          const R delta = deltaFactor / (eps * ((R)1. - eps));
-          greject = (ConvScreen2(delta) - FZ) * invF20;
-        }
+        const bool first = NormCond > r[0];
+        if (first) eps = (R)0.5 - epsRange * cbrt13(r[1]);
+        else eps = epsMin + epsRange * r[1];
+        const R delta = deltaFactor / (eps * ((R)1. - eps));
+        const R screen = delta > (R)1.4 ? (R)42.038 - (R)8.29 * vlog(delta + (R)0.958)
+                         : first        ? (R)42.184 - delta * ((R)7.444 - (R)1.623 * delta)
+                                        : (R)41.326 - delta * ((R)5.848 - (R)0.902 * delta);
+        greject = (screen - FZ) * (first ? invF10 : invF20);
       } while (greject < r[2]);
     } else {  // SampleEnergyRateWithLPM :203-237
       const R z23 = (R)T.elZ23[z], ilVarS1 = (R)T.elILS1[z], ilVarS1Cond = (R)T.elILS1Cond[z];
