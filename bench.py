@@ -214,11 +214,12 @@ def time_lookups(ctx, n, reps, seed=2026):
 
 
 def cpu_baseline(tables, tracks: dict, min_seconds: float, seed: int):
-    """The oracle (C restatement of the reference path) on the host cores:
-    repeated passes over the first 250k tracks of the same batch until
-    min_seconds of work are done."""
-    from oracle.oracle_py import Oracle
-    o = Oracle()
+    """The path on the host cores: the reference's own run-time code compiled
+    from its sources (oracle/_ref, kind "reference") when that library is
+    present, else the oracle's C restatement (kind "port"); both are timed
+    when both exist. Repeated passes over the first 250k tracks of the same
+    batch until min_seconds of work are done."""
+    from oracle.oracle_py import Oracle, Ref, ref_available
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(int(cores), 64))
     n_full = len(tracks["ekin"])
@@ -227,17 +228,31 @@ def cpu_baseline(tables, tracks: dict, min_seconds: float, seed: int):
     for k, v in tracks.items():
         c = len(v) // n_full
         sample[k] = np.concatenate([v[i * n_full: i * n_full + n_sample] for i in range(c)])
-    t0 = time.perf_counter()
-    done = 0
-    while True:
-        o.step(tables, seed, sample, nthreads=cores)
-        done += n_sample
-        if time.perf_counter() - t0 >= min_seconds:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "track-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{done // n_sample} passes over the first {n_sample} tracks of the same batch "
-                      f"({done} track-steps, {dt:.1f} s, {cores} threads)"}
+
+    def timed(engine):
+        t0 = time.perf_counter()
+        done = 0
+        while True:
+            engine.step(tables, seed, sample, nthreads=cores)
+            done += n_sample
+            if time.perf_counter() - t0 >= min_seconds:
+                break
+        return done, time.perf_counter() - t0
+
+    runs = [("port", Oracle())]
+    if ref_available():
+        runs.insert(0, ("reference", Ref()))
+    out = None
+    for kind, engine in runs:
+        done, dt = timed(engine)
+        rec = {"value": done / dt, "unit": "track-steps/s", "cores": cores, "kind": kind,
+               "sample": f"{done // n_sample} passes over the first {n_sample} tracks of the same batch "
+                         f"({done} track-steps, {dt:.1f} s, {cores} threads)"}
+        if out is None:
+            out = rec
+        else:
+            out["also_" + kind] = {"value": rec["value"], "sample": rec["sample"]}
+    return out
 
 
 def log(msg: str) -> None:
