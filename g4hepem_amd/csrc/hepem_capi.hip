@@ -585,6 +585,12 @@ struct hepem_batch_plan {
   BatchDev* dDescs = nullptr;
   uint32_t* dQueue = nullptr;
   uint32_t* dCounters = nullptr;  // 4 per batch
+  // process sort of the pending interactions over all batches (as hepem_step)
+  bool processSort = true;
+  uint8_t* dCode = nullptr;       // process code per global slot
+  Staged* dStaged = nullptr;      // staged record per global slot
+  uint32_t* dPQueue = nullptr;    // per batch b: kNumProc regions of n_b entries from kNumProc*qoff_b
+  uint32_t* dPCounters = nullptr; // 32 words per batch: [kProcCountBase + c] pending slots of process c
   int shape = kNumShapes - 1;     // block size ShapeBlock(shape)
   int grid = 0;                   // step-kernel blocks (nchunks[shape])
 };
@@ -592,8 +598,17 @@ struct hepem_batch_plan {
 namespace {
 template <typename R, int S>
 void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
-  hipLaunchKernelGGL((k_step_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
-                     p->nchunks[S], p->dQueue, p->dCounters);
+  if (!p->processSort) {
+    hipLaunchKernelGGL((k_step_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+                       p->nchunks[S], p->dQueue, p->dCounters);
+    return;
+  }
+  hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+                     p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
+  hipLaunchKernelGGL(k_classify_proc_batched, dim3(p->ntiles), dim3(kBlock), 0, s, p->dDescs, p->nbatch, p->dCode, p->dPQueue,
+                     p->dPCounters);
+  hipLaunchKernelGGL(k_interact_batched<R>, dim3(p->nchunks[2]), dim3(kBlock), 0, s, T, seed, p->dDescs, p->nbatch,
+                     p->nchunks[2], p->dCode, p->dStaged, p->dPQueue, p->dPCounters);
 }
 template <typename R> void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
   switch (p->shape) {
@@ -607,6 +622,10 @@ void FreePlan(hepem_batch_plan* p) {
   if (p->dDescs) (void)hipFree(p->dDescs);
   if (p->dQueue) (void)hipFree(p->dQueue);
   if (p->dCounters) (void)hipFree(p->dCounters);
+  if (p->dCode) (void)hipFree(p->dCode);
+  if (p->dStaged) (void)hipFree(p->dStaged);
+  if (p->dPQueue) (void)hipFree(p->dPQueue);
+  if (p->dPCounters) (void)hipFree(p->dPCounters);
   delete p;
 }
 }  // namespace
@@ -643,6 +662,7 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
   hepem_batch_plan* p = new (std::nothrow) hepem_batch_plan();
   if (!p) return HEPEM_ERR_ALLOC;
   p->ctx = c;
+  p->processSort = c->processSort != 0;
   p->nbatch = nbatch;
   p->total = q;
   p->ntiles = (uint32_t)tiles;
@@ -650,7 +670,10 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
   p->grid = (int)(p->nchunks[p->shape] > 0 ? p->nchunks[p->shape] : 1);
   if (hipMalloc(&p->dDescs, h.size() * sizeof(BatchDev)) != hipSuccess ||
       hipMalloc(&p->dQueue, (q > 0 ? q : 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&p->dCounters, (size_t)4 * nbatch * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&p->dCounters, (size_t)4 * nbatch * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&p->dCode, (q > 0 ? q : 1)) != hipSuccess || hipMalloc(&p->dStaged, (q > 0 ? q : 1) * sizeof(Staged)) != hipSuccess ||
+      hipMalloc(&p->dPQueue, (size_t)kNumProc * (q > 0 ? q : 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&p->dPCounters, (size_t)32 * nbatch * sizeof(uint32_t)) != hipSuccess) {
     FreePlan(p);
     return HEPEM_ERR_ALLOC;
   }
@@ -673,6 +696,9 @@ extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, vo
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const int nc = 4 * p->nbatch;
   hipLaunchKernelGGL(k_fill_u32, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dCounters, nc, 0u);
+  if (p->processSort)
+    hipLaunchKernelGGL(k_fill_u32, dim3((32 * p->nbatch + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dPCounters,
+                       32 * p->nbatch, 0u);
   if (p->total > 0) {
     hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kBlock), 0, s, p->dDescs, p->nbatch, p->dQueue, p->dCounters);
     if (p->ctx->precision == HEPEM_FP64) LaunchStepBatched(p->ctx->t64, p, seed, s);

@@ -737,13 +737,13 @@ __global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t
 // receives the slots with process c; counters[kProcCountBase + c] counts them.
 // Tiles of kClassifyTile slots: wave ballots per process, an LDS scan, and one
 // atomic per tile and process.
-__global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* __restrict__ code,
-                                                          uint32_t* __restrict__ pqueue, uint32_t* __restrict__ counters) {
+__device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32_t* __restrict__ pqueue,
+                                 uint32_t* __restrict__ counters, int tile) {
   __shared__ uint32_t sW[kNumProc][kWaves], sBase[kNumProc];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int tile0 = blockIdx.x * kClassifyTile;
+  const int tile0 = tile * kClassifyTile;
   uint32_t cnt[kNumProc];
 #pragma unroll
   for (int c = 0; c < kNumProc; ++c) cnt[c] = 0u;
@@ -782,6 +782,11 @@ __global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* 
       pos[p] += (uint32_t)__popcll(m);
     }
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* __restrict__ code,
+                                                          uint32_t* __restrict__ pqueue, uint32_t* __restrict__ counters) {
+  ClassifyProcTile(n, code, pqueue, counters, blockIdx.x);
 }
 
 template <typename R>
@@ -871,6 +876,85 @@ __global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched(DevTables<R> T, 
     if ((uint32_t)j < cnt[0]) StepElectron(T, seed, in, out, res, n, i);
     else StepGamma(T, seed, in, out, res, n, i);
   }
+}
+
+// Two-stage form of k_step_batched: stage 1 over every batch (block c takes
+// chunk c, as k_step_batched), recording the process code and staged record of
+// slot i of batch b at global index qoff_b + i; k_classify_proc_batched sorts
+// each batch's pending slots by process (region p of batch b at
+// pqueue[kNumProc*qoff_b + p*n_b], counts at pcounters[32*b + kProcCountBase + p + 1]);
+// k_interact_batched samples the final states, block c taking the entries
+// of chunk c of the 256-entry chunks of every batch (so the batch, and its
+// descriptor, is uniform over the block).
+template <typename R, int S>
+__global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables<R> T, uint64_t seed,
+                                                                       const BatchDev* __restrict__ d, int nbatch,
+                                                                       uint32_t nchunks, const uint32_t* __restrict__ queue,
+                                                                       uint32_t* __restrict__ bcounters,
+                                                                       uint8_t* __restrict__ code, Staged* __restrict__ st) {
+  constexpr int BS = ShapeBlock(S);
+  const uint32_t c = blockIdx.x;
+  if (c >= nchunks) return;
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, c, [](const BatchDev& x) { return x.chunk0[S]; }));
+  const hepem_tracks in = d[b].in;
+  hepem_tracks out = d[b].out;
+  const hepem_step_out res = d[b].res;
+  const int n = in.n;
+  const int j = (int)(c - d[b].chunk0[S]) * BS + (int)threadIdx.x;
+  if (j >= n) return;
+  uint32_t* cnt = bcounters + 4 * b;
+  const uint32_t qoff = d[b].qoff;
+  const int i = (int)queue[qoff + j];
+  uint8_t* cb = code + qoff;
+  if (RejectTrack(T, in, out, res, n, i, cnt)) {
+    cb[i] = kPNone;
+    return;
+  }
+  CopyIdentity(in, out, i);
+  cb[i] = (uint32_t)j < cnt[0] ? StepElectronSelect(T, seed, in, out, res, st + qoff, n, i)
+                               : StepGammaSelect(T, seed, in, out, res, st + qoff, n, i);
+}
+
+__global__ void __launch_bounds__(kBlock) k_classify_proc_batched(const BatchDev* __restrict__ d, int nbatch,
+                                                                  const uint8_t* __restrict__ code,
+                                                                  uint32_t* __restrict__ pqueue,
+                                                                  uint32_t* __restrict__ pcounters) {
+  const uint32_t t = blockIdx.x;
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
+  const uint32_t qoff = d[b].qoff;
+  ClassifyProcTile(d[b].in.n, code + qoff, pqueue + (size_t)kNumProc * qoff, pcounters + 32 * b, (int)(t - d[b].tile0));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_interact_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
+                                                             int nbatch, uint32_t nchunks, const uint8_t* __restrict__ code,
+                                                             const Staged* __restrict__ st,
+                                                             const uint32_t* __restrict__ pqueue,
+                                                             const uint32_t* __restrict__ pcounters) {
+  constexpr int S = 2;  // 256-entry chunks
+  const uint32_t cidx = blockIdx.x;
+  if (cidx >= nchunks) return;
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, cidx, [](const BatchDev& x) { return x.chunk0[S]; }));
+  hepem_tracks out = d[b].out;
+  const hepem_step_out res = d[b].res;
+  const int n = out.n;
+  const uint32_t qoff = d[b].qoff;
+  const uint32_t* cnt = pcounters + 32 * b;
+  const int j = (int)(cidx - d[b].chunk0[S]) * kBlock + (int)threadIdx.x;
+  uint32_t first = 0;
+  int p = -1;
+#pragma unroll
+  for (int q = 0; q < kNumProc; ++q) {
+    const uint32_t c = cnt[kProcCountBase + q + 1];
+    if (p < 0 && (uint32_t)j < first + c) p = q;
+    if (p < 0) first += c;
+  }
+  if (p < 0) return;
+  const int i = (int)pqueue[(size_t)kNumProc * qoff + (size_t)p * n + (j - first)];
+  const uint8_t c = code[qoff + i];
+  const Staged r = st[qoff + i];
+  if (p + 1 < kPConv) ElectronInteract<R>(T, seed, out, res, n, i, r, c);
+  else GammaInteract<R>(T, seed, out, res, n, i, r, c);
 }
 
 template <typename R>

@@ -233,7 +233,9 @@ int  hepem_gamma_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, c
  * (counter fill, one batched classify, one batched step) -- no allocation, no
  * host copies -- so a run can be captured into a hipGraph and replayed.
  * Results are bit-identical to calling hepem_step on each batch with the same
- * seed. The batches' arrays must stay valid while the plan is used. */
+ * seed. A plan sorts pending interactions by process (as hepem_step_run) when
+ * its context does so at plan creation (hepem_ctx_set_process_sort). The
+ * batches' arrays must stay valid while the plan is used. */
 typedef struct hepem_batch_plan hepem_batch_plan;
 int  hepem_batch_plan_create(const hepem_ctx* ctx, const hepem_batch_desc* descs, int32_t nbatch,
                              hepem_batch_plan** out_plan);
