@@ -32,10 +32,12 @@ class Context:
         N.check("hepem_ctx_table_bytes", N.lib().hepem_ctx_table_bytes(self.h, C.byref(b)))
         return b.value
 
-    def set_process_sort(self, on: bool) -> None:
-        """hepem_step_run with (default) or without the sort of the pending
-        interactions by process; results are identical."""
-        N.check("hepem_ctx_set_process_sort", N.lib().hepem_ctx_set_process_sort(self.h, 1 if on else 0))
+    def set_process_sort(self, on) -> None:
+        """Sort of the pending interactions by process in hepem_step_run and
+        the Perform calls: True (always), False (never) or "auto" (for calls of
+        at least 32768 slots, the FP64 default); results are identical."""
+        mode = 2 if on == "auto" else (1 if on else 0)
+        N.check("hepem_ctx_set_process_sort", N.lib().hepem_ctx_set_process_sort(self.h, mode))
 
     def lookup(self, ptype: int, imc, ekin, lekin, out=None, stream=None):
         """Config-1 lookups on device tensors; returns a [5, n] float64 tensor

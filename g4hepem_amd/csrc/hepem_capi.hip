@@ -294,8 +294,8 @@ struct hepem_ctx {
   DevTables<float> t32;
   void* work = nullptr;       // internal work buffer for callers passing work == NULL
   uint64_t workBytes = 0;
-  int processSort = 1;        // hepem_step: sort pending interactions by process (two stages);
-                              // by default on in the FP64 form, off in the FP32 form (measured faster)
+  int processSort = HEPEM_SORT_AUTO;  // sort pending interactions by process (two stages): HEPEM_SORT_*;
+                                      // by default auto in the FP64 form, off in the FP32 form (measured faster)
 };
 
 extern "C" const char* hepem_status_string(int s) {
@@ -321,7 +321,7 @@ extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p
   if (!c) return HEPEM_ERR_ALLOC;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HEPEM_ERR_HIP; }
   c->precision = precision;
-  c->processSort = precision == HEPEM_FP64 ? 1 : 0;
+  c->processSort = precision == HEPEM_FP64 ? HEPEM_SORT_AUTO : HEPEM_SORT_OFF;
   Packer P;
   PackTables(P, c->t64, d, p);
   PackTables(P, c->t32, d, p);
@@ -414,11 +414,18 @@ extern "C" uint64_t hepem_step_work_bytes(int32_t n) {
   return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(4 * m * kNumProc);
 }
 
-extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int on) {
-  if (!c || (on != 0 && on != 1)) return HEPEM_ERR_INVALID;
-  c->processSort = on;
+extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
+  if (!c || (mode != HEPEM_SORT_OFF && mode != HEPEM_SORT_ON && mode != HEPEM_SORT_AUTO)) return HEPEM_ERR_INVALID;
+  c->processSort = mode;
   return HEPEM_OK;
 }
+
+namespace {
+// whether a step over n slots sorts its pending interactions by process
+bool SortsProcesses(const hepem_ctx* c, int64_t n) {
+  return c->processSort == HEPEM_SORT_ON || (c->processSort == HEPEM_SORT_AUTO && n >= HEPEM_SORT_AUTO_MIN_TRACKS);
+}
+}  // namespace
 
 extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
   if (!c || !TracksOk(in) || !work) return HEPEM_ERR_INVALID;
@@ -443,7 +450,7 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
   const WorkLayout w(work, n);
   const int grid = (n + kBlock - 1) / kBlock;
   const bool fp64 = c->precision == HEPEM_FP64;
-  if (!c->processSort) {
+  if (!SortsProcesses(c, n)) {
     if (fp64)
       hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue, w.counters);
     else
@@ -529,14 +536,31 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
   }
   const int grid = (n + kBlock - 1) / kBlock;
   const bool fp64 = c->precision == HEPEM_FP64;
-  if (!perform && fp64)
+  if (!perform && fp64) {
     hipLaunchKernelGGL((k_howfar<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, queue, counters);
-  else if (!perform)
+  } else if (!perform) {
     hipLaunchKernelGGL((k_howfar<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, queue, counters);
-  else if (fp64)
+  } else if (SortsProcesses(c, n)) {
+    // choice of the interaction, process sort, final states (as hepem_step_run)
+    const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
+    if (fp64)
+      hipLaunchKernelGGL((k_perform_select<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue,
+                         counters, wl.code, wl.st);
+    else
+      hipLaunchKernelGGL((k_perform_select<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue,
+                         counters, wl.code, wl.st);
+    hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, wl.code, wl.pqueue, counters);
+    if (fp64)
+      hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *res, wl.code, wl.st, wl.pqueue,
+                         counters);
+    else
+      hipLaunchKernelGGL(k_interact<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *res, wl.code, wl.st, wl.pqueue,
+                         counters);
+  } else if (fp64) {
     hipLaunchKernelGGL((k_perform<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue, counters);
-  else
+  } else {
     hipLaunchKernelGGL((k_perform<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue, counters);
+  }
   HIP_OK(hipGetLastError());
   return HEPEM_OK;
 }
@@ -662,7 +686,7 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
   hepem_batch_plan* p = new (std::nothrow) hepem_batch_plan();
   if (!p) return HEPEM_ERR_ALLOC;
   p->ctx = c;
-  p->processSort = c->processSort != 0;
+  p->processSort = SortsProcesses(c, q);
   p->nbatch = nbatch;
   p->total = q;
   p->ntiles = (uint32_t)tiles;

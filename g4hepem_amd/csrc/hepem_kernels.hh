@@ -553,21 +553,18 @@ HD uint8_t StageTrack(Staged* st, int i, uint8_t proc, bool isEl, int imc, int32
   return (uint8_t)(proc | (isEl ? kCodeElectron : 0) | (rng.has_gauss ? kCodeGauss : 0));
 }
 
-// Stage 1 of an e-/e+ slot: as StepElectron up to PerformDiscrete's choice
-// (G4HepEmElectronManager.icc:425-441). Returns the slot code; a complete step
-// (kPNone) also gets its post-step kinematics, secondaries and stopped flag
-// here, a pending one its staged record.
+// Stage 1 of an e-/e+ slot after its HowFar: the continuous step and
+// PerformDiscrete's choice (G4HepEmElectronManager.icc:425-441,461-483).
+// Returns the slot code; a complete step (kPNone) also gets its post-step
+// kinematics, secondaries and stopped flag here, a pending one its staged
+// record.
 template <typename R>
-HD uint8_t StepElectronSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
-                              const hepem_step_out& res, Staged* st, int n, int i) {
-  ElState<R> el;
-  Rng rng;
-  InitElectron(in, n, i, seed, el, rng);
+HD uint8_t ElectronSelectAfterHowFar(const DevTables<R>& T, ElState<R>& el, Rng& rng, const hepem_tracks& in,
+                                     hepem_tracks& out, const hepem_step_out& res, Staged* st, int n, int i) {
   Trk<R>& t = el.t;
   SecList<R> sl;
   sl.ns = 0;
   uint8_t proc = kPNone;
-  ElectronHowFar(T, el, rng);
   const int cont = ElectronPerformContinuous(T, el, rng);
   if (cont == 1) {
     if (!t.isEl) AnnihilateAtRest(sl, rng);
@@ -604,17 +601,36 @@ HD uint8_t StepElectronSelect(const DevTables<R>& T, uint64_t seed, const hepem_
   return kPNone;
 }
 
-// Stage 1 of a gamma slot: HowFar, SelectInteraction and the interaction-
-// length update of Perform (G4HepEmGammaManager.icc:27-62,173-219).
+// Stage 1 of an e-/e+ slot of the fused step: as StepElectron up to
+// PerformDiscrete's choice.
 template <typename R>
-HD uint8_t StepGammaSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
-                           const hepem_step_out& res, Staged* st, int n, int i) {
-  Trk<R> t;
-  t.isEl = false;
-  LoadTrack(in, n, i, t);
+HD uint8_t StepElectronSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
+                              const hepem_step_out& res, Staged* st, int n, int i) {
+  ElState<R> el;
   Rng rng;
-  rng.init(seed, (uint32_t)in.id[i], StreamHi(in, i), in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
-  R pe = GammaHowFar(T, t, rng);
+  InitElectron(in, n, i, seed, el, rng);
+  ElectronHowFar(T, el, rng);
+  return ElectronSelectAfterHowFar(T, el, rng, in, out, res, st, n, i);
+}
+
+// Stage 1 of an e-/e+ slot of the split Perform: the HowFar state comes from
+// the hand-over (with the caller's geometrical step).
+template <typename R>
+HD uint8_t PerformElectronSelect(const DevTables<R>& T, uint64_t seed, hepem_tracks& t, const hepem_howfar_state& hs,
+                                 const hepem_step_out& res, Staged* st, int n, int i) {
+  ElState<R> el;
+  Rng rng;
+  InitElectron(t, n, i, seed, el, rng);
+  LoadElectronHowFar(hs, n, i, el);
+  return ElectronSelectAfterHowFar(T, el, rng, t, t, res, st, n, i);
+}
+
+// Stage 1 of a gamma slot after its HowFar: SelectInteraction and the
+// interaction-length update of Perform, or only that update on a boundary
+// (G4HepEmGammaManager.icc:54-62,173-219; G4HepEmTrackingManager.cc:1092-1108).
+template <typename R>
+HD uint8_t GammaSelectAfterHowFar(const DevTables<R>& T, Trk<R>& t, R pe, Rng& rng, const hepem_tracks& in,
+                                  hepem_tracks& out, const hepem_step_out& res, Staged* st, int n, int i) {
   uint8_t proc = kPNone;
   if (t.onBoundary) {
     t.nial[0] -= t.gstep / t.mfp[0];
@@ -647,6 +663,34 @@ HD uint8_t StepGammaSelect(const DevTables<R>& T, uint64_t seed, const hepem_tra
   sl.ns = 0;
   WriteSecondaries(res, n, i, sl);
   return kPNone;
+}
+
+// Stage 1 of a gamma slot of the fused step: HowFar, then as above.
+template <typename R>
+HD uint8_t StepGammaSelect(const DevTables<R>& T, uint64_t seed, const hepem_tracks& in, hepem_tracks& out,
+                           const hepem_step_out& res, Staged* st, int n, int i) {
+  Trk<R> t;
+  t.isEl = false;
+  LoadTrack(in, n, i, t);
+  Rng rng;
+  rng.init(seed, (uint32_t)in.id[i], StreamHi(in, i), in.rng_ctr[i], in.rng_has_gauss[i] != 0, in.rng_gauss[i]);
+  const R pe = GammaHowFar(T, t, rng);
+  return GammaSelectAfterHowFar(T, t, pe, rng, in, out, res, st, n, i);
+}
+
+// Stage 1 of a gamma slot of the split Perform: step, mean free path and PE
+// cross section from the hand-over.
+template <typename R>
+HD uint8_t PerformGammaSelect(const DevTables<R>& T, uint64_t seed, hepem_tracks& t, const hepem_howfar_state& hs,
+                              const hepem_step_out& res, Staged* st, int n, int i) {
+  Trk<R> g;
+  g.isEl = false;
+  LoadTrack(t, n, i, g);
+  g.gstep = (R)hs.geom_step[i];
+  g.mfp[0] = (R)hs.mfp[i];
+  Rng rng;
+  rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
+  return GammaSelectAfterHowFar(T, g, (R)hs.pe_xsec[i], rng, t, t, res, st, n, i);
 }
 
 // Stage 2: track state of a pending slot from its staged record.
@@ -731,6 +775,29 @@ __global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t
   CopyIdentity(in, out, i);
   code[i] = (uint32_t)j < nE ? StepElectronSelect(T, seed, in, out, res, st, n, i)
                              : StepGammaSelect(T, seed, in, out, res, st, n, i);
+}
+
+// Stage 1 of the split Perform (as k_perform up to the choice of the
+// interaction). It also clears the process counters of the classification
+// that follows it on the stream.
+template <typename R, int KIND>
+__global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint64_t seed, hepem_tracks t,
+                                                           hepem_howfar_state hs, hepem_step_out res,
+                                                           const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
+                                                           uint8_t* __restrict__ code, Staged* __restrict__ st) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x < kNumProc) counters[kProcCountBase + 1 + threadIdx.x] = 0u;
+  const int n = t.n;
+  if (j >= n) return;
+  int i;
+  bool isE;
+  const bool kindOk = ResolveSlot<KIND>(t, j, queue, counters, i, isE);
+  if (!kindOk || hs.winner[i] == -99) {
+    WriteRejected<R>(t, t, res, n, i);
+    code[i] = kPNone;
+    return;
+  }
+  code[i] = isE ? PerformElectronSelect<R>(T, seed, t, hs, res, st, n, i) : PerformGammaSelect<R>(T, seed, t, hs, res, st, n, i);
 }
 
 // Orders the pending slots by process: region c-1 of pqueue (n entries each)

@@ -2,16 +2,19 @@
 //
 // The track stack is a pair of structure-of-arrays buffers of `capacity`
 // slots. One stepping round over the n tracks in flight:
-//   k_sh_safety    isotropic safety = distance to the nearer plane
-//   hepem_howfar   (library C-ABI)
+//   hepem_howfar   (library C-ABI); each track's isotropic safety (distance
+//                  to the nearer plane) was set when it was stacked
 //   k_sh_geometry  cut the geometrical step at the next plane along x; the
 //                  post-step boundary flag; end point of the straight move
 //   hepem_perform  (library C-ABI)
-//   k_sh_post      MSC displacement, slab crossing, scoring, leakage, and the
-//                  number of stack slots each track needs next round
-//   scan           exclusive prefix sum of those counts (3 kernels)
+//   k_sh_post      MSC displacement, slab crossing, scoring, leakage, the
+//                  number of stack slots each track needs next round and
+//                  their exclusive scan within the block
+//   k_scan_sums    exclusive scan of the block totals (one block)
 //   k_sh_scatter   survivors, then their secondaries, into the other buffer in
-//                  slot order (deterministic), planar with the new count.
+//                  slot order (deterministic), planar with the new count
+//   k_sh_inject    when the stack runs low, the primaries of the next events
+//                  after them.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -27,7 +30,7 @@ constexpr int kB = 256;
 constexpr double kFix = 1.0e9;           // fixed-point units per MeV
 constexpr double kGeomMinLength = 5.0e-8;
 constexpr double kElectronMass = 0.51099895000;
-constexpr int kMaxRounds = 1 << 20;       // stepping rounds per chunk
+constexpr int kMaxRounds = 1 << 20;       // stepping rounds without new primaries
 
 #define SH_OK(call)                                   \
   do {                                                \
@@ -59,7 +62,7 @@ struct Stack {
   hepem_tracks t;     // t.n = tracks in it (stride of the planar fields)
   double* x;
   int32_t* layer;     // slab index
-  int32_t* event;     // event index within the chunk
+  int32_t* event;     // deposit row of the event (events resident at once)
   int32_t* nstep;     // steps taken by the track
 };
 
@@ -68,23 +71,30 @@ __global__ void k_sh_fill64(int64_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-// primaries of events [e0, e0 + ne): at x = 0, moving along +x, entering slab 0
-__global__ void k_sh_init(Stack s, Slabs g, int ptype, double ekin, int e0, int ne) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ne) return;
+// Primaries of events e0 + j, j in [0, k): at x = 0, moving along +x,
+// entering slab 0; deposit row ev0 + j. They go to slots base[0] + j of a
+// buffer of total[0] tracks (base = total = NULL: slots j of a buffer of k).
+// Writes nothing if the buffer would exceed `cap`.
+__global__ void k_sh_inject(Stack s, Slabs g, int ptype, double ekin, int e0, int ev0, int k,
+                            const uint32_t* __restrict__ base, const uint32_t* __restrict__ total, int cap) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const int m = total ? (int)*total : k;
+  if (m > cap) return;
+  const int i = (base ? (int)*base : 0) + j;
   const hepem_tracks& t = s.t;
-  const uint64_t sid = PrimaryStream(e0 + i);
+  const uint64_t sid = PrimaryStream(e0 + j);
   t.ptype[i] = (int8_t)ptype;
   t.id[i] = (int32_t)(uint32_t)sid;
   t.rng_stream[i] = (uint32_t)(sid >> 32);
   t.imc[i] = g.imc[0];
   t.ekin[i] = ekin;
   t.dir[i] = 1.0;
-  t.dir[ne + i] = 0.0;
-  t.dir[2 * ne + i] = 0.0;
+  t.dir[m + i] = 0.0;
+  t.dir[2 * m + i] = 0.0;
   t.safety[i] = 0.0;
   t.on_boundary[i] = 1;
-  for (int k = 0; k < 4; ++k) t.num_ia_left[k * ne + i] = -1.0;
+  for (int c = 0; c < 4; ++c) t.num_ia_left[c * m + i] = -1.0;
   t.msc_init_range[i] = 1.0e+21;
   t.msc_dyn_range_factor[i] = 0.04;
   t.msc_tlimit_min[i] = 1.0E-7;
@@ -94,16 +104,40 @@ __global__ void k_sh_init(Stack s, Slabs g, int ptype, double ekin, int e0, int 
   t.rng_has_gauss[i] = 0;
   s.x[i] = 0.0;
   s.layer[i] = 0;
-  s.event[i] = i;
+  s.event[i] = ev0 + j;
   s.nstep[i] = 0;
 }
 
-__global__ void k_sh_safety(Stack s, Slabs g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= s.t.n) return;
-  const int l = s.layer[i];
-  const double x = s.x[i];
-  s.t.safety[i] = fmax(0.0, fmin(x - g.plane[l], g.plane[l + 1] - x));
+// isotropic safety of a point in slab l: distance to the nearer plane
+__device__ inline double Safety(const Slabs& g, int l, double x) {
+  return fmax(0.0, fmin(x - g.plane[l], g.plane[l + 1] - x));
+}
+
+// Exclusive scan of v over the block (blockDim.x == kB) and the block sums of
+// v and w. Every thread of the block must call it.
+__device__ inline uint32_t BlockScan(uint32_t v, uint32_t w, uint32_t& vsum, uint32_t& wsum) {
+  constexpr int kW = kB / 64;
+  __shared__ uint32_t shv[kW], shw[kW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = v, y = w;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t a = __shfl_up(x, o, 64);
+    if (lane >= o) x += a;
+  }
+  for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+  if (lane == 63) shv[wv] = x;
+  if (lane == 0) shw[wv] = y;
+  __syncthreads();
+  uint32_t base = 0, tv = 0, tw = 0;
+  for (int k = 0; k < kW; ++k) {
+    const uint32_t sv = shv[k];
+    if (k < wv) base += sv;
+    tv += sv;
+    tw += shw[k];
+  }
+  vsum = tv;
+  wsum = tw;
+  return base + x - v;
 }
 
 // Cuts the proposed geometrical step at the next plane along the pre-step
@@ -139,131 +173,152 @@ __global__ void k_sh_geometry(Stack s, Slabs g, hepem_howfar_state hs, double* x
 
 // After Perform: position, slab crossing, scoring and next-round slot count.
 // keep[i] = 1 if the track itself goes on, nsk[i] = secondaries to stack;
-// cnt[i] = keep[i] + nsk[i].
-__global__ void k_sh_post(Stack s, Slabs g, hepem_step_out res, const double* __restrict__ xend,
+// offs[i] = exclusive scan of keep + nsk within the block, bsum/bsec[block] =
+// the block's slot and secondary totals. Deposits are summed per wave and
+// (event, slab) before the atomic: lanes of a wave mostly hold tracks of one
+// event, and the fixed-point sum does not depend on the grouping.
+__global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out res, const double* __restrict__ xend,
                           const int8_t* __restrict__ side, unsigned long long* dep, unsigned long long* leak,
-                          uint8_t* keep, uint8_t* nsk, uint32_t* cnt, unsigned long long* nsecTotal) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                          uint8_t* keep, uint8_t* nsk, uint32_t* offs, uint32_t* bsum, uint32_t* bsec) {
+  const int i = blockIdx.x * kB + threadIdx.x;
   const int n = s.t.n;
-  if (i >= n) return;
+  const bool valid = i < n;
   const hepem_tracks& t = s.t;
-  const int ev = s.event[i];
-  int l = s.layer[i];
-  const double edep = res.edep[i];
-  if (edep > 0.0) atomicAdd(&dep[(size_t)ev * g.nbins + l], (unsigned long long)llrint(edep * kFix));
-  double x = xend[i];
-  const int sd = side[i];
-  const bool rejected = res.winner[i] == -99;
-  const bool isE = t.ptype[i] != 2;
-  if (sd == 0 && isE && !rejected) {
-    // MSC displacement (G4HepEmTrackingManager.cc:527-566) with the slab safety
-    const double d0 = res.displacement[i], d1 = res.displacement[n + i], d2 = res.displacement[2 * n + i];
-    const double dLength2 = d0 * d0 + d1 * d1 + d2 * d2;
-    if (dLength2 > kGeomMinLength * kGeomMinLength) {
-      const double dispR = sqrt(dLength2);
-      const double postSafety = 0.99 * fmax(0.0, fmin(x - g.plane[l], g.plane[l + 1] - x));
-      if (postSafety > 0.0 && dispR <= postSafety) {
-        x += d0;
-      } else if (dispR < postSafety) {
-        x += d0;
-      } else if (postSafety > kGeomMinLength) {
-        x += d0 * (postSafety / dispR);
+  constexpr unsigned long long kNone = ~0ull;
+  unsigned long long key = kNone, val = 0ull;
+  uint32_t c = 0u, ns = 0u;
+  if (valid) {
+    const int ev = s.event[i];
+    int l = s.layer[i];
+    const double edep = res.edep[i];
+    if (edep > 0.0) {
+      key = (unsigned long long)ev * (unsigned long long)g.nbins + (unsigned long long)l;
+      val = (unsigned long long)llrint(edep * kFix);
+    }
+    double x = xend[i];
+    const int sd = side[i];
+    const bool rejected = res.winner[i] == -99;
+    const bool isE = t.ptype[i] != 2;
+    if (sd == 0 && isE && !rejected) {
+      // MSC displacement (G4HepEmTrackingManager.cc:527-566) with the slab safety
+      const double d0 = res.displacement[i], d1 = res.displacement[n + i], d2 = res.displacement[2 * n + i];
+      const double dLength2 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (dLength2 > kGeomMinLength * kGeomMinLength) {
+        const double dispR = sqrt(dLength2);
+        const double postSafety = 0.99 * Safety(g, l, x);
+        if (postSafety > 0.0 && dispR <= postSafety) {
+          x += d0;
+        } else if (dispR < postSafety) {
+          x += d0;
+        } else if (postSafety > kGeomMinLength) {
+          x += d0 * (postSafety / dispR);
+        }
       }
     }
-  }
-  const double ekin = t.ekin[i];
-  bool alive = !rejected && ekin > 0.0;
-  bool left = false;
-  if (alive && sd != 0) {
-    l += sd;
-    if (l < 0 || l >= g.nbins) {
-      // leaves the stack: its kinetic energy (and an e+'s annihilation energy) is lost
-      const double e = ekin + (t.ptype[i] == 1 ? 2.0 * kElectronMass : 0.0);
-      atomicAdd(&leak[ev], (unsigned long long)llrint(e * kFix));
-      alive = false;
-      left = true;
-      l -= sd;  // keep the index in range for the record
+    const double ekin = t.ekin[i];
+    bool alive = !rejected && ekin > 0.0;
+    bool left = false;
+    if (alive && sd != 0) {
+      l += sd;
+      if (l < 0 || l >= g.nbins) {
+        // leaves the stack: its kinetic energy (and an e+'s annihilation energy) is lost
+        const double e = ekin + (t.ptype[i] == 1 ? 2.0 * kElectronMass : 0.0);
+        atomicAdd(&leak[ev], (unsigned long long)llrint(e * kFix));
+        alive = false;
+        left = true;
+        l -= sd;  // keep the index in range for the record
+      }
     }
+    s.x[i] = x;
+    s.layer[i] = l;
+    // a track that crossed a plane had no interaction, so a track leaving the
+    // stack has no secondaries; the guard keeps that an invariant of the stack
+    ns = (rejected || left) ? 0u : (uint32_t)res.nsec[i];
+    keep[i] = alive ? 1 : 0;
+    nsk[i] = (uint8_t)ns;
+    c = (alive ? 1u : 0u) + ns;
   }
-  s.x[i] = x;
-  s.layer[i] = l;
-  // a track that crossed a plane had no interaction, so a track leaving the
-  // stack has no secondaries; the guard keeps that an invariant of the stack
-  const int ns = (rejected || left) ? 0 : res.nsec[i];
-  keep[i] = alive ? 1 : 0;
-  nsk[i] = (uint8_t)ns;
-  cnt[i] = (alive ? 1u : 0u) + (uint32_t)ns;
-  if (ns > 0) atomicAdd(nsecTotal, (unsigned long long)ns);
+  // per-wave aggregation of the deposits: one atomic per distinct (event, slab)
+  unsigned long long pending = __ballot(key != kNone);
+  const int lane = threadIdx.x & 63;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const unsigned long long k = __shfl(key, leader, 64);
+    const bool mine = key == k;
+    unsigned long long v = mine ? val : 0ull;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == leader) atomicAdd(&dep[k], v);
+    pending &= ~__ballot(mine);
+  }
+  uint32_t tc, tns;
+  const uint32_t off = BlockScan(c, ns, tc, tns);
+  if (valid) offs[i] = off;
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x] = tc;
+    bsec[blockIdx.x] = tns;
+  }
 }
 
-// exclusive scan of cnt[0, n) into offs; bsum[b] = sum of block b
-__global__ void k_scan_block(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ offs, uint32_t* __restrict__ bsum, int n) {
-  __shared__ uint32_t sh[kB];
-  const int i = blockIdx.x * kB + threadIdx.x;
-  const uint32_t v = i < n ? cnt[i] : 0u;
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 1; o < kB; o <<= 1) {
-    const uint32_t a = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
-    __syncthreads();
-    sh[threadIdx.x] += a;
-    __syncthreads();
-  }
-  if (i < n) offs[i] = sh[threadIdx.x] - v;
-  if (threadIdx.x == kB - 1) bsum[blockIdx.x] = sh[kB - 1];
-}
-// in-place exclusive scan of the nb block sums by one block; total -> *total
-__global__ void k_scan_sums(uint32_t* bsum, int nb, uint32_t* total) {
-  __shared__ uint32_t sh[kB];
+// In-place exclusive scan of the nb block sums by one block; the round's
+// secondaries are added to *nsecTotal. total[1] = slots of survivors and
+// secondaries, total[0] = total[1] + `inject` (primaries stacked after them).
+__global__ void __launch_bounds__(kB) k_scan_sums(uint32_t* bsum, const uint32_t* __restrict__ bsec, int nb, int inject,
+                                                  uint32_t* total, unsigned long long* nsecTotal) {
   __shared__ uint32_t carry;
-  if (threadIdx.x == 0) carry = 0u;
+  __shared__ unsigned long long secs;
+  if (threadIdx.x == 0) {
+    carry = 0u;
+    secs = 0ull;
+  }
   __syncthreads();
   for (int base = 0; base < nb; base += kB) {
     const int i = base + threadIdx.x;
     const uint32_t v = i < nb ? bsum[i] : 0u;
-    sh[threadIdx.x] = v;
+    const uint32_t w = i < nb ? bsec[i] : 0u;
+    uint32_t tv, tw;
+    const uint32_t ex = BlockScan(v, w, tv, tw);
+    if (i < nb) bsum[i] = carry + ex;
     __syncthreads();
-    for (int o = 1; o < kB; o <<= 1) {
-      const uint32_t a = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
-      __syncthreads();
-      sh[threadIdx.x] += a;
-      __syncthreads();
+    if (threadIdx.x == 0) {
+      carry += tv;
+      secs += tw;
     }
-    if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == kB - 1) carry += sh[kB - 1];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
-}
-__global__ void k_scan_add(uint32_t* offs, const uint32_t* __restrict__ bsum, int n) {
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i < n) offs[i] += bsum[blockIdx.x];
+  if (threadIdx.x == 0) {
+    total[0] = carry + (uint32_t)inject;
+    total[1] = carry;
+    *nsecTotal += secs;
+  }
 }
 
-// Survivor i goes to slot offs[i], its secondaries to the following slots.
+// Survivor i goes to slot bsum[block] + offs[i], its secondaries to the
+// following slots; each starts with the isotropic safety of its position.
 // Writes nothing beyond `cap` (the host then reports the overflow).
-__global__ void k_sh_scatter(Stack a, Stack b, Slabs g, hepem_step_out res, const uint8_t* __restrict__ keep,
+__global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, hepem_step_out res, const uint8_t* __restrict__ keep,
                              const uint8_t* __restrict__ nsk, const uint32_t* __restrict__ offs,
-                             const uint32_t* __restrict__ total, int cap) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                             const uint32_t* __restrict__ bsum, const uint32_t* __restrict__ total, int cap) {
+  const int i = blockIdx.x * kB + threadIdx.x;
   const int n = a.t.n;
   if (i >= n) return;
   const int m = (int)*total;
   if (m > cap) return;
   const hepem_tracks& s = a.t;
   const hepem_tracks& d = b.t;
-  int j = (int)offs[i];
+  int j = (int)(offs[i] + bsum[blockIdx.x]);
+  const double x = a.x[i];
+  const int l = a.layer[i];
+  const double safety = Safety(g, l, x);
   const uint64_t sid = ((uint64_t)s.rng_stream[i] << 32) | (uint32_t)s.id[i];
   const int ns1 = a.nstep[i] + 1;
   if (keep[i]) {
     d.ptype[j] = s.ptype[i];
     d.id[j] = s.id[i];
     d.rng_stream[j] = s.rng_stream[i];
-    d.imc[j] = g.imc[a.layer[i] % g.nabs];
+    d.imc[j] = g.imc[l % g.nabs];
     d.ekin[j] = s.ekin[i];
     for (int c = 0; c < 3; ++c) d.dir[c * m + j] = s.dir[c * n + i];
-    d.safety[j] = 0.0;
+    d.safety[j] = safety;
     d.on_boundary[j] = s.on_boundary[i];
     for (int k = 0; k < 4; ++k) d.num_ia_left[k * m + j] = s.num_ia_left[k * n + i];
     d.msc_init_range[j] = s.msc_init_range[i];
@@ -273,8 +328,8 @@ __global__ void k_sh_scatter(Stack a, Stack b, Slabs g, hepem_step_out res, cons
     d.rng_ctr[j] = s.rng_ctr[i];
     d.rng_gauss[j] = s.rng_gauss[i];
     d.rng_has_gauss[j] = s.rng_has_gauss[i];
-    b.x[j] = a.x[i];
-    b.layer[j] = a.layer[i];
+    b.x[j] = x;
+    b.layer[j] = l;
     b.event[j] = a.event[i];
     b.nstep[j] = ns1;
     ++j;
@@ -286,10 +341,10 @@ __global__ void k_sh_scatter(Stack a, Stack b, Slabs g, hepem_step_out res, cons
     d.ptype[j] = res.sec_ptype[sl * n + i];
     d.id[j] = (int32_t)(uint32_t)cid;
     d.rng_stream[j] = (uint32_t)(cid >> 32);
-    d.imc[j] = g.imc[a.layer[i] % g.nabs];
+    d.imc[j] = g.imc[l % g.nabs];
     d.ekin[j] = res.sec_ekin[sl * n + i];
     for (int c = 0; c < 3; ++c) d.dir[c * m + j] = res.sec_dir[(c * 2 + sl) * n + i];
-    d.safety[j] = 0.0;
+    d.safety[j] = safety;
     d.on_boundary[j] = 0;
     for (int k = 0; k < 4; ++k) d.num_ia_left[k * m + j] = -1.0;
     d.msc_init_range[j] = 1.0e+21;
@@ -299,8 +354,8 @@ __global__ void k_sh_scatter(Stack a, Stack b, Slabs g, hepem_step_out res, cons
     d.rng_ctr[j] = 0u;
     d.rng_gauss[j] = 0.0;
     d.rng_has_gauss[j] = 0;
-    b.x[j] = a.x[i];
-    b.layer[j] = a.layer[i];
+    b.x[j] = x;
+    b.layer[j] = l;
     b.event[j] = a.event[i];
     b.nstep[j] = 0;
     ++j;
@@ -319,6 +374,15 @@ struct Arena {
     ptrs.push_back(v);
     p = static_cast<T*>(v);
     return true;
+  }
+};
+
+// one pinned host word, freed on every exit path
+struct PinnedU32 {
+  uint32_t* p = nullptr;
+  bool alloc() { return hipHostMalloc(reinterpret_cast<void**>(&p), 2 * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess; }
+  ~PinnedU32() {
+    if (p) (void)hipHostFree(p);
   }
 };
 
@@ -349,7 +413,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
                               hepem_slab_result* r, void* stream) {
   if (!ctx || !su || !r || !r->edep_sum || !r->edep_sum2 || nevents < 0) return HEPEM_ERR_INVALID;
   if (su->nlayers < 1 || su->nabs < 1 || su->nabs > HEPEM_SLAB_MAX_ABS || su->capacity < 1 || su->events_per_chunk < 1 ||
-      !(su->ekin0 > 0.0) || su->primary < 0 || su->primary > 2)
+      !(su->ekin0 > 0.0) || su->primary < 0 || su->primary > 2 || su->refill_below < 0)
     return HEPEM_ERR_INVALID;
   for (int a = 0; a < su->nabs; ++a)
     if (!(su->thickness[a] > 0.0) || su->imc[a] < 0) return HEPEM_ERR_INVALID;
@@ -362,6 +426,15 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   planes[0] = 0.0;
   for (int k = 0; k < nbins; ++k) planes[k + 1] = planes[k] + su->thickness[k % su->nabs];
 
+  // per-event deposit rows resident at once (events beyond are run afterwards)
+  constexpr int64_t kDepBytes = int64_t(1) << 30;
+  const int64_t rowBytes = (int64_t)nbins * (int64_t)sizeof(unsigned long long);
+  int64_t span = kDepBytes / rowBytes;
+  if (span < chunk) span = chunk;
+  if (span > nevents) span = nevents;
+  if (span < 1) span = 1;
+  const int refill = su->refill_below;
+
   Arena A;
   Stack s[2];
   hepem_howfar_state hs;
@@ -369,7 +442,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   double *dPlanes, *xend;
   int8_t* side;
   uint8_t *keep, *nsk, *work;
-  uint32_t *cnt, *offs, *bsum, *total;
+  uint32_t *offs, *bsum, *bsec, *total;
   unsigned long long *dep, *leak, *nsec;
   const int nblk = (cap + kB - 1) / kB;
   if (!AllocStack(A, s[0], cap) || !AllocStack(A, s[1], cap) || !A.get(hs.geom_step, cap) || !A.get(hs.range, cap) ||
@@ -380,10 +453,12 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       !A.get(res.displacement, 3 * (size_t)cap) || !A.get(res.winner, cap) || !A.get(res.stopped, cap) ||
       !A.get(res.nsec, cap) || !A.get(res.sec_ptype, 2 * (size_t)cap) || !A.get(res.sec_ekin, 2 * (size_t)cap) ||
       !A.get(res.sec_dir, 6 * (size_t)cap) || !A.get(dPlanes, planes.size()) || !A.get(xend, cap) || !A.get(side, cap) ||
-      !A.get(keep, cap) || !A.get(nsk, cap) || !A.get(cnt, cap) || !A.get(offs, cap) || !A.get(bsum, nblk) || !A.get(total, 1) ||
-      !A.get(dep, (size_t)chunk * nbins) || !A.get(leak, chunk) || !A.get(nsec, 1) ||
+      !A.get(keep, cap) || !A.get(nsk, cap) || !A.get(offs, cap) || !A.get(bsum, nblk) || !A.get(bsec, nblk) || !A.get(total, 2) ||
+      !A.get(dep, (size_t)span * nbins) || !A.get(leak, (size_t)span) || !A.get(nsec, 1) ||
       !A.get(work, hepem_step_work_bytes(cap)))
     return HEPEM_ERR_ALLOC;
+  PinnedU32 hm;  // next round's track count, read back every round
+  if (!hm.alloc()) return HEPEM_ERR_ALLOC;
   SH_OK(hipMemcpy(dPlanes, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
 
   Slabs g;
@@ -394,7 +469,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
 
   std::vector<Kahan> ks(nbins), ks2(nbins);
   Kahan kleak, ktot;
-  std::vector<unsigned long long> hdep((size_t)chunk * nbins), hleak(chunk);
+  std::vector<unsigned long long> hdep((size_t)span * nbins), hleak((size_t)span);
   r->steps = 0;
   r->tracks = 0;
   r->iterations = 0;
@@ -402,22 +477,31 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   unsigned long long hnsec = 0;
   hipLaunchKernelGGL(k_sh_fill64, dim3(1), dim3(kB), 0, st, reinterpret_cast<int64_t*>(nsec), 1);
 
-  for (int e0 = 0; e0 < nevents; e0 += chunk) {
-    const int ne = (nevents - e0) < chunk ? (nevents - e0) : chunk;
-    hipLaunchKernelGGL(k_sh_fill64, dim3((ne * nbins + kB - 1) / kB), dim3(kB), 0, st, reinterpret_cast<int64_t*>(dep),
-                       ne * nbins);
-    hipLaunchKernelGGL(k_sh_fill64, dim3((ne + kB - 1) / kB), dim3(kB), 0, st, reinterpret_cast<int64_t*>(leak), ne);
+  for (int64_t s0 = 0; s0 < nevents; s0 += span) {
+    const int ns = (int)((nevents - s0) < span ? (nevents - s0) : span);
+    hipLaunchKernelGGL(k_sh_fill64, dim3((unsigned)(((int64_t)ns * nbins + kB - 1) / kB)), dim3(kB), 0, st,
+                       reinterpret_cast<int64_t*>(dep), ns * nbins);
+    hipLaunchKernelGGL(k_sh_fill64, dim3((ns + kB - 1) / kB), dim3(kB), 0, st, reinterpret_cast<int64_t*>(leak), ns);
+    int next = 0;  // next event of this span to stack
+    int n = 0;     // tracks in flight
     int cur = 0;
-    s[cur].t.n = ne;
-    hipLaunchKernelGGL(k_sh_init, dim3((ne + kB - 1) / kB), dim3(kB), 0, st, s[cur], g, su->primary, su->ekin0, e0, ne);
-    SH_OK(hipGetLastError());
-    r->tracks += ne;
-    int n = ne;
     int rounds = 0;
-    while (n > 0) {
+    while (true) {
+      if (n == 0) {
+        if (next >= ns) break;
+        const int k = (ns - next) < chunk ? (ns - next) : chunk;
+        s[cur].t.n = k;
+        hipLaunchKernelGGL(k_sh_inject, dim3((k + kB - 1) / kB), dim3(kB), 0, st, s[cur], g, su->primary, su->ekin0,
+                           (int)(s0 + next), next, k, nullptr, nullptr, cap);
+        SH_OK(hipGetLastError());
+        r->tracks += k;
+        next += k;
+        n = k;
+        rounds = 0;
+      }
       // every step of an e-/e+ loses energy and every gamma step ends on a plane
-      // or in an interaction, so a chunk ends; the bound only turns a broken
-      // invariant into an error instead of an endless loop
+      // or in an interaction, so the tracks in flight run out; the bound only
+      // turns a broken invariant into an error instead of an endless loop
       if (++rounds > kMaxRounds) return HEPEM_ERR_CAPACITY;
       if (n > r->max_in_flight) r->max_in_flight = n;
       Stack& a = s[cur];
@@ -425,35 +509,41 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       a.t.n = n;
       hs.n = n;
       const int grid = (n + kB - 1) / kB;
-      hipLaunchKernelGGL(k_sh_safety, dim3(grid), dim3(kB), 0, st, a, g);
       int rc = hepem_howfar(ctx, seed, &a.t, &hs, work, stream);
       if (rc != HEPEM_OK) return rc;
       hipLaunchKernelGGL(k_sh_geometry, dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side);
       rc = hepem_perform(ctx, seed, &a.t, &hs, &res, work, stream);
       if (rc != HEPEM_OK) return rc;
-      hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, nsk, cnt, nsec);
-      hipLaunchKernelGGL(k_scan_block, dim3(grid), dim3(kB), 0, st, cnt, offs, bsum, n);
-      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kB), 0, st, bsum, grid, total);
-      hipLaunchKernelGGL(k_scan_add, dim3(grid), dim3(kB), 0, st, offs, bsum, n);
-      hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, nsk, offs, total, cap);
+      hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, nsk, offs, bsum, bsec);
+      // primaries of the next events join once the stack runs low
+      const int k = (refill > 0 && n < refill && next < ns) ? ((ns - next) < chunk ? (ns - next) : chunk) : 0;
+      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kB), 0, st, bsum, bsec, grid, k, total, nsec);
+      hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, nsk, offs, bsum, total, cap);
+      if (k > 0) {
+        hipLaunchKernelGGL(k_sh_inject, dim3((k + kB - 1) / kB), dim3(kB), 0, st, b, g, su->primary, su->ekin0,
+                           (int)(s0 + next), next, k, total + 1, total, cap);
+        r->tracks += k;
+        next += k;
+        rounds = 0;
+      }
       SH_OK(hipGetLastError());
-      uint32_t m = 0;
-      SH_OK(hipMemcpyAsync(&m, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      SH_OK(hipMemcpyAsync(hm.p, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       SH_OK(hipStreamSynchronize(st));
+      const uint32_t m = *hm.p;
       r->steps += n;
       r->iterations += 1;
       if (m > (uint32_t)cap) return HEPEM_ERR_CAPACITY;
       n = (int)m;
       cur ^= 1;
     }
-    SH_OK(hipMemcpyAsync(hdep.data(), dep, (size_t)ne * nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    SH_OK(hipMemcpyAsync(hleak.data(), leak, (size_t)ne * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    SH_OK(hipMemcpyAsync(hdep.data(), dep, (size_t)ns * nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    SH_OK(hipMemcpyAsync(hleak.data(), leak, (size_t)ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     SH_OK(hipStreamSynchronize(st));
     if (r->edep_event)
-      for (size_t k = 0; k < (size_t)ne * nbins; ++k) r->edep_event[(size_t)e0 * nbins + k] = (int64_t)hdep[k];
+      for (size_t k = 0; k < (size_t)ns * nbins; ++k) r->edep_event[(size_t)s0 * nbins + k] = (int64_t)hdep[k];
     if (r->leak_event)
-      for (int e = 0; e < ne; ++e) r->leak_event[e0 + e] = (int64_t)hleak[e];
-    for (int e = 0; e < ne; ++e) {
+      for (int e = 0; e < ns; ++e) r->leak_event[s0 + e] = (int64_t)hleak[e];
+    for (int e = 0; e < ns; ++e) {
       for (int k = 0; k < nbins; ++k) {
         const double v = (double)hdep[(size_t)e * nbins + k] / kFix;
         ks[k].add(v);

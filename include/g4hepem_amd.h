@@ -182,10 +182,17 @@ uint64_t hepem_step_work_bytes(int32_t n);
 int  hepem_step_prepare(const hepem_ctx* ctx, const hepem_tracks* in, void* work, void* hip_stream);
 int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
                     hepem_step_out* res, void* work, void* hip_stream);
-/* hepem_step_run in one fused kernel (on = 0) or with the process sort
- * (on = 1). Results are identical; only the speed differs. Default: on for
- * HEPEM_FP64, off for HEPEM_FP32T (the faster of the two on MI355X). */
-int  hepem_ctx_set_process_sort(hepem_ctx* ctx, int on);
+/* hepem_step_run and hepem_*_perform in one fused kernel (HEPEM_SORT_OFF),
+ * with the process sort (HEPEM_SORT_ON), or with the sort for calls of at
+ * least HEPEM_SORT_AUTO_MIN_TRACKS slots (HEPEM_SORT_AUTO: below that the two
+ * extra launches cost more than the sort saves). Results are identical; only
+ * the speed differs. Default: HEPEM_SORT_AUTO for HEPEM_FP64, HEPEM_SORT_OFF
+ * for HEPEM_FP32T (the faster choices on MI355X). */
+#define HEPEM_SORT_OFF  0
+#define HEPEM_SORT_ON   1
+#define HEPEM_SORT_AUTO 2
+#define HEPEM_SORT_AUTO_MIN_TRACKS 32768
+int  hepem_ctx_set_process_sort(hepem_ctx* ctx, int mode);
 
 /* Number of tracks the last step on `work` rejected (invalid couple index or
  * non-positive/non-finite energy; such tracks get winner -99 and are left
@@ -216,7 +223,8 @@ int  hepem_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const h
                    hepem_step_out* res, void* work, void* hip_stream);
 /* The same for batches of one kind: every slot must hold an e-/e+
  * (electron calls) or a gamma (gamma calls); other slots are rejected. No
- * queue is built; `work` only holds the rejection counter (NULL allowed). */
+ * particle queue is built; `work` holds the rejection counter and, with the
+ * process sort, the staged interactions (NULL allowed). */
 int  hepem_electron_howfar(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
                            void* work, void* hip_stream);
 int  hepem_electron_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
@@ -233,9 +241,10 @@ int  hepem_gamma_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, c
  * (counter fill, one batched classify, one batched step) -- no allocation, no
  * host copies -- so a run can be captured into a hipGraph and replayed.
  * Results are bit-identical to calling hepem_step on each batch with the same
- * seed. A plan sorts pending interactions by process (as hepem_step_run) when
- * its context does so at plan creation (hepem_ctx_set_process_sort). The
- * batches' arrays must stay valid while the plan is used. */
+ * seed. A plan sorts pending interactions by process (as hepem_step_run) as
+ * its context's mode at plan creation says for the plan's total slot count
+ * (hepem_ctx_set_process_sort). The batches' arrays must stay valid while the
+ * plan is used. */
 typedef struct hepem_batch_plan hepem_batch_plan;
 int  hepem_batch_plan_create(const hepem_ctx* ctx, const hepem_batch_desc* descs, int32_t nbatch,
                              hepem_batch_plan** out_plan);

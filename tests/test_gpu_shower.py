@@ -56,14 +56,19 @@ def test_shower_matches_cpu_harness(tables1, ctx1, oracle):
     assert abs(steps - g["steps"]) < 0.02 * steps
 
 
-def test_shower_independent_of_chunking(ctx1):
+@pytest.mark.parametrize("chunk,refill", [(7, 0), (3, 2000), (5, 1 << 30)])
+def test_shower_independent_of_chunking(ctx1, chunk, refill):
+    """Per-event results do not depend on how many events are stacked together
+    or on when the next events join the tracks still in flight (refill 1<<30:
+    new primaries every round)."""
     from g4hepem_amd._native import slab_setup
-    a = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 2, 500.0, capacity=1 << 20, events_per_chunk=7), 40, 5,
-                      per_event=True)
+    a = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 2, 500.0, capacity=1 << 20, events_per_chunk=chunk,
+                                 refill_below=refill), 40, 5, per_event=True)
     b = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 2, 500.0, capacity=1 << 20, events_per_chunk=40), 40, 5,
                       per_event=True)
     assert np.array_equal(a["edep_event"], b["edep_event"]) and np.array_equal(a["leak_event"], b["leak_event"])
     assert a["steps"] == b["steps"] and a["tracks"] == b["tracks"]
+    assert a["edep_sum"].tobytes() == b["edep_sum"].tobytes()
 
 
 def test_shower_capacity_error(ctx1):

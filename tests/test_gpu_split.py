@@ -79,12 +79,19 @@ def test_split_equals_fused(ctx64, ctx32, dev, form):
         assert same_bits(a_t[k][m], b_t[k][m]), k
 
 
-def test_split_geometry_golden(golden_geom, tables2, ctx64, dev):
-    """Geometry-limited 3-step chain against the compiled reference's vectors."""
+@pytest.mark.parametrize("sort", ["auto", "on"])
+def test_split_geometry_golden(golden_geom, tables2, ctx64, dev, sort):
+    """Geometry-limited 3-step chain against the compiled reference's vectors,
+    with the default Perform (one kernel at this size) and with the process
+    sort forced on."""
+    from g4hepem_amd import HEPEM_FP64, Context
     assert tables2.digest() == str(golden_geom["digest_set2"])
+    if sort == "on":
+        ctx64 = Context(tables2, HEPEM_FP64)
+        ctx64.set_process_sort(True)
     cur = golden_tracks(golden_geom, "gc_in")
     for s in range(3):
-        _print(f"geometry chain step {s}")
+        _print(f"geometry chain step {s} (sort {sort})")
         alive = cur["ekin"] > 0
         lim = golden_geom[f"gc{s}_limit"]
         t, res, nrej, _ = gpu_split_step(ctx64, STEP_SEED + 2, cur, dev, geom_limit=lim)
@@ -140,3 +147,31 @@ def test_split_per_kind_entry_points(ctx64, dev):
         assert nrej == wrong.sum()
         assert np.all(res["winner"][wrong] == -99) and np.all(res["winner"][~wrong] != -99)
         assert same_bits(t["ekin"][wrong], mixed["ekin"][wrong])
+
+
+@pytest.mark.parametrize("kind", ["mixed", "electron", "gamma"])
+def test_split_process_sort_equals_unsorted(tables2, dev, kind):
+    """Perform with the pending interactions sorted by process (three kernels)
+    equals the single Perform kernel bit for bit, with a geometry limit and a
+    rejected track."""
+    from g4hepem_amd import HEPEM_FP64, Context
+    from g4hepem_amd.tracks import step_tracks
+    ctx = Context(tables2, HEPEM_FP64)
+    n = 150000
+    tr = step_tracks(n, 66, 4, lnemin=math.log(1e-3), lnemax=math.log(1e7))
+    lim = np.exp(np.random.Generator(np.random.Philox(67)).uniform(math.log(1e-4), math.log(20.0), n))
+    if kind != "mixed":
+        sel = (tr["ptype"] != 2) if kind == "electron" else (tr["ptype"] == 2)
+        tr = {k: v.reshape(TRACK_FIELDS[k][1], -1)[:, sel].ravel() for k, v in tr.items()}
+        lim = lim[sel]
+    tr["imc"][11] = 9  # one rejected track
+    _print(f"split process sort {kind}: {len(tr['ekin'])} tracks")
+    ctx.set_process_sort(True)
+    a_t, a_r, a_rej, _ = gpu_split_step(ctx, 35, tr, dev, geom_limit=lim, kind=kind)
+    ctx.set_process_sort(False)
+    b_t, b_r, b_rej, _ = gpu_split_step(ctx, 35, tr, dev, geom_limit=lim, kind=kind)
+    assert a_rej == b_rej == 1
+    for k in OUT_FIELDS:
+        assert same_bits(a_r[k], b_r[k]), (kind, k)
+    for k in TRACK_FIELDS:
+        assert same_bits(a_t[k], b_t[k]), (kind, k)
