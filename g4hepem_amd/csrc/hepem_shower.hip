@@ -10,9 +10,10 @@
 //   k_sh_post      MSC displacement, slab crossing, scoring, leakage, the
 //                  number of stack slots each track needs next round and
 //                  their exclusive scan within the block
-//   k_scan_sums    exclusive scan of the block totals (one block)
-//   k_sh_scatter   survivors, then their secondaries, into the other buffer in
-//                  slot order (deterministic), planar with the new count
+//   k_scan_sums    exclusive scans of the block totals (one block)
+//   k_sh_scatter   survivors and their secondaries into the other buffer:
+//                  e-/e+ first, then gammas, each in slot order of the parents
+//                  (deterministic), planar with the new count
 //   k_sh_inject    when the stack runs low, the primaries of the next events
 //                  after them.
 #include <hip/hip_runtime.h>
@@ -113,31 +114,44 @@ __device__ inline double Safety(const Slabs& g, int l, double x) {
   return fmax(0.0, fmin(x - g.plane[l], g.plane[l + 1] - x));
 }
 
-// Exclusive scan of v over the block (blockDim.x == kB) and the block sums of
-// v and w. Every thread of the block must call it.
-__device__ inline uint32_t BlockScan(uint32_t v, uint32_t w, uint32_t& vsum, uint32_t& wsum) {
+// Exclusive scans of a and b over the block (blockDim.x == kB) into xa, xb and
+// the block sums of a, b and w. Every thread of the block must call it.
+__device__ inline void BlockScan2(uint32_t a, uint32_t b, uint32_t w, uint32_t& xa, uint32_t& xb, uint32_t& sa, uint32_t& sb,
+                                  uint32_t& sw) {
   constexpr int kW = kB / 64;
-  __shared__ uint32_t shv[kW], shw[kW];
+  __shared__ uint32_t sh[3][kW];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  uint32_t x = v, y = w;
+  uint32_t x = a, y = b, z = w;
   for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t a = __shfl_up(x, o, 64);
-    if (lane >= o) x += a;
+    const uint32_t ux = __shfl_up(x, o, 64), uy = __shfl_up(y, o, 64);
+    if (lane >= o) {
+      x += ux;
+      y += uy;
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
-  if (lane == 63) shv[wv] = x;
-  if (lane == 0) shw[wv] = y;
+  for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
+  if (lane == 63) {
+    sh[0][wv] = x;
+    sh[1][wv] = y;
+  }
+  if (lane == 0) sh[2][wv] = z;
   __syncthreads();
-  uint32_t base = 0, tv = 0, tw = 0;
+  uint32_t ba = 0, bb = 0, ta = 0, tb = 0, tw = 0;
   for (int k = 0; k < kW; ++k) {
-    const uint32_t sv = shv[k];
-    if (k < wv) base += sv;
-    tv += sv;
-    tw += shw[k];
+    const uint32_t u = sh[0][k], v = sh[1][k];
+    if (k < wv) {
+      ba += u;
+      bb += v;
+    }
+    ta += u;
+    tb += v;
+    tw += sh[2][k];
   }
-  vsum = tv;
-  wsum = tw;
-  return base + x - v;
+  xa = ba + x - a;
+  xb = bb + y - b;
+  sa = ta;
+  sb = tb;
+  sw = tw;
 }
 
 // Cuts the proposed geometrical step at the next plane along the pre-step
@@ -171,22 +185,26 @@ __global__ void k_sh_geometry(Stack s, Slabs g, hepem_howfar_state hs, double* x
   }
 }
 
-// After Perform: position, slab crossing, scoring and next-round slot count.
-// keep[i] = 1 if the track itself goes on, nsk[i] = secondaries to stack;
-// offs[i] = exclusive scan of keep + nsk within the block, bsum/bsec[block] =
-// the block's slot and secondary totals. Deposits are summed per wave and
-// (event, slab) before the atomic: lanes of a wave mostly hold tracks of one
-// event, and the fixed-point sum does not depend on the grouping.
+// After Perform: position, slab crossing, scoring and next-round slot counts.
+// The next stack holds the e-/e+ first and the gammas after them (so the
+// library's particle-type queue covers runs of consecutive slots), each part
+// in slot order of the parents. keep[i] = 1 if the track itself goes on,
+// nsk[i] = secondaries to stack; offE/offG[i] = exclusive scans within the
+// block of the e-/e+ and gamma slots track i needs; bsumE/bsumG/bsec[block] =
+// the block's totals of those and of its secondaries. Deposits are summed per
+// block and (event, slab) before the global atomic; the fixed-point sum does
+// not depend on the grouping.
 __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out res, const double* __restrict__ xend,
                           const int8_t* __restrict__ side, unsigned long long* dep, unsigned long long* leak,
-                          uint8_t* keep, uint8_t* nsk, uint32_t* offs, uint32_t* bsum, uint32_t* bsec) {
+                          uint8_t* keep, uint8_t* nsk, uint32_t* offE, uint32_t* offG, uint32_t* bsumE, uint32_t* bsumG,
+                          uint32_t* bsec) {
   const int i = blockIdx.x * kB + threadIdx.x;
   const int n = s.t.n;
   const bool valid = i < n;
   const hepem_tracks& t = s.t;
   constexpr unsigned long long kNone = ~0ull;
   unsigned long long key = kNone, val = 0ull;
-  uint32_t c = 0u, ns = 0u;
+  uint32_t cE = 0u, cG = 0u, ns = 0u;
   if (valid) {
     const int ev = s.event[i];
     int l = s.layer[i];
@@ -236,83 +254,125 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
     ns = (rejected || left) ? 0u : (uint32_t)res.nsec[i];
     keep[i] = alive ? 1 : 0;
     nsk[i] = (uint8_t)ns;
-    c = (alive ? 1u : 0u) + ns;
+    if (alive) {
+      if (isE) ++cE;
+      else ++cG;
+    }
+    for (uint32_t sl = 0; sl < ns; ++sl) {
+      if (res.sec_ptype[sl * n + i] == 2) ++cG;
+      else ++cE;
+    }
   }
-  // per-wave aggregation of the deposits: one atomic per distinct (event, slab)
-  unsigned long long pending = __ballot(key != kNone);
-  const int lane = threadIdx.x & 63;
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const unsigned long long k = __shfl(key, leader, 64);
-    const bool mine = key == k;
-    unsigned long long v = mine ? val : 0ull;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == leader) atomicAdd(&dep[k], v);
-    pending &= ~__ballot(mine);
+  // per-block aggregation of the deposits in an LDS hash table (open
+  // addressing, at most kB keys in 2 kB entries, so every probe sequence ends):
+  // one global atomic per distinct (event, slab) of the block
+  constexpr int kHash = 2 * kB;
+  __shared__ unsigned long long hkey[kHash], hval[kHash];
+  for (int e = threadIdx.x; e < kHash; e += kB) {
+    hkey[e] = kNone;
+    hval[e] = 0ull;
   }
-  uint32_t tc, tns;
-  const uint32_t off = BlockScan(c, ns, tc, tns);
-  if (valid) offs[i] = off;
+  __syncthreads();
+  if (key != kNone) {
+    uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 55) & (kHash - 1);
+    while (true) {
+      const unsigned long long prev = atomicCAS(&hkey[h], kNone, key);
+      if (prev == kNone || prev == key) {
+        atomicAdd(&hval[h], val);
+        break;
+      }
+      h = (h + 1) & (kHash - 1);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kHash; e += kB) {
+    const unsigned long long k = hkey[e];
+    if (k != kNone) atomicAdd(&dep[k], hval[e]);
+  }
+  uint32_t xE, xG, tE, tG, tns;
+  BlockScan2(cE, cG, ns, xE, xG, tE, tG, tns);
+  if (valid) {
+    offE[i] = xE;
+    offG[i] = xG;
+  }
   if (threadIdx.x == 0) {
-    bsum[blockIdx.x] = tc;
+    bsumE[blockIdx.x] = tE;
+    bsumG[blockIdx.x] = tG;
     bsec[blockIdx.x] = tns;
   }
 }
 
-// In-place exclusive scan of the nb block sums by one block; the round's
-// secondaries are added to *nsecTotal. total[1] = slots of survivors and
-// secondaries, total[0] = total[1] + `inject` (primaries stacked after them).
-__global__ void __launch_bounds__(kB) k_scan_sums(uint32_t* bsum, const uint32_t* __restrict__ bsec, int nb, int inject,
-                                                  uint32_t* total, unsigned long long* nsecTotal) {
-  __shared__ uint32_t carry;
+// In-place exclusive scans of the nb block sums of e-/e+ and gamma slots by
+// one block; the round's secondaries are added to *nsecTotal. The `inject`
+// primaries (e-/e+ if injectE) go to the end of their part of the next stack:
+// total[0] = tracks of the next stack, total[1] = first slot of the injected
+// primaries, total[2] = slots of the e-/e+ part.
+__global__ void __launch_bounds__(kB) k_scan_sums(uint32_t* bsumE, uint32_t* bsumG, const uint32_t* __restrict__ bsec, int nb,
+                                                  int inject, int injectE, uint32_t* total, unsigned long long* nsecTotal) {
+  __shared__ uint32_t carryE, carryG;
   __shared__ unsigned long long secs;
   if (threadIdx.x == 0) {
-    carry = 0u;
+    carryE = 0u;
+    carryG = 0u;
     secs = 0ull;
   }
   __syncthreads();
   for (int base = 0; base < nb; base += kB) {
     const int i = base + threadIdx.x;
-    const uint32_t v = i < nb ? bsum[i] : 0u;
+    const uint32_t a = i < nb ? bsumE[i] : 0u;
+    const uint32_t b = i < nb ? bsumG[i] : 0u;
     const uint32_t w = i < nb ? bsec[i] : 0u;
-    uint32_t tv, tw;
-    const uint32_t ex = BlockScan(v, w, tv, tw);
-    if (i < nb) bsum[i] = carry + ex;
+    uint32_t xa, xb, ta, tb, tw;
+    BlockScan2(a, b, w, xa, xb, ta, tb, tw);
+    if (i < nb) {
+      bsumE[i] = carryE + xa;
+      bsumG[i] = carryG + xb;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-      carry += tv;
+      carryE += ta;
+      carryG += tb;
       secs += tw;
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    total[0] = carry + (uint32_t)inject;
-    total[1] = carry;
+    const uint32_t k = (uint32_t)inject;
+    const uint32_t elen = carryE + (injectE ? k : 0u);
+    total[0] = carryE + carryG + k;
+    total[1] = injectE ? carryE : elen + carryG;
+    total[2] = elen;
     *nsecTotal += secs;
   }
 }
 
-// Survivor i goes to slot bsum[block] + offs[i], its secondaries to the
-// following slots; each starts with the isotropic safety of its position.
-// Writes nothing beyond `cap` (the host then reports the overflow).
+// Survivor i and its secondaries go to the next e-/e+ slots from
+// bsumE[block] + offE[i] and the next gamma slots from total[2] +
+// bsumG[block] + offG[i], in the order survivor, secondary 0, secondary 1;
+// each starts with the isotropic safety of its position. Writes nothing beyond
+// `cap` (the host then reports the overflow).
 __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, hepem_step_out res, const uint8_t* __restrict__ keep,
-                             const uint8_t* __restrict__ nsk, const uint32_t* __restrict__ offs,
-                             const uint32_t* __restrict__ bsum, const uint32_t* __restrict__ total, int cap) {
+                             const uint8_t* __restrict__ nsk, const uint32_t* __restrict__ offE,
+                             const uint32_t* __restrict__ offG, const uint32_t* __restrict__ bsumE,
+                             const uint32_t* __restrict__ bsumG, const uint32_t* __restrict__ total, int cap) {
   const int i = blockIdx.x * kB + threadIdx.x;
   const int n = a.t.n;
   if (i >= n) return;
-  const int m = (int)*total;
+  const int m = (int)total[0];
   if (m > cap) return;
   const hepem_tracks& s = a.t;
   const hepem_tracks& d = b.t;
-  int j = (int)(offs[i] + bsum[blockIdx.x]);
+  int jE = (int)(bsumE[blockIdx.x] + offE[i]);
+  int jG = (int)(total[2] + bsumG[blockIdx.x] + offG[i]);
   const double x = a.x[i];
   const int l = a.layer[i];
   const double safety = Safety(g, l, x);
   const uint64_t sid = ((uint64_t)s.rng_stream[i] << 32) | (uint32_t)s.id[i];
   const int ns1 = a.nstep[i] + 1;
   if (keep[i]) {
-    d.ptype[j] = s.ptype[i];
+    const int8_t pt = s.ptype[i];
+    const int j = pt == 2 ? jG++ : jE++;
+    d.ptype[j] = pt;
     d.id[j] = s.id[i];
     d.rng_stream[j] = s.rng_stream[i];
     d.imc[j] = g.imc[l % g.nabs];
@@ -332,13 +392,14 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
     b.layer[j] = l;
     b.event[j] = a.event[i];
     b.nstep[j] = ns1;
-    ++j;
   }
   const int nsec = nsk[i];
   for (int sl = 0; sl < 2; ++sl) {
     if (sl >= nsec) break;
     const uint64_t cid = ChildStream(sid, ns1, sl);
-    d.ptype[j] = res.sec_ptype[sl * n + i];
+    const int8_t pt = res.sec_ptype[sl * n + i];
+    const int j = pt == 2 ? jG++ : jE++;
+    d.ptype[j] = pt;
     d.id[j] = (int32_t)(uint32_t)cid;
     d.rng_stream[j] = (uint32_t)(cid >> 32);
     d.imc[j] = g.imc[l % g.nabs];
@@ -358,7 +419,6 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
     b.layer[j] = l;
     b.event[j] = a.event[i];
     b.nstep[j] = 0;
-    ++j;
   }
 }
 
@@ -442,7 +502,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   double *dPlanes, *xend;
   int8_t* side;
   uint8_t *keep, *nsk, *work;
-  uint32_t *offs, *bsum, *bsec, *total;
+  uint32_t *offs, *bsum, *bsec, *total;  // offs: e-/e+ then gamma block offsets (2 x cap); bsum: 2 x nblk
   unsigned long long *dep, *leak, *nsec;
   const int nblk = (cap + kB - 1) / kB;
   if (!AllocStack(A, s[0], cap) || !AllocStack(A, s[1], cap) || !A.get(hs.geom_step, cap) || !A.get(hs.range, cap) ||
@@ -453,7 +513,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       !A.get(res.displacement, 3 * (size_t)cap) || !A.get(res.winner, cap) || !A.get(res.stopped, cap) ||
       !A.get(res.nsec, cap) || !A.get(res.sec_ptype, 2 * (size_t)cap) || !A.get(res.sec_ekin, 2 * (size_t)cap) ||
       !A.get(res.sec_dir, 6 * (size_t)cap) || !A.get(dPlanes, planes.size()) || !A.get(xend, cap) || !A.get(side, cap) ||
-      !A.get(keep, cap) || !A.get(nsk, cap) || !A.get(offs, cap) || !A.get(bsum, nblk) || !A.get(bsec, nblk) || !A.get(total, 2) ||
+      !A.get(keep, cap) || !A.get(nsk, cap) || !A.get(offs, 2 * (size_t)cap) || !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) ||
       !A.get(dep, (size_t)span * nbins) || !A.get(leak, (size_t)span) || !A.get(nsec, 1) ||
       !A.get(work, hepem_step_work_bytes(cap)))
     return HEPEM_ERR_ALLOC;
@@ -514,11 +574,14 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       hipLaunchKernelGGL(k_sh_geometry, dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side);
       rc = hepem_perform(ctx, seed, &a.t, &hs, &res, work, stream);
       if (rc != HEPEM_OK) return rc;
-      hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, nsk, offs, bsum, bsec);
+      hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, nsk, offs, offs + cap,
+                         bsum, bsum + nblk, bsec);
       // primaries of the next events join once the stack runs low
       const int k = (refill > 0 && n < refill && next < ns) ? ((ns - next) < chunk ? (ns - next) : chunk) : 0;
-      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kB), 0, st, bsum, bsec, grid, k, total, nsec);
-      hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, nsk, offs, bsum, total, cap);
+      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kB), 0, st, bsum, bsum + nblk, bsec, grid, k, su->primary != 2 ? 1 : 0,
+                         total, nsec);
+      hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, nsk, offs, offs + cap, bsum, bsum + nblk,
+                         total, cap);
       if (k > 0) {
         hipLaunchKernelGGL(k_sh_inject, dim3((k + kB - 1) / kB), dim3(kB), 0, st, b, g, su->primary, su->ekin0,
                            (int)(s0 + next), next, k, total + 1, total, cap);
