@@ -167,7 +167,7 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int):
     the oracle's CPU harness on the first `cpu_events` events for the per-slab
     chi2 and the fraction of events identical to the GPU's."""
     from g4hepem_amd._native import slab_setup
-    su = slab_setup(50, [2.3, 5.7], [0, 1], 0, 1.0e4, capacity=1 << 23, events_per_chunk=1024, refill_below=1 << 20)
+    su = slab_setup(50, [2.3, 5.7], [0, 1], 0, 1.0e4, capacity=1 << 24, events_per_chunk=4096, refill_below=1 << 21)
     t0 = time.perf_counter()
     g = ctx.slab_run(su, nevents, seed, per_event=True)
     wall = time.perf_counter() - t0

@@ -114,11 +114,12 @@ __device__ inline double Safety(const Slabs& g, int l, double x) {
   return fmax(0.0, fmin(x - g.plane[l], g.plane[l + 1] - x));
 }
 
-// Exclusive scans of a and b over the block (blockDim.x == kB) into xa, xb and
+// Exclusive scans of a and b over the block (blockDim.x == NT) into xa, xb and
 // the block sums of a, b and w. Every thread of the block must call it.
+template <int NT>
 __device__ inline void BlockScan2(uint32_t a, uint32_t b, uint32_t w, uint32_t& xa, uint32_t& xb, uint32_t& sa, uint32_t& sb,
                                   uint32_t& sw) {
-  constexpr int kW = kB / 64;
+  constexpr int kW = NT / 64;
   __shared__ uint32_t sh[3][kW];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t x = a, y = b, z = w;
@@ -290,7 +291,7 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
     if (k != kNone) atomicAdd(&dep[k], hval[e]);
   }
   uint32_t xE, xG, tE, tG, tns;
-  BlockScan2(cE, cG, ns, xE, xG, tE, tG, tns);
+  BlockScan2<kB>(cE, cG, ns, xE, xG, tE, tG, tns);
   if (valid) {
     offE[i] = xE;
     offG[i] = xG;
@@ -307,7 +308,8 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
 // primaries (e-/e+ if injectE) go to the end of their part of the next stack:
 // total[0] = tracks of the next stack, total[1] = first slot of the injected
 // primaries, total[2] = slots of the e-/e+ part.
-__global__ void __launch_bounds__(kB) k_scan_sums(uint32_t* bsumE, uint32_t* bsumG, const uint32_t* __restrict__ bsec, int nb,
+constexpr int kScanB = 1024;  // threads of the one-block scan
+__global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t* bsumG, const uint32_t* __restrict__ bsec, int nb,
                                                   int inject, int injectE, uint32_t* total, unsigned long long* nsecTotal) {
   __shared__ uint32_t carryE, carryG;
   __shared__ unsigned long long secs;
@@ -317,13 +319,13 @@ __global__ void __launch_bounds__(kB) k_scan_sums(uint32_t* bsumE, uint32_t* bsu
     secs = 0ull;
   }
   __syncthreads();
-  for (int base = 0; base < nb; base += kB) {
+  for (int base = 0; base < nb; base += kScanB) {
     const int i = base + threadIdx.x;
     const uint32_t a = i < nb ? bsumE[i] : 0u;
     const uint32_t b = i < nb ? bsumG[i] : 0u;
     const uint32_t w = i < nb ? bsec[i] : 0u;
     uint32_t xa, xb, ta, tb, tw;
-    BlockScan2(a, b, w, xa, xb, ta, tb, tw);
+    BlockScan2<kScanB>(a, b, w, xa, xb, ta, tb, tw);
     if (i < nb) {
       bsumE[i] = carryE + xa;
       bsumG[i] = carryG + xb;
@@ -494,6 +496,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   if (span > nevents) span = nevents;
   if (span < 1) span = 1;
   const int refill = su->refill_below;
+  int growRounds = -1;  // rounds from stacking the first events to their largest stack (-1: not yet seen)
 
   Arena A;
   Stack s[2];
@@ -546,6 +549,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
     int n = 0;     // tracks in flight
     int cur = 0;
     int rounds = 0;
+    int sinceInject = 0;  // rounds stepped since primaries were last stacked
     while (true) {
       if (n == 0) {
         if (next >= ns) break;
@@ -558,6 +562,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
         next += k;
         n = k;
         rounds = 0;
+        sinceInject = 0;
       }
       // every step of an e-/e+ loses energy and every gamma step ends on a plane
       // or in an interaction, so the tracks in flight run out; the bound only
@@ -576,9 +581,13 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       if (rc != HEPEM_OK) return rc;
       hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, nsk, offs, offs + cap,
                          bsum, bsum + nblk, bsec);
-      // primaries of the next events join once the stack runs low
-      const int k = (refill > 0 && n < refill && next < ns) ? ((ns - next) < chunk ? (ns - next) : chunk) : 0;
-      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kB), 0, st, bsum, bsum + nblk, bsec, grid, k, su->primary != 2 ? 1 : 0,
+      // primaries of the next events join once the stack runs low, and no
+      // sooner after the last ones than the first events took to reach their
+      // largest stack: the stack then stays below about refill plus one chunk's
+      // largest stack
+      const bool refillNow = refill > 0 && n < refill && next < ns && growRounds >= 0 && sinceInject >= growRounds;
+      const int k = refillNow ? ((ns - next) < chunk ? (ns - next) : chunk) : 0;
+      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanB), 0, st, bsum, bsum + nblk, bsec, grid, k, su->primary != 2 ? 1 : 0,
                          total, nsec);
       hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, nsk, offs, offs + cap, bsum, bsum + nblk,
                          total, cap);
@@ -596,6 +605,8 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       r->steps += n;
       r->iterations += 1;
       if (m > (uint32_t)cap) return HEPEM_ERR_CAPACITY;
+      if (growRounds < 0 && (int)m < n) growRounds = sinceInject + 1;  // the stack shrank for the first time
+      sinceInject = k > 0 ? 0 : sinceInject + 1;
       n = (int)m;
       cur ^= 1;
     }

@@ -44,8 +44,10 @@ typedef struct hepem_slab_setup {
   double  ekin0;                        /* primary kinetic energy [MeV], along +x from x=0 */
   int32_t capacity;                     /* track-stack capacity (tracks in flight)         */
   int32_t events_per_chunk;             /* events whose primaries are stacked together     */
-  int32_t refill_below;                 /* stack the next events_per_chunk primaries as
-                                           soon as fewer tracks than this are in flight
+  int32_t refill_below;                 /* stack the next events_per_chunk primaries once
+                                           fewer tracks than this are in flight, no sooner
+                                           after the previous ones than the first events
+                                           took to reach their largest stack
                                            (0: only once the stack is empty)               */
 } hepem_slab_setup;
 
@@ -68,8 +70,9 @@ typedef struct hepem_slab_result {
 /* Simulates `nevents` events (event e has primary stream id derived from
  * (seed, e)), stacking their primaries setup->events_per_chunk at a time:
  * into the empty stack, or, with refill_below > 0, beside the tracks still in
- * flight once fewer than refill_below remain (so stepping rounds stay full).
- * Results do not depend on either setting. Allocates its stack on the
+ * flight once fewer than refill_below remain (so stepping rounds stay full);
+ * the stack then holds at most about refill_below plus the largest stack of one
+ * chunk of events. Results do not depend on either setting. Allocates its stack on the
  * context's device for the duration of the call; synchronises `hip_stream`.
  * HEPEM_ERR_CAPACITY if the stack would exceed setup->capacity. */
 int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* setup, int32_t nevents, uint64_t seed,

@@ -60,7 +60,7 @@ def test_shower_matches_cpu_harness(tables1, ctx1, oracle):
 def test_shower_independent_of_chunking(ctx1, chunk, refill):
     """Per-event results do not depend on how many events are stacked together
     or on when the next events join the tracks still in flight (refill 1<<30:
-    new primaries every round)."""
+    new primaries whenever the first events' growth period has passed)."""
     from g4hepem_amd._native import slab_setup
     a = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 2, 500.0, capacity=1 << 20, events_per_chunk=chunk,
                                  refill_below=refill), 40, 5, per_event=True)
