@@ -33,10 +33,9 @@ class Context:
         return b.value
 
     def set_process_sort(self, on) -> None:
-        """Sort of the pending interactions by process: True (always), False
-        (never) or "auto" (the FP64 default: hepem_step_run calls of at least
-        32768 slots; the split Perform calls then stay unsorted); results are
-        identical."""
+        """Sort of the pending interactions by process in the step and the
+        Perform calls: True (always), False (never) or "auto" (calls of at
+        least 32768 slots; the FP64 default); results are identical."""
         mode = 2 if on == "auto" else (1 if on else 0)
         N.check("hepem_ctx_set_process_sort", N.lib().hepem_ctx_set_process_sort(self.h, mode))
 

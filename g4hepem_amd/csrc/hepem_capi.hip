@@ -421,14 +421,10 @@ extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
 }
 
 namespace {
-// whether a fused step over n slots sorts its pending interactions by process
+// whether a step or Perform over n slots sorts its pending interactions by process
 bool SortsProcesses(const hepem_ctx* c, int64_t n) {
   return c->processSort == HEPEM_SORT_ON || (c->processSort == HEPEM_SORT_AUTO && n >= HEPEM_SORT_AUTO_MIN_TRACKS);
 }
-// whether a split Perform sorts: only on request (in shower populations most
-// e-/e+ steps end without a discrete interaction, and the unsorted kernel
-// measured faster)
-bool PerformSortsProcesses(const hepem_ctx* c) { return c->processSort == HEPEM_SORT_ON; }
 }  // namespace
 
 extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
@@ -544,7 +540,7 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
     hipLaunchKernelGGL((k_howfar<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, queue, counters);
   } else if (!perform) {
     hipLaunchKernelGGL((k_howfar<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, queue, counters);
-  } else if (PerformSortsProcesses(c)) {
+  } else if (SortsProcesses(c, n)) {
     // choice of the interaction, process sort, final states (as hepem_step_run)
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
     if (fp64)

@@ -182,14 +182,12 @@ uint64_t hepem_step_work_bytes(int32_t n);
 int  hepem_step_prepare(const hepem_ctx* ctx, const hepem_tracks* in, void* work, void* hip_stream);
 int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
                     hepem_step_out* res, void* work, void* hip_stream);
-/* hepem_step_run (and batch plans) in one fused kernel (HEPEM_SORT_OFF), with
- * the process sort (HEPEM_SORT_ON), or with the sort for calls of at least
+/* hepem_step_run, batch plans and the split Perform calls (hepem_perform,
+ * hepem_*_perform) in one fused kernel (HEPEM_SORT_OFF), with the process sort
+ * (HEPEM_SORT_ON), or with the sort for calls of at least
  * HEPEM_SORT_AUTO_MIN_TRACKS slots (HEPEM_SORT_AUTO: below that the two extra
- * launches cost more than the sort saves). The split Perform calls
- * (hepem_perform, hepem_*_perform) sort only with HEPEM_SORT_ON: on shower
- * populations, where most e-/e+ steps end without a discrete interaction,
- * their single kernel is the faster. Results are identical; only the speed
- * differs. Default: HEPEM_SORT_AUTO for HEPEM_FP64, HEPEM_SORT_OFF for
+ * launches cost more than the sort saves). Results are identical; only the
+ * speed differs. Default: HEPEM_SORT_AUTO for HEPEM_FP64, HEPEM_SORT_OFF for
  * HEPEM_FP32T (the faster choices on MI355X). */
 #define HEPEM_SORT_OFF  0
 #define HEPEM_SORT_ON   1
