@@ -130,9 +130,12 @@ template <typename R> struct SecList {
 // G4HepEmTrack::SetNumIALeft / GetMFP at a run-time process index, written as
 // selects over the four slots so the arrays stay in registers
 template <typename R> HD void ResetNumIALeft(R* nial, int ip) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (k == ip) nial[k] = (R)-1.0;
+  // every slot is written with a select: a conditional store at a run-time
+  // slot would be merged into one indexed store and put the array in scratch
+  nial[0] = ip == 0 ? (R)-1.0 : nial[0];
+  nial[1] = ip == 1 ? (R)-1.0 : nial[1];
+  nial[2] = ip == 2 ? (R)-1.0 : nial[2];
+  nial[3] = ip == 3 ? (R)-1.0 : nial[3];
 }
 template <typename R> HD R PickMFP(const R* mfp, int ip) {
   return ip == 0 ? mfp[0] : ip == 1 ? mfp[1] : ip == 2 ? mfp[2] : mfp[3];
@@ -1014,30 +1017,28 @@ template <typename R> HD bool ApplyMeanEnergyLoss(const DevTables<R>& T, ElState
   Trk<R>& t = el.t;
   const double ekin = t.ekin;
   const R range = el.range;
-  if (pStep >= range || ekin <= T.minLossE) {
-    t.edep = ekin;
-    t.setE(0.0);
-    return true;
+  // one store site per field on every path (stores of different fields on
+  // the two sides of a branch get merged into an indexed store, which puts
+  // the track state in scratch memory)
+  double eloss = ekin;
+  bool stopped = true;
+  if (!(pStep >= range || ekin <= T.minLossE)) {
+    const DevEl<R>& e = T.el[t.isEl ? 0 : 1];
+    const int imc = t.imc;
+    const R lekin = t.logE();
+    eloss = (double)(pStep * GetRestDEDX(e, imc, ekin, lekin));
+    const double linLim = T.regLinLim[T.mcReg[imc]];
+    if (eloss > ekin * linLim) {
+      const double postRange = (double)range - (double)pStep;
+      eloss = ekin - GetInvRange(e, imc, postRange);
+    }
+    eloss = tmax(eloss, 0.0);
+    stopped = eloss >= ekin;
+    if (stopped) eloss = ekin;
   }
-  const DevEl<R>& e = T.el[t.isEl ? 0 : 1];
-  const int imc = t.imc;
-  const R lekin = t.logE();
-  double eloss = (double)(pStep * GetRestDEDX(e, imc, ekin, lekin));
-  const double linLim = T.regLinLim[T.mcReg[imc]];
-  if (eloss > ekin * linLim) {
-    const double postRange = (double)range - (double)pStep;
-    eloss = ekin - GetInvRange(e, imc, postRange);
-  }
-  eloss = tmax(eloss, 0.0);
-  if (eloss >= ekin) {
-    eloss = ekin;
-    t.setE(0.0);
-    t.edep = eloss;
-    return true;
-  }
-  t.setE(ekin - eloss);
+  t.setE(stopped ? 0.0 : ekin - eloss);
   t.edep = eloss;
-  return false;
+  return stopped;
 }
 
 template <typename R> HD void SampleMSC(const DevTables<R>& T, ElState<R>& el, Rng& rng) {  // :261-322
@@ -1087,16 +1088,14 @@ template <typename R> HD bool SampleLossFluctuations(const DevTables<R>& T, ElSt
     eloss = tmax(eloss, 0.0);
     finalEkin = preEkin - eloss;
   }
-  if (finalEkin <= T.elTrackCut) {
+  const bool stopped = finalEkin <= T.elTrackCut;
+  if (stopped) {
     eloss = preEkin;
     finalEkin = 0.0;
-    t.setE(finalEkin);
-    t.edep = eloss;
-    return true;
   }
   t.setE(finalEkin);
   t.edep = eloss;
-  return false;
+  return stopped;
 }
 
 template <typename R> HD bool CheckDelta(const DevTables<R>& T, Trk<R>& t, R rnd) {  // :408-423

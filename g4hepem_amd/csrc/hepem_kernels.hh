@@ -451,7 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_howfar(DevTables<R> T, uint64_t seed
 }
 
 template <typename R, int KIND>
-__global__ void __launch_bounds__(kBlock) k_perform(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
+__global__ void __launch_bounds__(kBlock, 3) k_perform(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
                                                     hepem_step_out res, const uint32_t* __restrict__ queue,
                                                     uint32_t* __restrict__ counters) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
