@@ -307,10 +307,12 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
 // one block; the round's secondaries are added to *nsecTotal. The `inject`
 // primaries (e-/e+ if injectE) go to the end of their part of the next stack:
 // total[0] = tracks of the next stack, total[1] = first slot of the injected
-// primaries, total[2] = slots of the e-/e+ part.
+// primaries, total[2] = slots of the e-/e+ part. The track count is also
+// written to host memory, followed by `tag` once it is visible there.
 constexpr int kScanB = 1024;  // threads of the one-block scan
 __global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t* bsumG, const uint32_t* __restrict__ bsec, int nb,
-                                                  int inject, int injectE, uint32_t* total, unsigned long long* nsecTotal) {
+                                                  int inject, int injectE, uint32_t* total, unsigned long long* nsecTotal,
+                                                  volatile uint32_t* hostCount, uint32_t tag) {
   __shared__ uint32_t carryE, carryG;
   __shared__ unsigned long long secs;
   if (threadIdx.x == 0) {
@@ -345,6 +347,10 @@ __global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t*
     total[1] = injectE ? carryE : elen + carryG;
     total[2] = elen;
     *nsecTotal += secs;
+    // the host reads the next round's track count while the scatter runs
+    hostCount[0] = carryE + carryG + k;
+    __threadfence_system();
+    hostCount[1] = tag;
   }
 }
 
@@ -439,13 +445,41 @@ struct Arena {
   }
 };
 
-// one pinned host word, freed on every exit path
-struct PinnedU32 {
-  uint32_t* p = nullptr;
-  bool alloc() { return hipHostMalloc(reinterpret_cast<void**>(&p), 2 * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess; }
-  ~PinnedU32() {
+// Two words of pinned, device-mapped, coherent host memory: the next round's
+// track count and the tag of the round that wrote it. Freed on every exit
+// path (after the stream has drained).
+struct HostCount {
+  uint32_t* p = nullptr;  // host address
+  uint32_t* d = nullptr;  // device address of the same words
+  bool alloc() {
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return false;
+    p[0] = 0u;
+    p[1] = 0u;
+    return hipHostGetDevicePointer(reinterpret_cast<void**>(&d), p, 0) == hipSuccess;
+  }
+  ~HostCount() {
     if (p) (void)hipHostFree(p);
   }
+  // Waits until a kernel has published `tag`; false on a stream error, or if
+  // the stream drained without it.
+  bool wait(uint32_t tag, hipStream_t st) const {
+    for (uint64_t spin = 1;; ++spin) {
+      if (__atomic_load_n(&p[1], __ATOMIC_ACQUIRE) == tag) return true;
+      if ((spin & 1023u) == 0u) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e == hipSuccess) return __atomic_load_n(&p[1], __ATOMIC_ACQUIRE) == tag;
+        if (e != hipErrorNotReady) return false;
+      }
+    }
+  }
+  uint32_t count() const { return __atomic_load_n(&p[0], __ATOMIC_ACQUIRE); }
+};
+
+// Synchronises the stream when the run ends, on every path, before the
+// buffers its kernels use are freed.
+struct StreamDrain {
+  hipStream_t s;
+  ~StreamDrain() { (void)hipStreamSynchronize(s); }
 };
 
 bool AllocStack(Arena& A, Stack& s, int cap) {
@@ -520,8 +554,10 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       !A.get(dep, (size_t)span * nbins) || !A.get(leak, (size_t)span) || !A.get(nsec, 1) ||
       !A.get(work, hepem_step_work_bytes(cap)))
     return HEPEM_ERR_ALLOC;
-  PinnedU32 hm;  // next round's track count, read back every round
-  if (!hm.alloc()) return HEPEM_ERR_ALLOC;
+  HostCount hc;  // next round's track count, published by k_scan_sums
+  if (!hc.alloc()) return HEPEM_ERR_ALLOC;
+  StreamDrain drain{st};  // declared last: drains the stream before hc and A are freed
+  uint32_t tag = 0;
   SH_OK(hipMemcpy(dPlanes, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
 
   Slabs g;
@@ -587,8 +623,9 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       // largest stack
       const bool refillNow = refill > 0 && n < refill && next < ns && growRounds >= 0 && sinceInject >= growRounds;
       const int k = refillNow ? ((ns - next) < chunk ? (ns - next) : chunk) : 0;
+      ++tag;
       hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanB), 0, st, bsum, bsum + nblk, bsec, grid, k, su->primary != 2 ? 1 : 0,
-                         total, nsec);
+                         total, nsec, hc.d, tag);
       hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, nsk, offs, offs + cap, bsum, bsum + nblk,
                          total, cap);
       if (k > 0) {
@@ -599,9 +636,10 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
         rounds = 0;
       }
       SH_OK(hipGetLastError());
-      SH_OK(hipMemcpyAsync(hm.p, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-      SH_OK(hipStreamSynchronize(st));
-      const uint32_t m = *hm.p;
+      // the next round is enqueued as soon as its size is known; stream order
+      // keeps it behind this round's scatter
+      if (!hc.wait(tag, st)) return HEPEM_ERR_HIP;
+      const uint32_t m = hc.count();
       r->steps += n;
       r->iterations += 1;
       if (m > (uint32_t)cap) return HEPEM_ERR_CAPACITY;
