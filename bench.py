@@ -167,7 +167,7 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int):
     the oracle's CPU harness on the first `cpu_events` events for the per-slab
     chi2 and the fraction of events identical to the GPU's."""
     from g4hepem_amd._native import slab_setup
-    su = slab_setup(50, [2.3, 5.7], [0, 1], 0, 1.0e4, capacity=1 << 24, events_per_chunk=4096, refill_below=1 << 21)
+    su = slab_setup(50, [2.3, 5.7], [0, 1], 0, 1.0e4, capacity=1 << 23, events_per_chunk=4096, refill_below=1 << 21)
     t0 = time.perf_counter()
     g = ctx.slab_run(su, nevents, seed, per_event=True)
     wall = time.perf_counter() - t0
@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="headline FP64 step only (for profiling)")
     ap.add_argument("--shower-events", type=int, default=10000, help="config-5 events (0: skip)")
     ap.add_argument("--shower-cpu-events", type=int, default=100, help="config-5 events for the CPU harness")
+    ap.add_argument("--shower-only", action="store_true",
+                    help="config-5 slab showers only, without the CPU harness (for profiling); prints their JSON object")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="before the steps, copy a buffer of this many MiB device-to-device "
                          "(reference kernel of known traffic for hardware byte counters)")
@@ -286,6 +288,11 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU visible"
     dev = torch.device("cuda:0")
+    if args.shower_only:
+        tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr
+        c5 = run_config5(Context(tables_slab, HEPEM_FP64), tables_slab, args.seed, args.shower_events, 0)
+        print(json.dumps({"config5_slab_shower": c5}), flush=True)
+        return
     tables2 = Tables(SET2, 0.7)
     n = args.n
     tracks = step_tracks(n, args.seed, len(SET2))
