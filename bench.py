@@ -193,6 +193,49 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int):
     return out
 
 
+def fp32_errors(r64: dict, t64: dict, r32: dict, t32: dict) -> dict:
+    """Config 4's error report: the FP32-table step against the FP64 step on
+    the same tracks and streams. Integer decisions as agreement fractions. A
+    track that drew another number of random numbers (a rejection test that
+    FP32 rounding decided the other way) sampled another final state, so the
+    per-track errors are over the tracks whose winner, secondary count and
+    draw count agree; the totals over all tracks compare the samples
+    statistically."""
+    n = len(r64["winner"])
+    same = (r64["winner"] == r32["winner"]) & (r64["nsec"] == r32["nsec"]) & (t64["rng_ctr"] == t32["rng_ctr"])
+
+    def rel(a, b, m):
+        m = m & (b != 0)
+        if not m.any():
+            return None
+        r = np.abs(a[m] - b[m]) / np.abs(b[m])
+        return {"max": float(r.max()), "mean": float(r.mean())}
+
+    def total(a, b, m=None):
+        m = np.ones(len(a), bool) if m is None else m
+        return float(abs(a[m].sum() - b[m].sum()) / abs(b[m].sum()))
+
+    d64, d32 = t64["dir"].reshape(3, n), t32["dir"].reshape(3, n)
+    ddir = np.abs(d64 - d32).max(axis=0)[same]
+    s64, s32 = r64["sec_ekin"].reshape(2, n)[0], r32["sec_ekin"].reshape(2, n)[0]
+    has64, has32 = r64["nsec"] >= 1, r32["nsec"] >= 1
+    return {
+        "winner_agreement": float((r64["winner"] == r32["winner"]).mean()),
+        "nsec_agreement": float((r64["nsec"] == r32["nsec"]).mean()),
+        "draw_count_agreement": float((t64["rng_ctr"] == t32["rng_ctr"]).mean()),
+        "compared_tracks": int(same.sum()),
+        "true_step": rel(r32["true_step"], r64["true_step"], same),
+        "geom_step": rel(r32["geom_step"], r64["geom_step"], same),
+        "edep": rel(r32["edep"], r64["edep"], same),
+        "post_step_ekin": rel(t32["ekin"], t64["ekin"], same),
+        "post_step_dir_max_abs": float(ddir.max()) if ddir.size else None,
+        "post_step_dir_mean_abs": float(ddir.mean()) if ddir.size else None,
+        "secondary0_ekin": rel(s32, s64, same & has64),
+        "total_edep_rel": total(r32["edep"], r64["edep"]),
+        "total_secondary0_ekin_rel": float(abs(s32[has32].sum() - s64[has64].sum()) / s64[has64].sum()),
+    }
+
+
 def time_lookups(ctx, n, reps, seed=2026):
     import torch
     from g4hepem_amd.tracks import lookup_queries
@@ -343,10 +386,13 @@ def main():
     }
     if not args.no_extras:
         log("FP32-table step")
+        r64, t64 = res.to_numpy(), tout.to_numpy()   # results of the last FP64 step
         ctx32 = Context(tables2, HEPEM_FP32T)
         ms32, run32, _ = time_steps(ctx32, tin, tout, res, work, args.steps, args.warmup, args.seed)
         line["fp32_table_form"] = {"value": n / (ms32 * 1e-3), "unit": "track-steps/s", "ms_per_step": ms32,
-                                   "kernel_ms": run32, "speedup_vs_fp64": ms64 / ms32}
+                                   "kernel_ms": run32, "speedup_vs_fp64": ms64 / ms32,
+                                   "errors_vs_fp64": fp32_errors(r64, t64, res.to_numpy(), tout.to_numpy())}
+        del r64, t64
         log("config-1 lookups")
         tables1 = Tables(SET1, 0.7)
         ctx1 = Context(tables1, HEPEM_FP64)
