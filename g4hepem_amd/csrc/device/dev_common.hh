@@ -222,6 +222,17 @@ struct Rng {
 // which are double in both forms: the interval fraction (x - x1)/(x2 - x1) of an
 // interpolation loses all precision to cancellation when x and x1 are rounded
 // to float near a grid node. In the FP64 form they alias the R arrays.
+// One restricted macroscopic cross-section block (ioni or brem of one
+// couple) of G4HepEmElectronData::fResMacXSecData with its header unpacked, so
+// a lookup reads one descriptor and then the [E, xs, xs''] triples at
+// xsData[start] instead of following the header chain.
+template <typename R>
+struct XsBlock {
+  double minE, maxE;    // first and last grid energy
+  int start, n;         // first triple, number of triples
+  R maxV, logMin, ild;  // largest cross section, log of the first energy, 1/log-spacing
+};
+
 template <typename R>
 struct DevEl {
   const R* lossGrid;    // [N] loss energy grid
@@ -230,7 +241,9 @@ struct DevEl {
   const double* lossData64;
   int N;
   R lossLogMin, lossILD;
+  int rangeMonotone;    // every couple's range column is non-decreasing (enables the bin hint)
   const int* xsStart;   // [couple] start of the ioni block in xsData
+  const XsBlock<R>* xsBlock;  // [2 per couple]: ioni, brem
   const R* xsData;      // restricted macroscopic cross sections
   const double* xsData64;
   const R* nucGrid;     // [nucN]

@@ -96,6 +96,21 @@ HD int FindLowerBin(const double* xd, int num, double x, int step) {  // FindLow
   }
   return mu - 1;
 }
+// FindLowerBin on a non-decreasing table, trying the bins hint-3 .. hint
+// first (five independent loads instead of a chain of dependent ones). On such
+// a table the search returns the largest k <= num-2 with xd[step*k] <= x
+// whatever path it takes, so a bin that satisfies that is the same answer.
+HD int FindLowerBinHint(const double* xd, int num, double x, int step, int hint) {
+  if (hint >= 3 && hint <= num - 2) {
+    const double v0 = xd[step * (hint - 3)], v1 = xd[step * (hint - 2)], v2 = xd[step * (hint - 1)];
+    const double v3 = xd[step * hint], v4 = xd[step * (hint + 1)];
+    if (v3 <= x && (x < v4 || hint == num - 2)) return hint;
+    if (v2 <= x && x < v3) return hint - 1;
+    if (v1 <= x && x < v2) return hint - 2;
+    if (v0 <= x && x < v1) return hint - 3;
+  }
+  return FindLowerBin(xd, num, x, step);
+}
 
 // ---- track state held in registers (G4HepEmTrack, G4HepEmElectronTrack,
 //      G4HepEmGammaTrack, G4HepEmMSCTrackData) ----------------------------------------------
@@ -149,7 +164,9 @@ template <typename R> HD R GetRestDEDX(const DevEl<R>& e, int imc, double ekin, 
   return tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.lossData + e.N * (5 * imc + 2), ekin, lekin, e.lossLogMin, e.lossILD));
 }
 // :504-520; the low-range quadratic branch is an FP64 island in both forms
-template <typename R> HD double GetInvRange(const DevEl<R>& e, int imc, double range) {
+// binHint: the loss-grid bin of an energy whose range is near `range` (at or
+// above it), or -1
+template <typename R> HD double GetInvRange(const DevEl<R>& e, int imc, double range, int binHint = -1) {
   const int N = e.N;
   const double* xd = e.lossData64 + 5 * N * imc;
   const double minRange = xd[0];
@@ -157,19 +174,14 @@ template <typename R> HD double GetInvRange(const DevEl<R>& e, int imc, double r
     const double dum = range / minRange;
     return tmax(0.0, e.lossGrid64[0] * dum * dum);
   }
-  const int i = FindLowerBin(xd, N, range, 2);
+  const int i = e.rangeMonotone ? FindLowerBinHint(xd, N, range, 2, binHint) : FindLowerBin(xd, N, range, 2);
   const R* sd = e.lossData + 5 * N * imc + 4 * N;
   return (double)tmax((R)0.0, Spline(xd[2 * i], xd[2 * (i + 1)], e.lossGrid[i], e.lossGrid[i + 1], sd[i], sd[i + 1], range));
 }
 template <typename R> HD R GetRestMacXSec(const DevEl<R>& e, int imc, double ekin, R lekin, bool isioni) {  // :522-532
-  const int iIoni = e.xsStart[imc];
-  const int nIoni = (int)e.xsData64[iIoni];
-  const int iStart = isioni ? iIoni : iIoni + 3 * nIoni + 5;
-  const double* x64 = e.xsData64 + iStart;
-  const R* x = e.xsData + iStart;
-  const int nData = (int)x64[0];
-  if (ekin < x64[5]) return (R)0.0;
-  return tmax((R)0.0, SplineLog3(nData, x64 + 5, x + 5, ekin, lekin, x[3], x[4]));
+  const XsBlock<R> b = e.xsBlock[2 * imc + (isioni ? 0 : 1)];
+  if (ekin < b.minE) return (R)0.0;
+  return tmax((R)0.0, SplineLog3(b.n, e.xsData64 + b.start, e.xsData + b.start, ekin, lekin, b.logMin, b.ild));
 }
 template <typename R> HD R GetMacXSecNuclear(const DevEl<R>& e, int imat, double ekin, R lekin) {  // :534-541
   if (ekin < e.nucGrid64[0]) return (R)0.0;
@@ -177,22 +189,15 @@ template <typename R> HD R GetMacXSecNuclear(const DevEl<R>& e, int imat, double
 }
 template <typename R> HD R GetRestMacXSecForStepping(const DevEl<R>& e, int imc, double ekin, R lekin, bool isioni) {  // :544-568
   const R log08 = (R)-0.22314355131420971;
-  const int iIoni = e.xsStart[imc];
-  const int nIoni = (int)e.xsData64[iIoni];
-  const int iStart = isioni ? iIoni : iIoni + 3 * nIoni + 5;
-  const double* x64 = e.xsData64 + iStart;
-  const R* x = e.xsData + iStart;
-  const int nData = (int)x64[0];
-  const double minE = x64[5], maxE = x64[1];
-  const R maxV = x[2];
-  if (ekin > maxE) {
+  const XsBlock<R> b = e.xsBlock[2 * imc + (isioni ? 0 : 1)];
+  if (ekin > b.maxE) {
     const double ekinReduced = 0.8 * ekin;
-    if (ekinReduced < maxE) return tmax((R)0.0, maxV);
+    if (ekinReduced < b.maxE) return tmax((R)0.0, b.maxV);
     ekin = ekinReduced;
     lekin += log08;
   }
-  if (ekin < minE) return (R)0.0;
-  return tmax((R)0.0, SplineLog3(nData, x64 + 5, x + 5, ekin, lekin, x[3], x[4]));
+  if (ekin < b.minE) return (R)0.0;
+  return tmax((R)0.0, SplineLog3(b.n, e.xsData64 + b.start, e.xsData + b.start, ekin, lekin, b.logMin, b.ild));
 }
 template <typename R> HD R GetTransportMFP(const DevEl<R>& e, int im, double ekin, R lekin) {  // :576-583
   const R tr = tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.tr1 + 2 * e.N * im, ekin, lekin, e.lossLogMin, e.lossILD));
@@ -871,7 +876,7 @@ template <typename R> struct ElState {
 };
 
 template <typename R>
-HD void ConvertTrueToGeometricLength(const DevTables<R>& T, Msc<R>& m, R ekin, R range, int imc, bool isEl) {  // :602-650
+HD void ConvertTrueToGeometricLength(const DevTables<R>& T, Msc<R>& m, R ekin, R lekin, R range, int imc, bool isEl) {  // :602-650
   m.par1 = (R)-1.;
   m.par2 = (R)0.;
   m.par3 = (R)0.;
@@ -892,7 +897,7 @@ HD void ConvertTrueToGeometricLength(const DevTables<R>& T, Msc<R>& m, R ekin, R
   } else {
     const R rfin = tmax(range - m.trueStep, (R)0.01 * range);
     const DevEl<R>& e = T.el[isEl ? 0 : 1];
-    const double t1 = GetInvRange(e, imc, (double)rfin);
+    const double t1 = GetInvRange(e, imc, (double)rfin, LogBin(lekin, e.lossLogMin, e.lossILD, e.N));
     const int imat = T.mcMat[imc];
     const R lambda1 = GetTransportMFP(e, imat, t1, vlog((R)t1));
     m.par1 = (m.lambtr1 - lambda1) / (m.lambtr1 * m.trueStep);
@@ -977,7 +982,7 @@ template <typename R> HD void HowFarToMSC(const DevTables<R>& T, ElState<R>& el,
     m.active = true;
     m.lambtr1 = GetTransportMFP(e, imat, ekinD, lekin);
     UMSCStepLimit(T, m, ekin, imat, ireg, range, t.safety, t.onBoundary, t.isEl, rng);
-    ConvertTrueToGeometricLength(T, m, ekin, range, imc, t.isEl);
+    ConvertTrueToGeometricLength(T, m, ekin, lekin, range, imc, t.isEl);
     const R mscTrue = m.trueStep;
     if (mscTrue < pStep) {
       t.winner = -2;
@@ -1030,7 +1035,7 @@ template <typename R> HD bool ApplyMeanEnergyLoss(const DevTables<R>& T, ElState
     const double linLim = T.regLinLim[T.mcReg[imc]];
     if (eloss > ekin * linLim) {
       const double postRange = (double)range - (double)pStep;
-      eloss = ekin - GetInvRange(e, imc, postRange);
+      eloss = ekin - GetInvRange(e, imc, postRange, LogBin(lekin, e.lossLogMin, e.lossILD, e.N));
     }
     eloss = tmax(eloss, 0.0);
     stopped = eloss >= ekin;

@@ -87,8 +87,41 @@ template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronD
   E.lossILD = (R)e->fELossEILDelta;
   P.addX(E.lossGrid, E.lossGrid64, e->fELossEnergyGrid, N);
   P.addX(E.lossData, E.lossData64, e->fELossData, (size_t)5 * N * e->fNumMatCuts);
+  // the inverse-range search may start from a bin hint only where the search
+  // result cannot depend on the path: non-decreasing range columns
+  E.rangeMonotone = 1;
+  for (int mc = 0; mc < e->fNumMatCuts && E.rangeMonotone; ++mc) {
+    const double* r = e->fELossData + (size_t)5 * N * mc;
+    for (int k = 0; k + 1 < N; ++k)
+      if (!(r[2 * k] <= r[2 * (k + 1)])) {
+        E.rangeMonotone = 0;
+        break;
+      }
+  }
   P.add(E.xsStart, e->fResMacXSecStartIndexPerMatCut, e->fNumMatCuts);
   P.addX(E.xsData, E.xsData64, e->fResMacXSecData, e->fResMacXSecNumData);
+  {
+    // block header: [0] triples, [1] last energy, [2] largest value, [3] log of
+    // the first energy, [4] 1/log-spacing, then the triples from [5]
+    std::vector<XsBlock<R>> blocks((size_t)2 * e->fNumMatCuts);
+    const double* d = e->fResMacXSecData;
+    for (int mc = 0; mc < e->fNumMatCuts; ++mc) {
+      const int iIoni = e->fResMacXSecStartIndexPerMatCut[mc];
+      const int nIoni = (int)d[iIoni];
+      for (int k = 0; k < 2; ++k) {
+        const int s0 = k == 0 ? iIoni : iIoni + 3 * nIoni + 5;
+        XsBlock<R>& b = blocks[2 * (size_t)mc + k];
+        b.minE = d[s0 + 5];
+        b.maxE = d[s0 + 1];
+        b.start = s0 + 5;
+        b.n = (int)d[s0];
+        b.maxV = (R)d[s0 + 2];
+        b.logMin = (R)d[s0 + 3];
+        b.ild = (R)d[s0 + 4];
+      }
+    }
+    P.add(E.xsBlock, blocks);
+  }
   E.nucN = e->fENucEnergyGridSize;
   E.nucLogMin = (R)e->fENucLogMinEkin;
   E.nucILD = (R)e->fENucEILDelta;

@@ -412,7 +412,7 @@ HD bool ResolveSlot(const hepem_tracks& t, int j, const uint32_t* queue, const u
 }
 
 template <typename R, int KIND>
-__global__ void __launch_bounds__(kBlock) k_howfar(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
+__global__ void __launch_bounds__(kBlock, 4) k_howfar(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
                                                    const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = t.n;
@@ -1041,7 +1041,7 @@ __global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, in
   const R r = GetRestRange(e, mc, E, lE);
   out[i] = (double)GetRestDEDX(e, mc, E, lE);
   out[n + i] = (double)r;
-  out[2 * n + i] = GetInvRange(e, mc, (double)r);
+  out[2 * n + i] = GetInvRange(e, mc, (double)r, LogBin(lE, e.lossLogMin, e.lossILD, e.N));
   out[3 * n + i] = (double)GetRestMacXSec(e, mc, E, lE, true);
   out[4 * n + i] = (double)GetRestMacXSec(e, mc, E, lE, false);
 }
