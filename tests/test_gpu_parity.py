@@ -123,6 +123,23 @@ def test_lookup_parity_100k(tables1, ctx1, oracle, dev):
         assert same_bits(out, ref)
 
 
+def test_lookup_at_grid_nodes(tables1, ctx1, oracle, dev):
+    """Energies on the loss-grid nodes and one ulp either side, where the
+    range lands on (or next to) a bin edge of the inverse-range search: the
+    hinted search returns the binary search's bin, so the lookups stay
+    bit-exact against the oracle."""
+    from g4hepem_amd.data_layout import arr, view
+    ed = view(tables1).fTheElectronData.contents
+    grid = arr(ed.fELossEnergyGrid, ed.fELossEnergyGridSize)
+    e = np.concatenate([grid, np.nextafter(grid, 0.0), np.nextafter(grid, np.inf), 0.5 * (grid[1:] + grid[:-1])])
+    e = np.tile(e, 2)
+    imc = np.repeat(np.arange(2, dtype=np.int32), len(e) // 2)
+    le = np.log(e)
+    for pt in (0, 1):
+        _print(f"lookup at grid nodes ptype={pt}: {len(e)} queries")
+        assert same_bits(gpu_lookup(ctx1, pt, imc, e, le, dev), oracle.lookup(tables1, pt, imc, e, le))
+
+
 def test_lookup_invalid_couple_gives_nan(ctx1, dev):
     out = gpu_lookup(ctx1, 0, np.array([5, -1, 0], np.int32), np.array([1.0, 1.0, 1.0]), np.zeros(3), dev)
     assert np.all(np.isnan(out[:, :2])) and np.all(np.isfinite(out[:, 2]))
