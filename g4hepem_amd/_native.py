@@ -112,6 +112,8 @@ def lib() -> C.CDLL:
     L.hepem_tables_build.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.c_double, C.POINTER(vp), C.POINTER(vp)]
     L.hepem_tables_free.argtypes = [vp, vp]
     L.hepem_tables_serialize.argtypes = [vp, vp, vp, C.POINTER(u64)]
+    L.hepem_tables_from_json.argtypes = [C.c_char_p, u64, C.POINTER(vp), C.POINTER(vp)]
+    L.hepem_tables_to_json.argtypes = [vp, vp, vp, C.POINTER(u64)]
     L.hepem_ctx_create.argtypes = [vp, vp, C.c_int, C.POINTER(vp)]
     L.hepem_ctx_destroy.argtypes = [vp]
     L.hepem_ctx_table_bytes.argtypes = [vp, C.POINTER(u64)]
@@ -138,7 +140,8 @@ def lib() -> C.CDLL:
         getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), vp, vp]
     for name in ("hepem_perform", "hepem_electron_perform", "hepem_gamma_perform"):
         getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), C.POINTER(StepOut), vp, vp]
-    for name in ("hepem_tables_build", "hepem_tables_free", "hepem_tables_serialize", "hepem_ctx_create",
+    for name in ("hepem_tables_build", "hepem_tables_free", "hepem_tables_serialize", "hepem_tables_from_json",
+                 "hepem_tables_to_json", "hepem_ctx_create",
                  "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_step",
                  "hepem_step_prepare", "hepem_step_run", "hepem_step_rejected", "hepem_ctx_set_process_sort",
                  "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",
@@ -156,7 +159,8 @@ def check(where: str, rc: int) -> None:
 
 
 class Tables:
-    """G4HepEmData + G4HepEmParameters built by the offline table builder."""
+    """G4HepEmData + G4HepEmParameters: built by the offline table builder, or
+    read from a G4HepEmState JSON document (Tables.from_json)."""
 
     def __init__(self, materials, prod_cut_mm: float = 0.7):
         self.materials = list(materials)
@@ -166,6 +170,36 @@ class Tables:
         check("hepem_tables_build", lib().hepem_tables_build(names, len(self.materials), self.prod_cut_mm,
                                                               C.byref(d), C.byref(p)))
         self.data, self.pars = d, p
+
+    @classmethod
+    def from_json(cls, text) -> "Tables":
+        """Tables from a G4HepEmState document in the reference's JSON schema
+        (G4HepEmStateToJson); `text` is str, bytes, or a path to such a file.
+        A G4HepEmData document plus a G4HepEmParameters one can be given as a
+        pair (data_text, parameters_text)."""
+        import os
+        if isinstance(text, tuple):
+            data_t, pars_t = (t.encode() if isinstance(t, str) else bytes(t) for t in text)
+            text = b'{"fData":' + data_t + b',"fParameters":' + pars_t + b"}"
+        elif isinstance(text, (str, os.PathLike)) and not str(text).lstrip().startswith("{"):
+            with open(text, "rb") as f:
+                text = f.read()
+        elif isinstance(text, str):
+            text = text.encode()
+        t = cls.__new__(cls)
+        t.materials, t.prod_cut_mm = None, None
+        d, p = C.c_void_p(), C.c_void_p()
+        check("hepem_tables_from_json", lib().hepem_tables_from_json(text, len(text), C.byref(d), C.byref(p)))
+        t.data, t.pars = d, p
+        return t
+
+    def to_json(self) -> str:
+        """The tables as a G4HepEmState document in the reference's JSON schema."""
+        size = C.c_uint64()
+        check("hepem_tables_to_json", lib().hepem_tables_to_json(self.data, self.pars, None, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        check("hepem_tables_to_json", lib().hepem_tables_to_json(self.data, self.pars, buf, C.byref(size)))
+        return buf.raw[:size.value].decode()
 
     def serialize(self) -> bytes:
         size = C.c_uint64()

@@ -39,6 +39,27 @@ int hepem_tables_free(struct G4HepEmData* data, struct G4HepEmParameters* pars);
 int hepem_tables_serialize(const struct G4HepEmData* data, const struct G4HepEmParameters* pars,
                            uint8_t* buf, uint64_t* size);
 
+/* G4HepEmState documents in the reference's JSON schema
+ * (G4HepEm/G4HepEmDataJsonIO/src/G4HepEmDataJsonIOImpl.hh; written by
+ * G4HepEmStateToJson, G4HepEmDataJsonIO.cc:45-53), so tables that Geant4
+ * built elsewhere load unchanged.
+ * hepem_tables_from_json replaces G4HepEmStateFromJson (:56-61): it reads a
+ * document {"fData": ..., "fParameters": ...} of `len` bytes into newly
+ * allocated tables (free them with hepem_tables_free). A document holding
+ * only G4HepEmData (G4HepEmDataToJson) and one holding only the parameters
+ * (G4HepEmParametersToJson) can be joined into that form as the two values.
+ * Returns HEPEM_ERR_INVALID, and allocates nothing, on malformed JSON, a
+ * missing member or array sizes that do not fit together.
+ * hepem_tables_to_json replaces G4HepEmStateToJson: with buf == NULL only
+ * *size is set; otherwise buf holds *size bytes and receives the text (not
+ * NUL-terminated), *size its length; HEPEM_ERR_CAPACITY if it does not fit.
+ * Doubles are written with 17 significant digits, so reading the text back
+ * reproduces every value exactly. */
+int hepem_tables_from_json(const char* text, uint64_t len, struct G4HepEmData** out_data,
+                           struct G4HepEmParameters** out_pars);
+int hepem_tables_to_json(const struct G4HepEmData* data, const struct G4HepEmParameters* pars, char* buf,
+                         uint64_t* size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,6 @@
-"""TEST INFRASTRUCTURE ONLY: ctypes binding of the C oracle (oracle/liboracle.so)
-and of the reference harness (oracle/_ref/libhepem_ref.so).
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the C oracle (oracle/liboracle.so),
+of the reference harness (oracle/_ref/libhepem_ref.so) and of the reference's
+JSON serializer (oracle/_ref/libhepem_json_ref.so).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
 module, as the checker; the product library never calls it.
@@ -14,6 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_LIB = HERE / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libhepem_ref.so"
+REF_JSON_LIB = HERE / "_ref" / "libhepem_json_ref.so"
 
 TRACK_FIELDS = {
     "ptype": ("int8", 1), "id": ("int32", 1), "imc": ("int32", 1), "ekin": ("float64", 1),
@@ -183,3 +185,45 @@ class Ref(_Base):
 
 def ref_available() -> bool:
     return REF_LIB.exists()
+
+
+class RefJson:
+    """The reference's G4HepEmStateToJson / G4HepEmStateFromJson
+    (G4HepEm/G4HepEmDataJsonIO/src/G4HepEmDataJsonIO.cc:45-61) on this
+    project's tables, through oracle/ref/json_harness.cc."""
+
+    def __init__(self):
+        L = C.CDLL(str(REF_JSON_LIB))
+        L.refjson_write_state.argtypes = [P, P, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64)]
+        L.refjson_write_state.restype = C.c_int
+        L.refjson_free_text.argtypes = [C.c_char_p]
+        L.refjson_read_state.argtypes = [C.c_char_p, C.c_uint64]
+        L.refjson_read_state.restype = P
+        L.refjson_state_data.argtypes = [P]
+        L.refjson_state_data.restype = P
+        L.refjson_state_pars.argtypes = [P]
+        L.refjson_state_pars.restype = P
+        L.refjson_free_state.argtypes = [P]
+        self.L = L
+
+    def write_state(self, tables) -> bytes:
+        out, n = C.c_char_p(), C.c_uint64()
+        assert self.L.refjson_write_state(tables.data, tables.pars, C.byref(out), C.byref(n)) == 0
+        text = C.string_at(out, n.value)
+        self.L.refjson_free_text(out)
+        return text
+
+    def read_state(self, text: bytes):
+        """(state handle, data pointer, parameters pointer), or None if the
+        reference rejects the text; free the handle with free_state."""
+        st = self.L.refjson_read_state(text, len(text))
+        if not st:
+            return None
+        return st, self.L.refjson_state_data(st), self.L.refjson_state_pars(st)
+
+    def free_state(self, st) -> None:
+        self.L.refjson_free_state(st)
+
+
+def ref_json_available() -> bool:
+    return REF_JSON_LIB.exists()
