@@ -2,6 +2,7 @@
 // context creation (one device image of every table, FP64 and FP32 copies),
 // and the launch wrappers of the lookup and stepping kernels.
 #include "hepem_kernels.hh"
+#include "hepem_internal.hh"
 
 #include <hip/hip_runtime.h>
 
@@ -329,6 +330,9 @@ struct hepem_ctx {
   uint64_t workBytes = 0;
   int processSort = HEPEM_SORT_AUTO;  // sort pending interactions by process (two stages): HEPEM_SORT_*;
                                       // by default auto in the FP64 form, off in the FP32 form (measured faster)
+  // per couple: e-, e+, gamma production thresholds and the region's apply-cuts flag
+  std::vector<double> cuts;
+  std::vector<uint8_t> applyCuts;
 };
 
 extern "C" const char* hepem_status_string(int s) {
@@ -355,6 +359,19 @@ extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HEPEM_ERR_HIP; }
   c->precision = precision;
   c->processSort = precision == HEPEM_FP64 ? HEPEM_SORT_AUTO : HEPEM_SORT_OFF;
+  {
+    const G4HepEmMatCutData* mc = d->fTheMatCutData;
+    c->cuts.resize(3 * (size_t)mc->fNumMatCutData);
+    c->applyCuts.resize((size_t)mc->fNumMatCutData);
+    for (int i = 0; i < mc->fNumMatCutData; ++i) {
+      const G4HepEmMCCData& m = mc->fMatCutData[i];
+      c->cuts[3 * (size_t)i] = m.fSecElProdCutE;
+      c->cuts[3 * (size_t)i + 1] = m.fSecPosProdCutE;
+      c->cuts[3 * (size_t)i + 2] = m.fSecGamProdCutE;
+      const int r = m.fG4RegionIndex;
+      c->applyCuts[(size_t)i] = (r >= 0 && r < p->fNumRegions && p->fParametersPerRegion[r].fIsApplyCuts) ? 1 : 0;
+    }
+  }
   Packer P;
   PackTables(P, c->t64, d, p);
   PackTables(P, c->t32, d, p);
@@ -367,6 +384,13 @@ extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p
   }
   for (auto& f : P.fix) *f.first = static_cast<const uint8_t*>(c->image) + f.second;
   *out = c;
+  return HEPEM_OK;
+}
+
+int HepemCoupleCuts(const hepem_ctx* c, int imc, double cuts[3], int* apply) {
+  if (!c || !cuts || !apply || imc < 0 || (size_t)imc >= c->applyCuts.size()) return HEPEM_ERR_INVALID;
+  for (int k = 0; k < 3; ++k) cuts[k] = c->cuts[3 * (size_t)imc + k];
+  *apply = c->applyCuts[(size_t)imc];
   return HEPEM_OK;
 }
 

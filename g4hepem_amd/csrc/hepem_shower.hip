@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "g4hepem_amd_shower.h"
+#include "hepem_internal.hh"
 
 namespace {
 
@@ -56,6 +57,10 @@ struct Slabs {
   int nabs;
   int imc[HEPEM_SLAB_MAX_ABS];
   const double* plane;  // nbins + 1 plane positions [mm], plane[0] = 0
+  // per absorber: production thresholds (e-, e+, gamma) of its couple and
+  // whether its region applies them to secondaries
+  double cutE[HEPEM_SLAB_MAX_ABS], cutP[HEPEM_SLAB_MAX_ABS], cutG[HEPEM_SLAB_MAX_ABS];
+  int applyCuts[HEPEM_SLAB_MAX_ABS];
 };
 
 // one stack buffer
@@ -190,14 +195,14 @@ __global__ void k_sh_geometry(Stack s, Slabs g, hepem_howfar_state hs, double* x
 // The next stack holds the e-/e+ first and the gammas after them (so the
 // library's particle-type queue covers runs of consecutive slots), each part
 // in slot order of the parents. keep[i] = 1 if the track itself goes on,
-// nsk[i] = secondaries to stack; offE/offG[i] = exclusive scans within the
+// bit s of smask[i] = secondary slot s is stacked; offE/offG[i] = exclusive scans within the
 // block of the e-/e+ and gamma slots track i needs; bsumE/bsumG/bsec[block] =
 // the block's totals of those and of its secondaries. Deposits are summed per
 // block and (event, slab) before the global atomic; the fixed-point sum does
 // not depend on the grouping.
 __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out res, const double* __restrict__ xend,
                           const int8_t* __restrict__ side, unsigned long long* dep, unsigned long long* leak,
-                          uint8_t* keep, uint8_t* nsk, uint32_t* offE, uint32_t* offG, uint32_t* bsumE, uint32_t* bsumG,
+                          uint8_t* keep, uint8_t* smask, uint32_t* offE, uint32_t* offG, uint32_t* bsumE, uint32_t* bsumG,
                           uint32_t* bsec) {
   const int i = blockIdx.x * kB + threadIdx.x;
   const int n = s.t.n;
@@ -208,12 +213,9 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
   uint32_t cE = 0u, cG = 0u, ns = 0u;
   if (valid) {
     const int ev = s.event[i];
-    int l = s.layer[i];
-    const double edep = res.edep[i];
-    if (edep > 0.0) {
-      key = (unsigned long long)ev * (unsigned long long)g.nbins + (unsigned long long)l;
-      val = (unsigned long long)llrint(edep * kFix);
-    }
+    const int l0 = s.layer[i];  // the pre-step slab: scoring and the couple's cuts
+    int l = l0;
+    double edep = res.edep[i];
     double x = xend[i];
     const int sd = side[i];
     const bool rejected = res.winner[i] == -99;
@@ -252,16 +254,40 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
     s.layer[i] = l;
     // a track that crossed a plane had no interaction, so a track leaving the
     // stack has no secondaries; the guard keeps that an invariant of the stack
-    ns = (rejected || left) ? 0u : (uint32_t)res.nsec[i];
+    const int nsec = (rejected || left) ? 0 : res.nsec[i];
+    // StackSecondaries (G4HepEmTrackingManager.cc:1274-1313): where the
+    // couple's region applies cuts, a secondary below its production
+    // threshold is deposited at the step instead of stacked (an e+ with its
+    // annihilation energy, when the gamma threshold exceeds the electron mass)
+    const int a = l0 % g.nabs;
+    uint32_t mask = 0u;
+    for (int sl = 0; sl < nsec; ++sl) {
+      const int8_t pt = res.sec_ptype[sl * n + i];
+      const double se = res.sec_ekin[sl * n + i];
+      bool cut = false;
+      if (g.applyCuts[a]) {
+        if (pt == 0) cut = se < g.cutE[a];
+        else if (pt == 1) cut = kElectronMass < g.cutG[a] && se < g.cutP[a];
+        else cut = se < g.cutG[a];
+      }
+      if (cut) {
+        edep += pt == 1 ? se + 2.0 * kElectronMass : se;
+      } else {
+        mask |= 1u << sl;
+        ++ns;
+        if (pt == 2) ++cG;
+        else ++cE;
+      }
+    }
     keep[i] = alive ? 1 : 0;
-    nsk[i] = (uint8_t)ns;
+    smask[i] = (uint8_t)mask;
     if (alive) {
       if (isE) ++cE;
       else ++cG;
     }
-    for (uint32_t sl = 0; sl < ns; ++sl) {
-      if (res.sec_ptype[sl * n + i] == 2) ++cG;
-      else ++cE;
+    if (edep > 0.0) {
+      key = (unsigned long long)ev * (unsigned long long)g.nbins + (unsigned long long)l0;
+      val = (unsigned long long)llrint(edep * kFix);
     }
   }
   // per-block aggregation of the deposits in an LDS hash table (open
@@ -354,13 +380,13 @@ __global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t*
   }
 }
 
-// Survivor i and its secondaries go to the next e-/e+ slots from
+// Survivor i and its stacked secondaries go to the next e-/e+ slots from
 // bsumE[block] + offE[i] and the next gamma slots from total[2] +
 // bsumG[block] + offG[i], in the order survivor, secondary 0, secondary 1;
 // each starts with the isotropic safety of its position. Writes nothing beyond
 // `cap` (the host then reports the overflow).
 __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, hepem_step_out res, const uint8_t* __restrict__ keep,
-                             const uint8_t* __restrict__ nsk, const uint32_t* __restrict__ offE,
+                             const uint8_t* __restrict__ smask, const uint32_t* __restrict__ offE,
                              const uint32_t* __restrict__ offG, const uint32_t* __restrict__ bsumE,
                              const uint32_t* __restrict__ bsumG, const uint32_t* __restrict__ total, int cap) {
   const int i = blockIdx.x * kB + threadIdx.x;
@@ -401,9 +427,9 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
     b.event[j] = a.event[i];
     b.nstep[j] = ns1;
   }
-  const int nsec = nsk[i];
+  const uint32_t mask = smask[i];
   for (int sl = 0; sl < 2; ++sl) {
-    if (sl >= nsec) break;
+    if (!((mask >> sl) & 1u)) continue;
     const uint64_t cid = ChildStream(sid, ns1, sl);
     const int8_t pt = res.sec_ptype[sl * n + i];
     const int j = pt == 2 ? jG++ : jE++;
@@ -538,7 +564,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   hepem_step_out res;
   double *dPlanes, *xend;
   int8_t* side;
-  uint8_t *keep, *nsk, *work;
+  uint8_t *keep, *smask, *work;
   uint32_t *offs, *bsum, *bsec, *total;  // offs: e-/e+ then gamma block offsets (2 x cap); bsum: 2 x nblk
   unsigned long long *dep, *leak, *nsec;
   const int nblk = (cap + kB - 1) / kB;
@@ -550,7 +576,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       !A.get(res.displacement, 3 * (size_t)cap) || !A.get(res.winner, cap) || !A.get(res.stopped, cap) ||
       !A.get(res.nsec, cap) || !A.get(res.sec_ptype, 2 * (size_t)cap) || !A.get(res.sec_ekin, 2 * (size_t)cap) ||
       !A.get(res.sec_dir, 6 * (size_t)cap) || !A.get(dPlanes, planes.size()) || !A.get(xend, cap) || !A.get(side, cap) ||
-      !A.get(keep, cap) || !A.get(nsk, cap) || !A.get(offs, 2 * (size_t)cap) || !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) ||
+      !A.get(keep, cap) || !A.get(smask, cap) || !A.get(offs, 2 * (size_t)cap) || !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) ||
       !A.get(dep, (size_t)span * nbins) || !A.get(leak, (size_t)span) || !A.get(nsec, 1) ||
       !A.get(work, hepem_step_work_bytes(cap)))
     return HEPEM_ERR_ALLOC;
@@ -563,7 +589,18 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   Slabs g;
   g.nbins = nbins;
   g.nabs = su->nabs;
-  for (int a = 0; a < HEPEM_SLAB_MAX_ABS; ++a) g.imc[a] = a < su->nabs ? su->imc[a] : 0;
+  for (int a = 0; a < HEPEM_SLAB_MAX_ABS; ++a) {
+    g.imc[a] = a < su->nabs ? su->imc[a] : 0;
+    g.cutE[a] = g.cutP[a] = g.cutG[a] = 0.0;
+    g.applyCuts[a] = 0;
+    if (a < su->nabs) {
+      double cuts[3];
+      if (HepemCoupleCuts(ctx, su->imc[a], cuts, &g.applyCuts[a]) != HEPEM_OK) return HEPEM_ERR_INVALID;
+      g.cutE[a] = cuts[0];
+      g.cutP[a] = cuts[1];
+      g.cutG[a] = cuts[2];
+    }
+  }
   g.plane = dPlanes;
 
   std::vector<Kahan> ks(nbins), ks2(nbins);
@@ -615,7 +652,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       hipLaunchKernelGGL(k_sh_geometry, dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side);
       rc = hepem_perform(ctx, seed, &a.t, &hs, &res, work, stream);
       if (rc != HEPEM_OK) return rc;
-      hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, nsk, offs, offs + cap,
+      hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, smask, offs, offs + cap,
                          bsum, bsum + nblk, bsec);
       // primaries of the next events join once the stack runs low, and no
       // sooner after the last ones than the first events took to reach their
@@ -626,7 +663,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       ++tag;
       hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanB), 0, st, bsum, bsum + nblk, bsec, grid, k, su->primary != 2 ? 1 : 0,
                          total, nsec, hc.d, tag);
-      hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, nsk, offs, offs + cap, bsum, bsum + nblk,
+      hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, smask, offs, offs + cap, bsum, bsum + nblk,
                          total, cap);
       if (k > 0) {
         hipLaunchKernelGGL(k_sh_inject, dim3((k + kB - 1) / kB), dim3(kB), 0, st, b, g, su->primary, su->ekin0,

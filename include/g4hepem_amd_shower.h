@@ -12,9 +12,13 @@
  *   with the reference's safety rule, :527-566) -> score -> stack survivors
  *   and secondaries (secondaries start at the post-step point).
  * Tracks are followed until the physics stops them (tracking cuts) or they
- * leave the stack of slabs. There is no sub-stepping of the along-step part
- * (the reference's Geant4 integration loop, :482-606) and no production-cut
- * filtering of secondaries (the reference's default, :1274-1313).
+ * leave the stack of slabs. Secondaries are stacked as StackSecondaries does
+ * (:1274-1313): where the region of the parent's couple applies cuts
+ * (G4HepEmRegionParmeters::fIsApplyCuts), a secondary below that couple's
+ * production threshold is deposited at the parent's step instead (an e+ with
+ * 2 mc^2 added, when the gamma threshold exceeds the electron mass). There is
+ * no sub-stepping of the along-step part (the reference's Geant4 integration
+ * loop, :482-606).
  *
  * Every track owns the Philox stream selected by a 64-bit id (id, rng_stream)
  * derived from its parent's id, the parent's step number and the secondary

@@ -2090,8 +2090,7 @@ static void sh_event(sh_job* J, int ev) {
       step_one(&SJ, 0);
       J->steps++;
       const int onb = a_onb;
-      /* scoring */
-      if (r_ed > 0.0) J->dep[(size_t)ev * nbins + l] += (int64_t)llrint(r_ed * SH_FIX);
+      const int l0 = l;  /* the pre-step slab: scoring and the couple's cuts */
       double x;
       if (onb) {
         x = sd > 0 ? hi : lo;
@@ -2124,6 +2123,29 @@ static void sh_event(sh_job* J, int ev) {
       const uint64_t sid = ((uint64_t)t.hi << 32) | (uint32_t)t.id;
       const int ns1 = t.nstep + 1;
       const int ns = left ? 0 : r_ns;
+      /* StackSecondaries (G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:1274-1313):
+         where the couple's region applies cuts, a secondary below its production
+         threshold is deposited at the step instead of stacked */
+      int stacked[2] = {1, 1};
+      double ed = r_ed;
+      {
+        const struct G4HepEmMCCData* mcc = &J->d->fTheMatCutData->fMatCutData[t.imc];
+        const int apply = J->p->fParametersPerRegion[mcc->fG4RegionIndex].fIsApplyCuts;
+        for (int s = 0; s < ns; ++s) {
+          int cut = 0;
+          if (apply) {
+            if (r_spt[s] == 0) cut = r_sek[s] < mcc->fSecElProdCutE;
+            else if (r_spt[s] == 1) cut = 0.51099895000 < mcc->fSecGamProdCutE && r_sek[s] < mcc->fSecPosProdCutE;
+            else cut = r_sek[s] < mcc->fSecGamProdCutE;
+          }
+          if (cut) {
+            ed += r_spt[s] == 1 ? r_sek[s] + 2.0 * 0.51099895000 : r_sek[s];
+            stacked[s] = 0;
+          }
+        }
+      }
+      /* scoring */
+      if (ed > 0.0) J->dep[(size_t)ev * nbins + l0] += (int64_t)llrint(ed * SH_FIX);
       /* secondaries: pushed in reverse so that slot 0 is followed first */
       if (top + ns + 1 > cap) {
         cap *= 2;
@@ -2132,6 +2154,7 @@ static void sh_event(sh_job* J, int ev) {
         stk = nstk;
       }
       for (int s = ns - 1; s >= 0; --s) {
+        if (!stacked[s]) continue;
         sh_track c;
         memset(&c, 0, sizeof(c));
         const uint64_t cid = sh_child_stream(sid, ns1, s);

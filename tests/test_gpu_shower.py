@@ -90,3 +90,33 @@ def test_shower_photon_transmission_closed_form(tables1, ctx1, oracle):
                                   np.array([0.5]))
     p = math.exp(-d * xs[0])
     assert abs(zero - p) < 4 * math.sqrt(p * (1 - p) / nev), (zero, p)
+
+
+def test_shower_production_cuts_follow_region_flag(tables1, ctx1, oracle):
+    """Secondaries below the couple's production threshold are deposited, not
+    stacked, exactly where the region applies cuts (StackSecondaries,
+    G4HepEmTrackingManager.cc:1274-1313). The same tables with fIsApplyCuts
+    switched off (through the JSON form) stack every secondary; both settings
+    agree with the CPU harness and conserve every event's energy."""
+    import json
+    from g4hepem_amd import HEPEM_FP64, Context
+    from g4hepem_amd._native import Tables, slab_setup
+    doc = json.loads(tables1.to_json())
+    assert all(r["fIsApplyCuts"] for r in doc["fParameters"]["fParametersPerRegion"])
+    for r in doc["fParameters"]["fParametersPerRegion"]:
+        r["fIsApplyCuts"] = False
+    nocut = Tables.from_json(json.dumps(doc))
+    nev, e0 = 60, 1000.0
+    su = slab_setup(LAYERS, THICK, [0, 1], 0, e0, capacity=1 << 21, events_per_chunk=30)
+    runs = {}
+    for name, tables, ctx in (("cuts", tables1, ctx1), ("no cuts", nocut, Context(nocut, HEPEM_FP64))):
+        _print(f"shower with {name}")
+        g = ctx.slab_run(su, nev, 78, per_event=True)
+        dep, leak, steps, tracks = oracle.slab_shower(tables, su, nev, 78, nthreads=16)
+        bal = (g["edep_event"].sum(1) + g["leak_event"]) / 1e9
+        assert np.abs(bal - e0).max() < 1e-4
+        same = np.all(g["edep_event"] == dep, axis=1) & (g["leak_event"] == leak)
+        _print(f"  tracks {g['tracks']} (cpu {tracks}), identical events {same.mean():.3f}")
+        assert same.mean() > 0.5
+        runs[name] = g
+    assert runs["cuts"]["tracks"] < 0.8 * runs["no cuts"]["tracks"]
