@@ -161,23 +161,57 @@ def time_batched(ctx, batches, seed, reps):
     return ms, shape
 
 
-def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int):
-    """10 GeV e- into 50 x (2.3 mm Pb + 5.7 mm lAr) at 0.7 mm: GPU showers of
-    `nevents` events (wall clock of the whole driver, host syncs included), and
-    the oracle's CPU harness on the first `cpu_events` events for the per-slab
-    chi2 and the fraction of events identical to the GPU's."""
+def _chi2_means(a, b):
+    v = a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b)
+    m = v > 0
+    return {"chi2": float((((a.mean(0) - b.mean(0)) ** 2)[m] / v[m]).sum()), "ndf": int(m.sum())}
+
+
+def _slab_setup(woodcock_min_ekin: float = 0.0):
     from g4hepem_amd._native import slab_setup
-    su = slab_setup(50, [2.3, 5.7], [0, 1], 0, 1.0e4, capacity=1 << 23, events_per_chunk=4096, refill_below=1 << 21)
+    return slab_setup(50, [2.3, 5.7], [0, 1], 0, 1.0e4, capacity=1 << 23, events_per_chunk=4096, refill_below=1 << 21,
+                      woodcock_min_ekin=woodcock_min_ekin)
+
+
+def _timed_showers(ctx, su, nevents: int, seed: int):
+    ctx.slab_run(su, min(64, nevents), seed + 1)  # untimed: first launches of the driver's kernels
     t0 = time.perf_counter()
     g = ctx.slab_run(su, nevents, seed, per_event=True)
     wall = time.perf_counter() - t0
+    return g, wall, {"events": nevents, "wall_s": wall, "events_per_s": nevents / wall,
+                     "track_steps_per_s": g["steps"] / wall, "steps": g["steps"], "tracks": g["tracks"],
+                     "iterations": g["iterations"], "max_in_flight": g["max_in_flight"]}
+
+
+def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=None):
+    """10 GeV e- into 50 x (2.3 mm Pb + 5.7 mm lAr) at 0.7 mm: GPU showers of
+    `nevents` events (wall clock of the whole driver, host syncs included), and
+    the oracle's CPU harness on the first `cpu_events` events for the per-slab
+    chi2 and the fraction of events identical to the GPU's. With `ctx32` (the
+    FP32-table context) the same events in that form, and in both cases the
+    chi2 of its per-slab means against the FP64 run; then the FP64 run again
+    with Woodcock tracking of the gammas above 0.2 MeV."""
+    su = _slab_setup()
+    g, wall, out = _timed_showers(ctx, su, nevents, seed)
     ev = g["edep_event"] / 1e9
     mean, rms = ev.mean(0), ev.std(0)
-    out = {"events": nevents, "wall_s": wall, "events_per_s": nevents / wall,
-           "track_steps_per_s": g["steps"] / wall, "steps": g["steps"], "tracks": g["tracks"],
-           "iterations": g["iterations"], "max_in_flight": g["max_in_flight"],
-           "mean_edep_total_MeV": float(ev.sum(1).mean()), "mean_leak_MeV": float(g["leak_event"].mean() / 1e9),
-           "layer_mean_MeV": [round(float(v), 4) for v in mean], "layer_rms_MeV": [round(float(v), 4) for v in rms]}
+    out.update({"mean_edep_total_MeV": float(ev.sum(1).mean()), "mean_leak_MeV": float(g["leak_event"].mean() / 1e9),
+                "layer_mean_MeV": [round(float(v), 4) for v in mean], "layer_rms_MeV": [round(float(v), 4) for v in rms]})
+    if ctx32 is not None:
+        g32, _, o32 = _timed_showers(ctx32, su, nevents, seed)
+        e32 = g32["edep_event"] / 1e9
+        o32.update({"mean_edep_total_MeV": float(e32.sum(1).mean()), "vs_fp64": _chi2_means(ev, e32),
+                    "speedup_vs_fp64": o32["events_per_s"] / out["events_per_s"],
+                    "layer_mean_MeV": [round(float(v), 4) for v in e32.mean(0)],
+                    "layer_rms_MeV": [round(float(v), 4) for v in e32.std(0)]})
+        out["fp32_table_form"] = o32
+        del g32
+    gw, _, ow = _timed_showers(ctx, _slab_setup(0.2), nevents, seed)
+    ew = gw["edep_event"] / 1e9
+    ow.update({"woodcock_min_ekin_MeV": 0.2, "mean_edep_total_MeV": float(ew.sum(1).mean()),
+               "vs_normal_tracking": _chi2_means(ev, ew), "speedup_vs_normal": ow["events_per_s"] / out["events_per_s"]})
+    out["woodcock_fp64"] = ow
+    del gw
     if cpu_events > 0:
         from oracle.oracle_py import Oracle
         cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -333,7 +367,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.shower_only:
         tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr
-        c5 = run_config5(Context(tables_slab, HEPEM_FP64), tables_slab, args.seed, args.shower_events, 0)
+        c5 = run_config5(Context(tables_slab, HEPEM_FP64), tables_slab, args.seed, args.shower_events, 0,
+                         Context(tables_slab, HEPEM_FP32T))
         print(json.dumps({"config5_slab_shower": c5}), flush=True)
         return
     tables2 = Tables(SET2, 0.7)
@@ -424,7 +459,8 @@ def main():
         log("config-5 slab showers")
         tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr
         line["config5_slab_shower"] = run_config5(Context(tables_slab, HEPEM_FP64), tables_slab, args.seed,
-                                                  args.shower_events, 0 if args.no_cpu else args.shower_cpu_events)
+                                                  args.shower_events, 0 if args.no_cpu else args.shower_cpu_events,
+                                                  Context(tables_slab, HEPEM_FP32T))
     if not args.no_cpu:
         log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(tables2, tracks, args.cpu_seconds, args.seed)

@@ -53,7 +53,7 @@ class SlabSetup(C.Structure):
     """hepem_slab_setup (include/g4hepem_amd_shower.h)."""
     _fields_ = [("nlayers", C.c_int32), ("nabs", C.c_int32), ("thickness", C.c_double * 4), ("imc", C.c_int32 * 4),
                 ("primary", C.c_int32), ("ekin0", C.c_double), ("capacity", C.c_int32), ("events_per_chunk", C.c_int32),
-                ("refill_below", C.c_int32)]
+                ("refill_below", C.c_int32), ("woodcock_min_ekin", C.c_double)]
 
 
 class SlabResult(C.Structure):
@@ -62,13 +62,14 @@ class SlabResult(C.Structure):
 
 
 def slab_setup(nlayers: int, thickness, imc, primary: int, ekin0: float, capacity: int = 1 << 22,
-               events_per_chunk: int = 256, refill_below: int = 0) -> SlabSetup:
+               events_per_chunk: int = 256, refill_below: int = 0, woodcock_min_ekin: float = 0.0) -> SlabSetup:
     s = SlabSetup()
     s.nlayers, s.nabs = nlayers, len(thickness)
     for k, (t, m) in enumerate(zip(thickness, imc)):
         s.thickness[k], s.imc[k] = t, m
     s.primary, s.ekin0, s.capacity, s.events_per_chunk = primary, ekin0, capacity, events_per_chunk
     s.refill_below = refill_below
+    s.woodcock_min_ekin = woodcock_min_ekin
     return s
 
 

@@ -333,6 +333,9 @@ struct hepem_ctx {
   // per couple: e-, e+, gamma production thresholds and the region's apply-cuts flag
   std::vector<double> cuts;
   std::vector<uint8_t> applyCuts;
+  // per couple: G4HepEm material index and its density
+  std::vector<int> coupleMat;
+  std::vector<double> coupleDensity;
 };
 
 extern "C" const char* hepem_status_string(int s) {
@@ -370,6 +373,8 @@ extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p
       c->cuts[3 * (size_t)i + 2] = m.fSecGamProdCutE;
       const int r = m.fG4RegionIndex;
       c->applyCuts[(size_t)i] = (r >= 0 && r < p->fNumRegions && p->fParametersPerRegion[r].fIsApplyCuts) ? 1 : 0;
+      c->coupleMat.push_back(m.fHepEmMatIndex);
+      c->coupleDensity.push_back(d->fTheMaterialData->fMaterialData[m.fHepEmMatIndex].fDensity);
     }
   }
   Packer P;
@@ -384,6 +389,17 @@ extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p
   }
   for (auto& f : P.fix) *f.first = static_cast<const uint8_t*>(c->image) + f.second;
   *out = c;
+  return HEPEM_OK;
+}
+
+int HepemPrecision(const hepem_ctx* c) { return c->precision; }
+const DevTables<double>& HepemTables64(const hepem_ctx* c) { return c->t64; }
+const DevTables<float>& HepemTables32(const hepem_ctx* c) { return c->t32; }
+
+int HepemCoupleMaterial(const hepem_ctx* c, int imc, int* imat, double* density) {
+  if (!c || !imat || !density || imc < 0 || (size_t)imc >= c->coupleMat.size()) return HEPEM_ERR_INVALID;
+  *imat = c->coupleMat[(size_t)imc];
+  *density = c->coupleDensity[(size_t)imc];
   return HEPEM_OK;
 }
 

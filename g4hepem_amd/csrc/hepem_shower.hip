@@ -5,7 +5,10 @@
 //   hepem_howfar   (library C-ABI); each track's isotropic safety (distance
 //                  to the nearer plane) was set when it was stacked
 //   k_sh_geometry  cut the geometrical step at the next plane along x; the
-//                  post-step boundary flag; end point of the straight move
+//                  post-step boundary flag; end point of the straight move;
+//                  with Woodcock tracking on, a gamma above its energy limit
+//                  is moved through the stack to its next interaction point
+//                  or out of the stack instead
 //   hepem_perform  (library C-ABI)
 //   k_sh_post      MSC displacement, slab crossing, scoring, leakage, the
 //                  number of stack slots each track needs next round and
@@ -18,11 +21,14 @@
 //                  after them.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <vector>
 
+#include "device/dev_physics.hh"
 #include "g4hepem_amd_shower.h"
 #include "hepem_internal.hh"
 
@@ -61,7 +67,21 @@ struct Slabs {
   // whether its region applies them to secondaries
   double cutE[HEPEM_SLAB_MAX_ABS], cutP[HEPEM_SLAB_MAX_ABS], cutG[HEPEM_SLAB_MAX_ABS];
   int applyCuts[HEPEM_SLAB_MAX_ABS];
+  // Woodcock tracking: gammas above wdtMin (0: off) through the whole stack,
+  // whose exit planes are 0 and xmax, with the couple wdtImc of its densest
+  // material wdtMat; mat[a] = material of absorber a
+  double wdtMin, xmax;
+  int wdtImc, wdtMat;
+  int mat[HEPEM_SLAB_MAX_ABS];
+  unsigned wdtMask;  // bit a: absorber a is of material wdtMat
 };
+
+static_assert(HEPEM_SLAB_MAX_ABS == 4, "AbsorberCouple selects over four absorbers");
+// g.imc[a] as selects (a per-lane index into the kernel argument would be a
+// memory load in the innermost loop)
+__device__ inline int AbsorberCouple(const Slabs& g, int a) {
+  return a == 0 ? g.imc[0] : (a == 1 ? g.imc[1] : (a == 2 ? g.imc[2] : g.imc[3]));
+}
 
 // one stack buffer
 struct Stack {
@@ -163,12 +183,139 @@ __device__ inline void BlockScan2(uint32_t a, uint32_t b, uint32_t w, uint32_t& 
 // Cuts the proposed geometrical step at the next plane along the pre-step
 // direction; xend = end point of the straight move, side = +1/-1 when the
 // step ends on the upper/lower plane of the slab, 0 inside.
-__global__ void k_sh_geometry(Stack s, Slabs g, hepem_howfar_state hs, double* xend, int8_t* side) {
+// Moves (l, a, lo = plane[l], hi = plane[l + 1]) to the slab holding x:
+// plane[l] <= x < plane[l + 1], clamped to [0, nbins); a = l % nabs. The
+// planes increase, so the walk ends at the one slab a bisection would find;
+// a gamma's delta step crosses few slabs, so it reads few planes.
+__device__ inline void WalkToSlab(const Slabs& g, double x, int& l, int& a, double& lo, double& hi) {
+  while (true) {
+    if (l < g.nbins - 1 && x >= hi) {
+      ++l;
+      a = a + 1 == g.nabs ? 0 : a + 1;
+      lo = hi;
+      hi = g.plane[l + 1];
+    } else if (l > 0 && x < lo) {
+      --l;
+      a = a == 0 ? g.nabs - 1 : a - 1;
+      hi = lo;
+      lo = g.plane[l];
+    } else {
+      break;
+    }
+  }
+}
+
+constexpr int kWdtMaxIter = 1 << 20;  // delta steps of one Woodcock step (then: taken to the exit)
+
+// Woodcock tracking of gamma slot i through the stack
+// (G4HepEmWoodcockHelper::KeepTracking, G4HepEm/G4HepEm/src/G4HepEmWoodcockHelper.cc:145-300,
+// and its driver, G4HepEmTrackingManager.cc:955-1062). The distance to the
+// stack's exit less 1e-3 mm is eaten up in steps sampled with the densest
+// material's total cross section Sigma_w; a point in another material is an
+// interaction with probability Sigma/Sigma_w, one in the densest material
+// always. On an interaction the slot moves there with the couple, total mean
+// free path and PE cross section of the point; on reaching the exit it ends
+// on the stack's boundary. Either way the number of interaction lengths left
+// is reset and the step adds nothing to it. Returns false (nothing written) if
+// the gamma is within 1e-3 + 1e-6 mm of the exit (FindWDTVolume, :141-142).
+template <typename R>
+__device__ bool WoodcockStep(const hepem::DevTables<R>& T, uint64_t seed, Stack& s, const Slabs& g,
+                             hepem_howfar_state& hs, double* xend, int8_t* side, int i, int l, double x, double ux) {
+  using namespace hepem;
+  const double distOut = ux > 0.0 ? (g.xmax - x) / ux : (ux < 0.0 ? (0.0 - x) / ux : 1.0e300);
+  double dist = distOut - 1.0E-3 > 0.0 ? distOut - 1.0E-3 : 0.0;
+  if (!(dist >= 1.0E-6)) return false;
+  constexpr R kBig = std::numeric_limits<R>::max();
+  const hepem_tracks& t = s.t;
+  Rng rng;
+  rng.init(seed, (uint32_t)t.id[i], t.rng_stream ? t.rng_stream[i] : 0u, t.rng_ctr[i], t.rng_has_gauss[i] != 0,
+           t.rng_gauss[i]);
+  Trk<R> gt;
+  gt.isEl = false;
+  gt.setE(t.ekin[i]);
+  gt.imc = g.wdtImc;
+  R wpe = (R)0.0;
+  const R wmx = GetTotalMacXSec(T, gt, wpe);
+  const R wmfp = wmx > (R)0.0 ? (R)1.0 / wmx : kBig;
+  // total and PE cross sections of the other absorbers' couples at this energy
+  // (the energy stays the same through the step), ahead of the loop so that
+  // its lanes do not diverge over them
+  R mxa[HEPEM_SLAB_MAX_ABS], pea[HEPEM_SLAB_MAX_ABS];
+#pragma unroll
+  for (int k = 0; k < HEPEM_SLAB_MAX_ABS; ++k) {
+    mxa[k] = pea[k] = (R)0.0;
+    if (k < g.nabs && !((g.wdtMask >> k) & 1u)) {
+      gt.imc = g.imc[k];
+      mxa[k] = GetTotalMacXSec(T, gt, pea[k]);
+    }
+  }
+  R mfp = wmfp, pint = wpe;
+  int lp = l, ap = l % g.nabs, imcp = g.wdtImc;
+  double plo = g.plane[l], phi = g.plane[l + 1];
+  bool boundary = false;
+  double L = 0.0;
+  for (int it = 0;; ++it) {
+    if (it >= kWdtMaxIter) {
+      L += dist;
+      boundary = true;
+      break;
+    }
+    const R pstep = wmfp < kBig ? -vlog((R)rng.flat()) * wmfp : kBig;
+    if (dist < (double)pstep) {
+      L += dist;
+      boundary = true;
+      break;
+    }
+    L += (double)pstep;
+    dist -= (double)pstep;
+    WalkToSlab(g, x + L * ux, lp, ap, plo, phi);
+    if (!((g.wdtMask >> ap) & 1u)) {
+      const R mx = ap == 0 ? mxa[0] : (ap == 1 ? mxa[1] : (ap == 2 ? mxa[2] : mxa[3]));
+      if ((double)(mx * wmfp) > rng.flat()) {
+        imcp = AbsorberCouple(g, ap);
+        mfp = mx > (R)0.0 ? (R)1.0 / mx : kBig;
+        pint = ap == 0 ? pea[0] : (ap == 1 ? pea[1] : (ap == 2 ? pea[2] : pea[3]));
+        break;
+      }
+    } else {
+      imcp = g.wdtImc;
+      mfp = wmfp;
+      pint = wpe;
+      break;
+    }
+  }
+  t.rng_ctr[i] = rng.ctr;
+  t.num_ia_left[i] = -1.0;
+  hs.geom_step[i] = 0.0;
+  if (boundary) {
+    const int sd = ux < 0.0 ? -1 : 1;
+    s.layer[i] = sd > 0 ? g.nbins - 1 : 0;
+    t.on_boundary[i] = 1;
+    side[i] = (int8_t)sd;
+    xend[i] = sd > 0 ? g.xmax : 0.0;
+  } else {
+    s.layer[i] = lp;
+    t.imc[i] = imcp;
+    hs.mfp[i] = (double)mfp;
+    hs.pe_xsec[i] = (double)pint;
+    t.on_boundary[i] = 0;
+    side[i] = 0;
+    xend[i] = x + L * ux;
+  }
+  return true;
+}
+
+template <typename R, bool kWoodcock>
+__global__ void k_sh_geometry(Stack s, Slabs g, hepem_howfar_state hs, double* xend, int8_t* side,
+                              hepem::DevTables<R> T, uint64_t seed) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = s.t.n;
   if (i >= n) return;
   const int l = s.layer[i];
   const double x = s.x[i], ux = s.t.dir[i];
+  if (kWoodcock && s.t.ptype[i] == 2 && hs.winner[i] != -99 && s.t.ekin[i] > g.wdtMin &&
+      WoodcockStep(T, seed, s, g, hs, xend, side, i, l, x, ux))
+    return;
   const double gs = hs.geom_step[i];
   double dist = 1.0e300;
   int sd = 0;
@@ -535,7 +682,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
                               hepem_slab_result* r, void* stream) {
   if (!ctx || !su || !r || !r->edep_sum || !r->edep_sum2 || nevents < 0) return HEPEM_ERR_INVALID;
   if (su->nlayers < 1 || su->nabs < 1 || su->nabs > HEPEM_SLAB_MAX_ABS || su->capacity < 1 || su->events_per_chunk < 1 ||
-      !(su->ekin0 > 0.0) || su->primary < 0 || su->primary > 2 || su->refill_below < 0)
+      !(su->ekin0 > 0.0) || su->primary < 0 || su->primary > 2 || su->refill_below < 0 || !(su->woodcock_min_ekin >= 0.0))
     return HEPEM_ERR_INVALID;
   for (int a = 0; a < su->nabs; ++a)
     if (!(su->thickness[a] > 0.0) || su->imc[a] < 0) return HEPEM_ERR_INVALID;
@@ -602,6 +749,32 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
     }
   }
   g.plane = dPlanes;
+  // Woodcock tracking: the stack is one envelope whose densest material (the
+  // first absorber of the largest density, G4HepEmWoodcockHelper::FindWDTMaterial,
+  // G4HepEmWoodcockHelper.cc:318-338) sets the majorant cross section
+  g.wdtMin = su->woodcock_min_ekin;
+  g.xmax = planes[(size_t)nbins];
+  g.wdtImc = su->imc[0];
+  g.wdtMat = -1;
+  {
+    double maxDensity = -1.0;
+    for (int a = 0; a < HEPEM_SLAB_MAX_ABS; ++a) {
+      g.mat[a] = -1;
+      if (a >= su->nabs) continue;
+      double density;
+      if (HepemCoupleMaterial(ctx, su->imc[a], &g.mat[a], &density) != HEPEM_OK) return HEPEM_ERR_INVALID;
+      if (density > maxDensity) {
+        maxDensity = density;
+        g.wdtMat = g.mat[a];
+        g.wdtImc = su->imc[a];
+      }
+    }
+    g.wdtMask = 0u;
+    for (int a = 0; a < su->nabs; ++a)
+      if (g.mat[a] == g.wdtMat) g.wdtMask |= 1u << a;
+  }
+  const bool woodcock = g.wdtMin > 0.0;
+  const bool fp64 = HepemPrecision(ctx) == HEPEM_FP64;
 
   std::vector<Kahan> ks(nbins), ks2(nbins);
   Kahan kleak, ktot;
@@ -649,7 +822,15 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       const int grid = (n + kB - 1) / kB;
       int rc = hepem_howfar(ctx, seed, &a.t, &hs, work, stream);
       if (rc != HEPEM_OK) return rc;
-      hipLaunchKernelGGL(k_sh_geometry, dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side);
+      if (woodcock && !fp64)
+        hipLaunchKernelGGL((k_sh_geometry<float, true>), dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side,
+                           HepemTables32(ctx), seed);
+      else if (woodcock)
+        hipLaunchKernelGGL((k_sh_geometry<double, true>), dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side,
+                           HepemTables64(ctx), seed);
+      else
+        hipLaunchKernelGGL((k_sh_geometry<double, false>), dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side,
+                           HepemTables64(ctx), seed);
       rc = hepem_perform(ctx, seed, &a.t, &hs, &res, work, stream);
       if (rc != HEPEM_OK) return rc;
       hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, smask, offs, offs + cap,

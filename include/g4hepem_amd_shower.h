@@ -19,6 +19,14 @@
  * 2 mc^2 added, when the gamma threshold exceeds the electron mass). There is
  * no sub-stepping of the along-step part (the reference's Geant4 integration
  * loop, :482-606).
+ * With woodcock_min_ekin > 0, a gamma above that energy is tracked through the
+ * stack as G4HepEmWoodcockHelper::KeepTracking does
+ * (G4HepEm/G4HepEm/src/G4HepEmWoodcockHelper.cc:145-300, driven at
+ * G4HepEmTrackingManager.cc:955-1062): steps of the densest absorber's mean
+ * free path, each point accepted as an interaction with probability
+ * Sigma(material there) / Sigma(densest), until an interaction or the stack's
+ * exit (less 1e-3 mm) is reached; the interaction then happens in the slab
+ * where it lies, with that slab's couple.
  *
  * Every track owns the Philox stream selected by a 64-bit id (id, rng_stream)
  * derived from its parent's id, the parent's step number and the secondary
@@ -53,6 +61,9 @@ typedef struct hepem_slab_setup {
                                            after the previous ones than the first events
                                            took to reach their largest stack
                                            (0: only once the stack is empty)               */
+  double  woodcock_min_ekin;            /* gammas above this kinetic energy [MeV] are
+                                           Woodcock-tracked through the whole stack (the
+                                           reference's default limit is 0.2); 0: off       */
 } hepem_slab_setup;
 
 typedef struct hepem_slab_result {

@@ -17,6 +17,7 @@
  */
 #include "hepem_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -1802,6 +1803,12 @@ typedef struct step_job {
   hepem_step_out* res;
   const double* geom_limit;
   int begin, end;
+  /* optional transport of a gamma between HowFar and Perform in place of
+     geom_limit (the slab shower's Woodcock tracking): returns 1 if it set the
+     step (gstep, on_boundary and, for an interaction, couple, mfp and PE
+     cross section), 0 to fall back to geom_limit */
+  int (*gamma_transport)(const struct step_job* J, int i, otl* tl, void* user);
+  void* user;
 } step_job;
 
 /* The caller's geometry between HowFar and Perform (G4HepEmTrackingManager.cc:
@@ -1899,7 +1906,7 @@ static void step_one(const step_job* J, int i) {
     /* G4HepEmTrackingManager.cc:999-1000,1092-1108: HowFar, (no geometry),
        then UpdateNumIALeft on a boundary or SelectInteraction + Perform */
     GammaHowFar(d, &tl);
-    apply_geometry(J, i, t);
+    if (!(J->gamma_transport && J->gamma_transport(J, i, &tl, J->user))) apply_geometry(J, i, t);
     if (t->on_boundary) {
       t->nial[0] -= t->gstep / t->mfp[0];
       t->edep = 0.0;
@@ -1994,6 +2001,7 @@ int oracle_step_geom(const struct G4HepEmData* d, const struct G4HepEmParameters
   for (int k = 0; k < nthreads; ++k) {
     jobs[k].d = d; jobs[k].p = p; jobs[k].seed = seed;
     jobs[k].in = in; jobs[k].out = out; jobs[k].res = res; jobs[k].geom_limit = geom_limit;
+    jobs[k].gamma_transport = NULL; jobs[k].user = NULL;
     jobs[k].begin = k * chunk < n ? k * chunk : n;
     jobs[k].end = (k + 1) * chunk < n ? (k + 1) * chunk : n;
   }
@@ -2032,9 +2040,91 @@ typedef struct sh_track {
   double x; int layer, nstep;
 } sh_track;
 
+/* Woodcock tracking of a gamma through the whole stack of slabs
+   (G4HepEm/G4HepEm/src/G4HepEmWoodcockHelper.cc:145-300, driven at
+   G4HepEmTrackingManager.cc:955-1062): steps of the densest absorber's mean
+   free path until a point is accepted as an interaction (always in the densest
+   material, else with probability Sigma/Sigma_max) or the exit of the stack,
+   less 1e-3 mm, is reached. The event loop sets the pre-step point; the hook
+   reports where the step ended. */
+typedef struct sh_wdt {
+  const double* plane; int nbins, nabs; const int* imc; int mat[HEPEM_SLAB_MAX_ABS];
+  int wdt_imc, wdt_mat; double wdt_min, xmax;
+  double x; int layer;                          /* pre-step point */
+  int done, boundary, layer_out; double x_out;  /* result of the last step */
+} sh_wdt;
+
+static int sh_locate(const double* plane, int nbins, double x) {
+  int lo = 0, hi = nbins;  /* plane[lo] <= x < plane[hi] */
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (x < plane[mid]) hi = mid; else lo = mid;
+  }
+  return lo;
+}
+
+static int sh_woodcock(const step_job* J, int i, otl* tl, void* user) {
+  (void)i;
+  sh_wdt* W = (sh_wdt*)user;
+  otrack* t = &tl->gm.t;
+  W->done = 0;
+  if (!(W->wdt_min > 0.0) || !(t->ekin > W->wdt_min)) return 0;
+  const double ux = t->dir[0];
+  const double distOut = ux > 0.0 ? (W->xmax - W->x) / ux : (ux < 0.0 ? (0.0 - W->x) / ux : 1.0e300);
+  double dist = distOut - 1.0E-3 > 0.0 ? distOut - 1.0E-3 : 0.0;
+  if (!(dist >= 1.0E-6)) return 0;
+  t->imc = W->wdt_imc;
+  tl->gm.pe_mxsec = 0.0;
+  const double wmx = GetTotalMacXSec(J->d, &tl->gm);
+  const double wmfp = wmx > 0.0 ? 1.0 / wmx : DBL_MAX;
+  const double wpe = tl->gm.pe_mxsec;
+  double L = 0.0, mx = 0.0, mpe = 0.0, mfp = wmfp, pint = wpe;
+  int prev = -1, lp = W->layer, imcp = W->wdt_imc, boundary = 0;
+  for (int it = 0;; ++it) {
+    if (it >= (1 << 20)) { L += dist; boundary = 1; break; }
+    const double pstep = wmfp < DBL_MAX ? -HLog(flat(tl->rng)) * wmfp : DBL_MAX;
+    if (dist < pstep) { L += dist; boundary = 1; break; }
+    L += pstep;
+    dist -= pstep;
+    const double xp = W->x + L * ux;
+    lp = sh_locate(W->plane, W->nbins, xp);
+    const int a = lp % W->nabs;
+    if (W->mat[a] != W->wdt_mat) {
+      if (W->imc[a] != prev) {
+        prev = W->imc[a];
+        t->imc = prev;
+        tl->gm.pe_mxsec = 0.0;
+        mx = GetTotalMacXSec(J->d, &tl->gm);
+        mpe = tl->gm.pe_mxsec;
+      }
+      if (mx * wmfp > flat(tl->rng)) { imcp = W->imc[a]; mfp = mx > 0.0 ? 1.0 / mx : DBL_MAX; pint = mpe; break; }
+    } else {
+      /* in the densest material: an interaction for sure, with its couple */
+      imcp = W->wdt_imc; mfp = wmfp; pint = wpe;
+      break;
+    }
+  }
+  t->nial[0] = -1.0;
+  t->gstep = 0.0;
+  W->done = 1;
+  W->boundary = boundary;
+  if (boundary) {
+    t->on_boundary = 1;
+  } else {
+    t->on_boundary = 0;
+    t->imc = imcp;
+    t->mfp[0] = mfp;
+    tl->gm.pe_mxsec = pint;
+    W->layer_out = lp;
+    W->x_out = W->x + L * ux;
+  }
+  return 1;
+}
+
 typedef struct sh_job {
   const struct G4HepEmData* d; const struct G4HepEmParameters* p; uint64_t seed;
   const hepem_slab_setup* su; const double* plane; int nbins;
+  sh_wdt wdt;          /* Woodcock set-up (per-step fields are per event) */
   int e_begin, e_end;
   int64_t* dep;        /* [nevents][nbins] fixed point */
   int64_t* leak;       /* [nevents] */
@@ -2067,9 +2157,11 @@ static void sh_event(sh_job* J, int ev) {
                     &a_ctr, &a_g, &a_hg, &a_hi};
   hepem_step_out R = {&r_ts, &r_gs, &r_ed, r_disp, &r_w, &r_st, &r_ns, r_spt, r_sek, r_sdir};
   double glim;
+  sh_wdt W = J->wdt;
   step_job SJ;
   SJ.d = J->d; SJ.p = J->p; SJ.seed = J->seed; SJ.in = &T; SJ.out = &T; SJ.res = &R; SJ.geom_limit = &glim;
   SJ.begin = 0; SJ.end = 1;
+  SJ.gamma_transport = W.wdt_min > 0.0 ? sh_woodcock : NULL; SJ.user = &W;
   while (top > 0) {
     sh_track t = stk[--top];
     for (;;) {
@@ -2087,13 +2179,26 @@ static void sh_event(sh_job* J, int ev) {
       glim = 1.0e300;
       if (ux > 0.0) { glim = (hi - t.x) / ux; sd = 1; }
       else if (ux < 0.0) { glim = (lo - t.x) / ux; sd = -1; }
+      W.x = t.x; W.layer = l; W.done = 0;
       step_one(&SJ, 0);
       J->steps++;
       const int onb = a_onb;
-      const int l0 = l;  /* the pre-step slab: scoring and the couple's cuts */
       double x;
-      if (onb) {
-        x = sd > 0 ? hi : lo;
+      if (W.done) {
+        /* Woodcock step: the pre-step point moves to where the step ended, an
+           interaction point or the stack's last slab before its exit */
+        if (W.boundary) {
+          sd = ux > 0.0 ? 1 : (ux < 0.0 ? -1 : 1);
+          l = sd > 0 ? nbins - 1 : 0;
+        } else {
+          l = W.layer_out;
+        }
+      }
+      const int l0 = l;  /* the pre-step slab: scoring and the couple's cuts */
+      if (W.done && !W.boundary) {
+        x = W.x_out;
+      } else if (onb) {
+        x = sd > 0 ? P[l + 1] : P[l];
       } else {
         x = t.x + r_gs * ux;
         x = x < lo ? lo : (x > hi ? hi : x);
@@ -2129,7 +2234,7 @@ static void sh_event(sh_job* J, int ev) {
       int stacked[2] = {1, 1};
       double ed = r_ed;
       {
-        const struct G4HepEmMCCData* mcc = &J->d->fTheMatCutData->fMatCutData[t.imc];
+        const struct G4HepEmMCCData* mcc = &J->d->fTheMatCutData->fMatCutData[J->su->imc[l0 % nabs]];
         const int apply = J->p->fParametersPerRegion[mcc->fG4RegionIndex].fIsApplyCuts;
         for (int s = 0; s < ns; ++s) {
           int cut = 0;
@@ -2195,6 +2300,21 @@ int oracle_slab_shower(const struct G4HepEmData* d, const struct G4HepEmParamete
   for (int k = 0; k < nbins; ++k) plane[k + 1] = plane[k] + su->thickness[k % su->nabs];
   memset(dep, 0, (size_t)nevents * nbins * sizeof(int64_t));
   memset(leak, 0, (size_t)nevents * sizeof(int64_t));
+  /* Woodcock set-up: the densest material of the stack, first absorber on a tie
+     (G4HepEmWoodcockHelper::FindWDTMaterial, G4HepEmWoodcockHelper.cc:318-338) */
+  sh_wdt W;
+  memset(&W, 0, sizeof(W));
+  W.plane = plane; W.nbins = nbins; W.nabs = su->nabs; W.imc = su->imc;
+  W.wdt_min = su->woodcock_min_ekin; W.xmax = plane[nbins]; W.wdt_imc = su->imc[0]; W.wdt_mat = -1;
+  {
+    double maxDensity = -1.0;
+    for (int a = 0; a < su->nabs; ++a) {
+      const int imat = d->fTheMatCutData->fMatCutData[su->imc[a]].fHepEmMatIndex;
+      const double density = d->fTheMaterialData->fMaterialData[imat].fDensity;
+      W.mat[a] = imat;
+      if (density > maxDensity) { maxDensity = density; W.wdt_mat = imat; W.wdt_imc = su->imc[a]; }
+    }
+  }
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
   if (nthreads > nevents) nthreads = nevents > 0 ? nevents : 1;
@@ -2202,7 +2322,7 @@ int oracle_slab_shower(const struct G4HepEmData* d, const struct G4HepEmParamete
   pthread_t th[256];
   for (int k = 0; k < nthreads; ++k) {
     sh_job* J = &jobs[k];
-    J->d = d; J->p = p; J->seed = seed; J->su = su; J->plane = plane; J->nbins = nbins;
+    J->d = d; J->p = p; J->seed = seed; J->su = su; J->plane = plane; J->nbins = nbins; J->wdt = W;
     J->e_begin = (int)((int64_t)nevents * k / nthreads);
     J->e_end = (int)((int64_t)nevents * (k + 1) / nthreads);
     J->dep = dep; J->leak = leak; J->steps = 0; J->tracks = 0; J->status = 0;

@@ -120,3 +120,70 @@ def test_shower_production_cuts_follow_region_flag(tables1, ctx1, oracle):
         assert same.mean() > 0.5
         runs[name] = g
     assert runs["cuts"]["tracks"] < 0.8 * runs["no cuts"]["tracks"]
+
+
+def test_shower_woodcock_matches_cpu_harness(tables1, ctx1, oracle):
+    """Woodcock tracking of the gammas above 0.2 MeV through the whole stack
+    (G4HepEmWoodcockHelper.cc:145-300): the device driver against the CPU
+    harness, which runs the same delta steps between its HowFar and Perform,
+    event by event."""
+    from g4hepem_amd._native import slab_setup
+    nev, e0 = 200, 1000.0
+    su = slab_setup(LAYERS, THICK, [0, 1], 0, e0, capacity=1 << 21, events_per_chunk=64, woodcock_min_ekin=0.2)
+    _print("shower 1 GeV e-, Woodcock gammas: gpu")
+    g = ctx1.slab_run(su, nev, 79, per_event=True)
+    _print(f"  steps {g['steps']} tracks {g['tracks']} iterations {g['iterations']}")
+    _print("shower 1 GeV e-, Woodcock gammas: cpu harness")
+    dep, leak, steps, tracks = oracle.slab_shower(tables1, su, nev, 79, nthreads=16)
+    for d, lk in ((g["edep_event"], g["leak_event"]), (dep, leak)):
+        assert np.abs((d.sum(1) + lk) / 1e9 - e0).max() < 1e-4
+    same = np.all(g["edep_event"] == dep, axis=1) & (g["leak_event"] == leak)
+    _print(f"  events identical to the CPU harness: {same.mean():.3f}")
+    assert same.mean() > 0.5
+    chi2, ndf = chi2_means(g["edep_event"] / 1e9, dep / 1e9)
+    _print(f"  chi2/ndf of per-slab means: {chi2:.1f}/{ndf}")
+    assert chi2 < ndf + 5 * np.sqrt(2 * ndf)
+    assert abs(steps - g["steps"]) < 0.02 * steps
+
+
+def test_shower_woodcock_transmission_closed_form(tables1, ctx1, oracle):
+    """1 MeV photons through 5.7 mm lAr then 2 mm Pb with Woodcock tracking:
+    the fraction of events without any deposit is
+    exp(-(5.7 Sigma_lAr + (2 - 1e-3) Sigma_Pb)) (the gamma leaves the
+    Woodcock envelope 1e-3 mm before its exit)."""
+    import math
+    from g4hepem_amd._native import slab_setup
+    e0, nev = 1.0, 50000
+    su = slab_setup(1, (5.7, 2.0), (1, 0), 2, e0, capacity=1 << 20, events_per_chunk=50000, woodcock_min_ekin=0.2)
+    g = ctx1.slab_run(su, nev, 10, per_event=True)
+    zero = (g["edep_event"].sum(1) == 0).mean()
+    xs, _, _ = oracle.probe_gamma(tables1, np.array([1, 0], np.int32), np.array([e0, e0]),
+                                  np.array([math.log(e0)] * 2), np.array([0.5, 0.5]))
+    p = math.exp(-(5.7 * xs[0] + (2.0 - 1.0e-3) * xs[1]))
+    assert abs(zero - p) < 4 * math.sqrt(p * (1 - p) / nev), (zero, p)
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32t"])
+def test_shower_woodcock_agrees_with_normal_tracking(tables1, ctx1, form):
+    """1 GeV e- showers: with Woodcock tracking of the gammas the per-slab
+    mean deposits agree with normal tracking (chi2), every event conserves
+    energy, fewer steps are taken, and the run does not depend on chunking."""
+    from g4hepem_amd import HEPEM_FP32T, Context
+    from g4hepem_amd._native import slab_setup
+    ctx = ctx1 if form == "fp64" else Context(tables1, HEPEM_FP32T)
+    nev, e0 = 1000, 1000.0
+    runs = {}
+    for w in (0.0, 0.2):
+        su = slab_setup(LAYERS, THICK, [0, 1], 0, e0, capacity=1 << 22, events_per_chunk=250, woodcock_min_ekin=w)
+        g = ctx.slab_run(su, nev, 80, per_event=True)
+        assert np.abs((g["edep_event"].sum(1) + g["leak_event"]) / 1e9 - e0).max() < 1e-4
+        runs[w] = g
+        _print(f"  {form} woodcock {w}: steps {g['steps']}")
+    chi2, ndf = chi2_means(runs[0.0]["edep_event"] / 1e9, runs[0.2]["edep_event"] / 1e9)
+    _print(f"  chi2/ndf of per-slab means: {chi2:.1f}/{ndf}")
+    assert chi2 < ndf + 5 * np.sqrt(2 * ndf)
+    assert runs[0.2]["steps"] < 0.8 * runs[0.0]["steps"]
+    su = slab_setup(LAYERS, THICK, [0, 1], 0, e0, capacity=1 << 22, events_per_chunk=7, refill_below=3000,
+                    woodcock_min_ekin=0.2)
+    c = ctx.slab_run(su, 100, 80, per_event=True)
+    assert np.array_equal(c["edep_event"], runs[0.2]["edep_event"][:100])
