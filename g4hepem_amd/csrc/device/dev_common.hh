@@ -111,6 +111,19 @@ HD double dsin(double x) { return sin(x); }
 HD float dsin(float x) { return __sinf(x); }
 HD double dcos(double x) { return cos(x); }
 HD float dcos(float x) { return __cosf(x); }
+// sine and cosine of one angle: one argument reduction on the device
+HD void dsincos(double x, double& s, double& c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  sincos(x, &s, &c);
+#else
+  s = sin(x);
+  c = cos(x);
+#endif
+}
+HD void dsincos(float x, float& s, float& c) {
+  s = __sinf(x);
+  c = __cosf(x);
+}
 HD double dabs(double x) { return __builtin_fabs(x); }
 HD float dabs(float x) { return __builtin_fabsf(x); }
 

@@ -373,8 +373,10 @@ template <typename R> HD void UMSCSampleDisplacement(R pStep, R thePhi, Msc<R>& 
   rng.flatArray(2, rr);
   const R psi = -vlog((R)1. - rr[0] * cbeta1) / cbeta;
   const R phi = (rr[1] < (R)0.5) ? thePhi + psi : thePhi - psi;
-  m.displ[0] = r * dcos(phi);
-  m.displ[1] = r * dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
+  m.displ[0] = r * cosPhi;
+  m.displ[1] = r * sinPhi;
   m.displ[2] = (R)0.0;
 }
 
@@ -388,8 +390,10 @@ HD void UMSCSampleScattering(const DevTables<R>& T, Msc<R>& m, R pStep, R preEki
   }
   const R sth = dsqrt(((R)1.0 - cost) * ((R)1.0 + cost));
   const R phi = (R)k2Pi * (R)rng.flat();
-  m.dir[0] = sth * dcos(phi);
-  m.dir[1] = sth * dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
+  m.dir[0] = sth * cosPhi;
+  m.dir[1] = sth * sinPhi;
   m.dir[2] = cost;
   m.isDisplace = m.isDisplace && isDisplace;
   if (m.isDisplace && pStep > m.zpath) UMSCSampleDisplacement(pStep, phi, m, rng);
@@ -570,8 +574,10 @@ template <typename R> HD void IoniSampleDirections(R primEkin, R deltaEkin, R* s
   const R cosTheta = tmax((R)-1.0, tmin(cost, (R)1.0));
   const R sinTheta = dsqrt(((R)1.0 - cosTheta) * ((R)1.0 + cosTheta));
   const R phi = (R)k2Pi * (R)rng.flat();
-  secDir[0] = sinTheta * dcos(phi);
-  secDir[1] = sinTheta * dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
+  secDir[0] = sinTheta * cosPhi;
+  secDir[1] = sinTheta * sinPhi;
   secDir[2] = cosTheta;
   Rotate(secDir, primDir);
   primDir[0] = elInitPTot * primDir[0] - deltaPTot * secDir[0];
@@ -774,8 +780,10 @@ template <typename R> HD void BremSampleDirections(R primEkin, R secEkin, R* sec
   const R cost = SampleCostModifiedTsai(primEkin, rng);
   const R sint = dsqrt(((R)1.0 - cost) * ((R)1.0 + cost));
   const R phi = (R)k2Pi * (R)rng.flat();
-  secDir[0] = sint * dcos(phi);
-  secDir[1] = sint * dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
+  secDir[0] = sint * cosPhi;
+  secDir[1] = sint * sinPhi;
   secDir[2] = cost;
   Rotate(secDir, primDir);
   const R me = (R)kElectronMassC2;
@@ -813,8 +821,10 @@ template <typename R> HD void AnnihilateAtRest(SecList<R>& sl, Rng& rng) {  // :
   Sec<R>* g2 = &sl.s1;
   sl.ns = 2;
   g1->ptype = g2->ptype = 2;
-  g1->dir[0] = sint * dcos(phi);
-  g1->dir[1] = sint * dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
+  g1->dir[0] = sint * cosPhi;
+  g1->dir[1] = sint * sinPhi;
   g1->dir[2] = cost;
   g2->dir[0] = -g1->dir[0];
   g2->dir[1] = -g1->dir[1];
@@ -850,8 +860,10 @@ template <typename R> HD void AnnihilateInFlight(Trk<R>& t, SecList<R>& sl, Rng&
   const R initEt = primEkin + (R)2. * me;
   const R ekinG1 = eps * initEt;
   g1->ekin = (double)ekinG1;
-  g1->dir[0] = sint * dcos(phi);
-  g1->dir[1] = sint * dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
+  g1->dir[0] = sint * cosPhi;
+  g1->dir[1] = sint * sinPhi;
   g1->dir[2] = cost;
   Rotate(g1->dir, primDir);
   g2->ekin = (double)(initEt - ekinG1);
@@ -1250,8 +1262,10 @@ template <typename R> HD void ComptonPerform(Trk<R>& t, SecList<R>& sl, Rng& rng
   const R cost = (R)1.0 - oneMinusCost;
   const R sint = dsqrt(tmax((R)0., sint2));
   const R phi = (R)k2Pi * (R)rng.flat();
-  t.dir[0] = sint * dcos(phi);
-  t.dir[1] = sint * dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
+  t.dir[0] = sint * cosPhi;
+  t.dir[1] = sint * sinPhi;
   t.dir[2] = cost;
   Rotate(t.dir, org);
   const R postE = primE * eps;
@@ -1404,7 +1418,8 @@ template <typename R> HD void ConvPerform(const DevTables<R>& T, Trk<R>& t, SecL
   se->ptype = 0;
   sp->ptype = 1;
   const R phi = (R)k2Pi * (R)rng.flat();
-  const R cosPhi = dcos(phi), sinPhi = dsin(phi);
+  R sinPhi, cosPhi;
+  dsincos(phi, sinPhi, cosPhi);
   const R costEl = SampleCostModifiedTsai(eKin, rng);
   const R sintEl = dsqrt(((R)1.0 - costEl) * ((R)1.0 + costEl));
   se->dir[0] = sintEl * cosPhi;
@@ -1472,8 +1487,10 @@ template <typename R> HD void PEPerform(const DevTables<R>& T, Trk<R>& t, R peMx
     const R costheta = (R)1.0 - tsam;
     const R sint = dsqrt(tsam * ((R)2.0 - tsam));
     const R phi = (R)k2Pi * (R)rng.flat();
-    s->dir[0] = sint * dcos(phi);
-    s->dir[1] = sint * dsin(phi);
+    R sinPhi, cosPhi;
+    dsincos(phi, sinPhi, cosPhi);
+    s->dir[0] = sint * cosPhi;
+    s->dir[1] = sint * sinPhi;
     s->dir[2] = costheta;
     Rotate(s->dir, t.dir);
     s->ekin = peE;

@@ -187,3 +187,19 @@ def test_shower_woodcock_agrees_with_normal_tracking(tables1, ctx1, form):
                     woodcock_min_ekin=0.2)
     c = ctx.slab_run(su, 100, 80, per_event=True)
     assert np.array_equal(c["edep_event"], runs[0.2]["edep_event"][:100])
+
+
+def test_shower_setup_errors_and_empty_run(ctx1):
+    """Invalid set-ups are rejected before any work; a run of no events
+    returns zero sums."""
+    import math
+    from g4hepem_amd import HepEmError
+    from g4hepem_amd._native import slab_setup
+    for w in (-1.0, math.nan):
+        with pytest.raises(HepEmError):
+            ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 0, 100.0, capacity=1024, woodcock_min_ekin=w), 2, 1)
+    with pytest.raises(HepEmError):
+        ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 99], 0, 100.0, capacity=1024), 2, 1)
+    g = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 0, 100.0, capacity=1024, woodcock_min_ekin=0.2), 0, 1,
+                      per_event=True)
+    assert g["steps"] == 0 and g["tracks"] == 0 and not g["edep_sum"].any() and g["leak_sum"] == 0.0
