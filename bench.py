@@ -433,9 +433,14 @@ def main():
         ctx1 = Context(tables1, HEPEM_FP64)
         q100k, ms100k = time_lookups(ctx1, 100_000, 200)
         q10m, ms10m = time_lookups(ctx1, 10_000_000, 20)
+        ctx1.set_lookup_staging(False)  # the same lookups with the tables read from the image
+        g10m, gms10m = time_lookups(ctx1, 10_000_000, 20)
+        ctx1.set_lookup_staging(True)
         line["config1_lookups"] = {"queries_per_s_100k": q100k, "ms_100k": ms100k,
                                    "queries_per_s_10M": q10m, "ms_10M": ms10m,
-                                   "achieved_GBs_10M": 60.0 * 10_000_000 / (ms10m * 1e-3) / 1e9}
+                                   "achieved_GBs_10M": 60.0 * 10_000_000 / (ms10m * 1e-3) / 1e9,
+                                   "tables_in": "LDS",
+                                   "tables_in_image": {"queries_per_s_10M": g10m, "ms_10M": gms10m}}
         del tin, tout, res
         torch.cuda.empty_cache()
         log("config-3 batches")

@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
     L.hepem_step_run.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(Tracks), C.POINTER(StepOut), vp, vp]
     L.hepem_step_rejected.argtypes = [vp, C.POINTER(C.c_uint32), vp]
     L.hepem_ctx_set_process_sort.argtypes = [vp, C.c_int]
+    L.hepem_ctx_set_lookup_staging.argtypes = [vp, C.c_int]
     L.hepem_step_batched.argtypes = [vp, u64, C.POINTER(BatchDesc), i32, vp]
     L.hepem_slab_run.argtypes = [vp, C.POINTER(SlabSetup), i32, u64, C.POINTER(SlabResult), vp]
     L.hepem_batch_plan_create.argtypes = [vp, C.POINTER(BatchDesc), i32, C.POINTER(vp)]
@@ -145,6 +146,7 @@ def lib() -> C.CDLL:
                  "hepem_tables_to_json", "hepem_ctx_create",
                  "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_step",
                  "hepem_step_prepare", "hepem_step_run", "hepem_step_rejected", "hepem_ctx_set_process_sort",
+                 "hepem_ctx_set_lookup_staging",
                  "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",
                  "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform",
                  "hepem_batch_plan_create", "hepem_batch_plan_destroy", "hepem_batch_plan_run", "hepem_batch_plan_tune",

@@ -39,6 +39,11 @@ class Context:
         mode = 2 if on == "auto" else (1 if on else 0)
         N.check("hepem_ctx_set_process_sort", N.lib().hepem_ctx_set_process_sort(self.h, mode))
 
+    def set_lookup_staging(self, on: bool) -> None:
+        """Config-1 lookups from tables staged in LDS (True, the default,
+        where they fit in 64 kB) or from the table image; results are identical."""
+        N.check("hepem_ctx_set_lookup_staging", N.lib().hepem_ctx_set_lookup_staging(self.h, 1 if on else 0))
+
     def lookup(self, ptype: int, imc, ekin, lekin, out=None, stream=None):
         """Config-1 lookups on device tensors; returns a [5, n] float64 tensor
         (dE/dx, range, inverse range, ioni and brem macroscopic cross sections)."""

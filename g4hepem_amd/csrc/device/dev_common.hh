@@ -259,6 +259,7 @@ struct DevEl {
   const XsBlock<R>* xsBlock;  // [2 per couple]: ioni, brem
   const R* xsData;      // restricted macroscopic cross sections
   const double* xsData64;
+  int xsN;              // entries of xsData
   const R* nucGrid;     // [nucN]
   const double* nucGrid64;
   const R* nucData;     // [2 nucN per material]

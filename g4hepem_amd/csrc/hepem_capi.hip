@@ -101,6 +101,7 @@ template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronD
   }
   P.add(E.xsStart, e->fResMacXSecStartIndexPerMatCut, e->fNumMatCuts);
   P.addX(E.xsData, E.xsData64, e->fResMacXSecData, e->fResMacXSecNumData);
+  E.xsN = e->fResMacXSecNumData;
   {
     // block header: [0] triples, [1] last energy, [2] largest value, [3] log of
     // the first energy, [4] 1/log-spacing, then the triples from [5]
@@ -328,6 +329,7 @@ struct hepem_ctx {
   DevTables<float> t32;
   void* work = nullptr;       // internal work buffer for callers passing work == NULL
   uint64_t workBytes = 0;
+  int lookupStaging = 1;  // config-1 lookups read the tables from LDS where they fit
   int processSort = HEPEM_SORT_AUTO;  // sort pending interactions by process (two stages): HEPEM_SORT_*;
                                       // by default auto in the FP64 form, off in the FP32 form (measured faster)
   // per couple: e-, e+, gamma production thresholds and the region's apply-cuts flag
@@ -432,11 +434,26 @@ extern "C" int hepem_lookup_eloss_xsec(const hepem_ctx* c, int ptype, int32_t n,
   if (!imc || !ekin || !lekin || !out) return HEPEM_ERR_INVALID;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const int grid = (n + kBlock - 1) / kBlock;
-  if (c->precision == HEPEM_FP64)
+  const bool fp64 = c->precision == HEPEM_FP64;
+  const size_t lds = fp64 ? LookupLdsBytes(c->t64.el[ptype], c->t64.nmc) : LookupLdsBytes(c->t32.el[ptype], c->t32.nmc);
+  if (c->lookupStaging && lds <= kLookupLdsMax) {
+    const int sgrid = grid < kLookupBlocks ? grid : kLookupBlocks;
+    if (fp64)
+      hipLaunchKernelGGL(k_lookup_lds<double>, dim3(sgrid), dim3(kBlock), lds, s, c->t64, ptype, n, imc, ekin, lekin, out);
+    else
+      hipLaunchKernelGGL(k_lookup_lds<float>, dim3(sgrid), dim3(kBlock), lds, s, c->t32, ptype, n, imc, ekin, lekin, out);
+  } else if (fp64) {
     hipLaunchKernelGGL(k_lookup<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, ptype, n, imc, ekin, lekin, out);
-  else
+  } else {
     hipLaunchKernelGGL(k_lookup<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, ptype, n, imc, ekin, lekin, out);
+  }
   HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_ctx_set_lookup_staging(hepem_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return HEPEM_ERR_INVALID;
+  c->lookupStaging = on;
   return HEPEM_OK;
 }
 

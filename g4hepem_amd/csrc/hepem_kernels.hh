@@ -1024,26 +1024,97 @@ __global__ void __launch_bounds__(kBlock) k_interact_batched(DevTables<R> T, uin
   else GammaInteract<R>(T, seed, out, res, n, i, r, c);
 }
 
+// the five config-1 quantities of query i
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, int n, const int32_t* __restrict__ imc,
-                                                   const double* __restrict__ ekin, const double* __restrict__ lekin,
-                                                   double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const DevEl<R>& e = T.el[ptype];
+HD void LookupOne(const DevEl<R>& e, int nmc, int n, int i, const int32_t* __restrict__ imc, const double* __restrict__ ekin,
+                  const double* __restrict__ lekin, double* __restrict__ out) {
   const int mc = imc[i];
-  if (mc < 0 || mc >= T.nmc) {
-    for (int k = 0; k < 5; ++k) out[k * n + i] = __longlong_as_double(0x7FF8000000000000LL);
+  const size_t sn = (size_t)n;  // 5 n entries may exceed the int range
+  if (mc < 0 || mc >= nmc) {
+    for (int k = 0; k < 5; ++k) out[k * sn + i] = __longlong_as_double(0x7FF8000000000000LL);
     return;
   }
   const double E = ekin[i];
   const R lE = (R)lekin[i];
   const R r = GetRestRange(e, mc, E, lE);
   out[i] = (double)GetRestDEDX(e, mc, E, lE);
-  out[n + i] = (double)r;
-  out[2 * n + i] = GetInvRange(e, mc, (double)r, LogBin(lE, e.lossLogMin, e.lossILD, e.N));
-  out[3 * n + i] = (double)GetRestMacXSec(e, mc, E, lE, true);
-  out[4 * n + i] = (double)GetRestMacXSec(e, mc, E, lE, false);
+  out[sn + i] = (double)r;
+  out[2 * sn + i] = GetInvRange(e, mc, (double)r, LogBin(lE, e.lossLogMin, e.lossILD, e.N));
+  out[3 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, true);
+  out[4 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, false);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, int n, const int32_t* __restrict__ imc,
+                                                   const double* __restrict__ ekin, const double* __restrict__ lekin,
+                                                   double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  LookupOne(T.el[ptype], T.nmc, n, i, imc, ekin, lekin, out);
+}
+
+// LDS staging of the config-1 lookups. The loss grid and data and the
+// cross-section blocks of every couple (the doubles, and in the FP32 form
+// the floats too) are a few kB per couple; each block copies them to LDS
+// once and then serves a grid-stride share of the queries from there, so the
+// table gathers leave the texture path to the query stream.
+constexpr size_t kLookupLdsMax = 64 * 1024;  // larger images use k_lookup
+constexpr int kLookupBlocks = 2048;          // about 8 resident blocks per CU
+
+__host__ __device__ inline size_t Al16(size_t b) { return (b + 15) & ~size_t(15); }
+
+// LDS bytes of the image (the arrays in the order k_lookup_lds stages them)
+template <typename R> __host__ __device__ size_t LookupLdsBytes(const DevEl<R>& e, int nmc) {
+  const size_t N = (size_t)e.N, L = 5 * N * (size_t)nmc, X = (size_t)e.xsN;
+  size_t b = Al16(N * sizeof(double)) + Al16(L * sizeof(double)) + Al16(X * sizeof(double)) +
+             Al16(2 * (size_t)nmc * sizeof(XsBlock<R>));
+  if (sizeof(R) != sizeof(double)) b += Al16(N * sizeof(R)) + Al16(L * sizeof(R)) + Al16(X * sizeof(R));
+  return b;
+}
+
+// copies `bytes` (a multiple of 4) from src to lds + off with the whole block;
+// returns the LDS copy and advances off
+__device__ inline const void* StageArray(unsigned char* lds, size_t& off, const void* src, size_t bytes) {
+  unsigned char* dst = lds + off;
+  if ((bytes & 15) == 0) {
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (size_t k = threadIdx.x; k < bytes / 16; k += blockDim.x) d4[k] = s4[k];
+  } else {
+    const uint32_t* s1 = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d1 = reinterpret_cast<uint32_t*>(dst);
+    for (size_t k = threadIdx.x; k < bytes / 4; k += blockDim.x) d1[k] = s1[k];
+  }
+  off += Al16(bytes);
+  return dst;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_lookup_lds(DevTables<R> T, int ptype, int n, const int32_t* __restrict__ imc,
+                                                       const double* __restrict__ ekin, const double* __restrict__ lekin,
+                                                       double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  DevEl<R> e = T.el[ptype];
+  const int nmc = T.nmc;
+  const size_t N = (size_t)e.N, L = 5 * N * (size_t)nmc, X = (size_t)e.xsN;
+  size_t off = 0;
+  e.lossGrid64 = static_cast<const double*>(StageArray(lds, off, e.lossGrid64, N * sizeof(double)));
+  e.lossData64 = static_cast<const double*>(StageArray(lds, off, e.lossData64, L * sizeof(double)));
+  e.xsData64 = static_cast<const double*>(StageArray(lds, off, e.xsData64, X * sizeof(double)));
+  e.xsBlock = static_cast<const XsBlock<R>*>(StageArray(lds, off, e.xsBlock, 2 * (size_t)nmc * sizeof(XsBlock<R>)));
+  if constexpr (sizeof(R) == sizeof(double)) {
+    // the FP64 form's value arrays are the double arrays
+    e.lossGrid = reinterpret_cast<const R*>(e.lossGrid64);
+    e.lossData = reinterpret_cast<const R*>(e.lossData64);
+    e.xsData = reinterpret_cast<const R*>(e.xsData64);
+  } else {
+    e.lossGrid = static_cast<const R*>(StageArray(lds, off, e.lossGrid, N * sizeof(R)));
+    e.lossData = static_cast<const R*>(StageArray(lds, off, e.lossData, L * sizeof(R)));
+    e.xsData = static_cast<const R*>(StageArray(lds, off, e.xsData, X * sizeof(R)));
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    LookupOne(e, nmc, n, (int)i, imc, ekin, lekin, out);
 }
 
 __global__ void k_reset_state(hepem_tracks t) {

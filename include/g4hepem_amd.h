@@ -194,6 +194,13 @@ int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
 #define HEPEM_SORT_AUTO 2
 #define HEPEM_SORT_AUTO_MIN_TRACKS 32768
 int  hepem_ctx_set_process_sort(hepem_ctx* ctx, int mode);
+/* A speed switch of hepem_lookup_eloss_xsec, like the process sort above:
+ * the particle's loss and cross-section tables staged in LDS (on = 1, the
+ * default; used where they take at most 64 kB) or read from the table image
+ * (on = 0). Results are identical. (The lookups themselves follow
+ * G4HepEmElectronManager's GetRestRange, GetRestDEDX, GetInvRange and
+ * GetRestMacXSec, G4HepEm/G4HepEmRun/include/G4HepEmElectronManager.icc:486-532.) */
+int  hepem_ctx_set_lookup_staging(hepem_ctx* ctx, int on);
 
 /* Number of tracks the last step on `work` rejected (invalid couple index or
  * non-positive/non-finite energy; such tracks get winner -99 and are left

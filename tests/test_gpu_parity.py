@@ -123,6 +123,35 @@ def test_lookup_parity_100k(tables1, ctx1, oracle, dev):
         assert same_bits(out, ref)
 
 
+@pytest.mark.parametrize("materials,form", [("set1", "fp64"), ("set1", "fp32"), ("eight", "fp64"), ("eight", "fp32")])
+def test_lookup_staging_equals_global(tables1, oracle, dev, materials, form):
+    """The lookups from tables staged in LDS equal those from the table image
+    bit for bit, with the grid-stride loop over more queries than one pass of
+    the staged grid; eight couples stage 56 kB in FP64, while FP32 needs
+    84 kB, more than the staging limit, so it reads the image either way. FP64
+    also matches the oracle bit for bit."""
+    from g4hepem_amd import HEPEM_FP32T, HEPEM_FP64, Context, Tables
+    from g4hepem_amd.tracks import lookup_queries
+    tables = tables1 if materials == "set1" else Tables(["G4_Pb", "G4_lAr", "G4_Si", "G4_PbWO4", "G4_W", "G4_Cu", "G4_Fe",
+                                                          "G4_WATER"], 0.7)
+    nmc = 2 if materials == "set1" else 8
+    ctx = Context(tables, HEPEM_FP64 if form == "fp64" else HEPEM_FP32T)
+    imc, e, le = lookup_queries(700_001, 77, nmc, margin=0.05)
+    imc[::1000] = -1  # invalid couples give NaN either way
+    for pt in (0, 1):
+        _print(f"lookup staging {materials} {form} ptype={pt}")
+        ctx.set_lookup_staging(True)
+        a = gpu_lookup(ctx, pt, imc, e, le, dev)
+        ctx.set_lookup_staging(False)
+        b = gpu_lookup(ctx, pt, imc, e, le, dev)
+        assert same_bits(a, b)
+        assert np.isnan(a[:, imc < 0]).all()
+        if form == "fp64":
+            ok = imc >= 0
+            ref = oracle.lookup(tables, pt, imc[ok], e[ok], le[ok])
+            assert same_bits(np.ascontiguousarray(a[:, ok]), ref)
+
+
 def test_lookup_at_grid_nodes(tables1, ctx1, oracle, dev):
     """Energies on the loss-grid nodes and one ulp either side, where the
     range lands on (or next to) a bin edge of the inverse-range search: the
