@@ -330,6 +330,7 @@ struct hepem_ctx {
   void* work = nullptr;       // internal work buffer for callers passing work == NULL
   uint64_t workBytes = 0;
   int lookupStaging = 1;  // config-1 lookups read the tables from LDS where they fit
+  int lookupGrid[2][2] = {{0, 0}, {0, 0}};  // [FP32 form][particle]: resident blocks of the staged kernel (0: not yet known)
   int processSort = HEPEM_SORT_AUTO;  // sort pending interactions by process (two stages): HEPEM_SORT_*;
                                       // by default auto in the FP64 form, off in the FP32 form (measured faster)
   // per couple: e-, e+, gamma production thresholds and the region's apply-cuts flag
@@ -437,7 +438,18 @@ extern "C" int hepem_lookup_eloss_xsec(const hepem_ctx* c, int ptype, int32_t n,
   const bool fp64 = c->precision == HEPEM_FP64;
   const size_t lds = fp64 ? LookupLdsBytes(c->t64.el[ptype], c->t64.nmc) : LookupLdsBytes(c->t32.el[ptype], c->t32.nmc);
   if (c->lookupStaging && lds <= kLookupLdsMax) {
-    const int sgrid = grid < kLookupBlocks ? grid : kLookupBlocks;
+    // one resident round of blocks: each block stages the tables once
+    int& resident = const_cast<hepem_ctx*>(c)->lookupGrid[fp64 ? 0 : 1][ptype];
+    if (resident == 0) {
+      int cus = 0, perCu = 0;
+      HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+      if (fp64)
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_lookup_lds<double>, kBlock, lds));
+      else
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_lookup_lds<float>, kBlock, lds));
+      resident = cus * perCu > 0 ? cus * perCu : kLookupBlocks;
+    }
+    const int sgrid = grid < resident ? grid : resident;
     if (fp64)
       hipLaunchKernelGGL(k_lookup_lds<double>, dim3(sgrid), dim3(kBlock), lds, s, c->t64, ptype, n, imc, ekin, lekin, out);
     else

@@ -1059,7 +1059,7 @@ __global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, in
 // once and then serves a grid-stride share of the queries from there, so the
 // table gathers leave the texture path to the query stream.
 constexpr size_t kLookupLdsMax = 64 * 1024;  // larger images use k_lookup
-constexpr int kLookupBlocks = 2048;          // about 8 resident blocks per CU
+constexpr int kLookupBlocks = 2048;          // grid if the occupancy query gives nothing
 
 __host__ __device__ inline size_t Al16(size_t b) { return (b + 15) & ~size_t(15); }
 
