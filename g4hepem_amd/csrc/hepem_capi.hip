@@ -537,7 +537,7 @@ extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, vo
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, w.counters, 64, 0u);
   if (n > 0) {
     const int grid = (n + kClassifyTile - 1) / kClassifyTile;
-    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kBlock), 0, s, n, in->ptype, w.queue, w.counters);
+    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kClassifyBlock), 0, s, n, in->ptype, w.queue, w.counters);
   }
   HIP_OK(hipGetLastError());
   return HEPEM_OK;
@@ -562,13 +562,13 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
     if (fp64) {
       hipLaunchKernelGGL(k_step_select<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue,
                          w.counters, w.code, w.st);
-      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, w.code, w.pqueue, w.counters);
+      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.pqueue, w.counters);
       hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *out, *res, w.code, w.st, w.pqueue,
                          w.counters);
     } else {
       hipLaunchKernelGGL(k_step_select<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue,
                          w.counters, w.code, w.st);
-      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, w.code, w.pqueue, w.counters);
+      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.pqueue, w.counters);
       hipLaunchKernelGGL(k_interact<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *out, *res, w.code, w.st, w.pqueue,
                          w.counters);
     }
@@ -651,7 +651,7 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
     else
       hipLaunchKernelGGL((k_perform_select<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue,
                          counters, wl.code, wl.st);
-    hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kBlock), 0, s, n, wl.code, wl.pqueue, counters);
+    hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kClassifyBlock), 0, s, n, wl.code, wl.pqueue, counters);
     if (fp64)
       hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *res, wl.code, wl.st, wl.pqueue,
                          counters);
@@ -731,7 +731,7 @@ void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_
   }
   hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
                      p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
-  hipLaunchKernelGGL(k_classify_proc_batched, dim3(p->ntiles), dim3(kBlock), 0, s, p->dDescs, p->nbatch, p->dCode, p->dPQueue,
+  hipLaunchKernelGGL(k_classify_proc_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dCode, p->dPQueue,
                      p->dPCounters);
   hipLaunchKernelGGL(k_interact_batched<R>, dim3(p->nchunks[2]), dim3(kBlock), 0, s, T, seed, p->dDescs, p->nbatch,
                      p->nchunks[2], p->dCode, p->dStaged, p->dPQueue, p->dPCounters);
@@ -826,7 +826,7 @@ extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, vo
     hipLaunchKernelGGL(k_fill_u32, dim3((32 * p->nbatch + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dPCounters,
                        32 * p->nbatch, 0u);
   if (p->total > 0) {
-    hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kBlock), 0, s, p->dDescs, p->nbatch, p->dQueue, p->dCounters);
+    hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue, p->dCounters);
     if (p->ctx->precision == HEPEM_FP64) LaunchStepBatched(p->ctx->t64, p, seed, s);
     else LaunchStepBatched(p->ctx->t32, p, seed, s);
   }

@@ -33,21 +33,23 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
 // passes over the tile: count per wave (ballot + popcount), one atomic per
 // block and kind for the tile's base, then scatter. The tile keeps atomic
 // traffic on the two queue counters to a few hundred operations per launch.
-constexpr int kClassifyItems = 16;
-constexpr int kClassifyTile = kBlock * kClassifyItems;
+constexpr int kClassifyBlock = 1024;  // threads of the classification kernels
+constexpr int kClassifyWaves = kClassifyBlock / 64;
+constexpr int kClassifyItems = 4;
+constexpr int kClassifyTile = kClassifyBlock * kClassifyItems;
 
 // Compacts tile `tile` of a batch of n slots into its queue (e-/e+ from the
 // front, gammas from the back); counters[0..1] count e-/e+ and gammas.
 __device__ void ClassifyTile(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
                              uint32_t* __restrict__ counters, int tile) {
-  __shared__ uint32_t sE[kBlock / 64], sG[kBlock / 64], sBase[2];
+  __shared__ uint32_t sE[kClassifyWaves], sG[kClassifyWaves], sBase[2];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int tile0 = tile * kClassifyTile;
   uint32_t cE = 0, cG = 0;
   for (int it = 0; it < kClassifyItems; ++it) {
-    const int i = tile0 + it * kBlock + threadIdx.x;
+    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
     const bool valid = i < n;
     const int8_t pt = valid ? ptype[i] : (int8_t)-1;
     const bool isE = valid && (pt == 0 || pt == 1);
@@ -61,7 +63,7 @@ __device__ void ClassifyTile(int n, const int8_t* __restrict__ ptype, uint32_t* 
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t tE = 0, tG = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
+    for (int w = 0; w < kClassifyWaves; ++w) {
       const uint32_t e = sE[w], g = sG[w];
       sE[w] = tE;
       sG[w] = tG;
@@ -74,7 +76,7 @@ __device__ void ClassifyTile(int n, const int8_t* __restrict__ ptype, uint32_t* 
   __syncthreads();
   uint32_t pE = sBase[0] + sE[wave], pG = sBase[1] + sG[wave];
   for (int it = 0; it < kClassifyItems; ++it) {
-    const int i = tile0 + it * kBlock + threadIdx.x;
+    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
     const bool valid = i < n;
     const int8_t pt = valid ? ptype[i] : (int8_t)-1;
     const bool isE = valid && (pt == 0 || pt == 1);
@@ -87,7 +89,7 @@ __device__ void ClassifyTile(int n, const int8_t* __restrict__ ptype, uint32_t* 
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_classify(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
+__global__ void __launch_bounds__(kClassifyBlock) k_classify(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue,
                                                      uint32_t* __restrict__ counters) {
   ClassifyTile(n, ptype, queue, counters, blockIdx.x);
 }
@@ -806,7 +808,7 @@ __global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint6
 // atomic per tile and process.
 __device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32_t* __restrict__ pqueue,
                                  uint32_t* __restrict__ counters, int tile) {
-  __shared__ uint32_t sW[kNumProc][kWaves], sBase[kNumProc];
+  __shared__ uint32_t sW[kNumProc][kClassifyWaves], sBase[kNumProc];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -815,7 +817,7 @@ __device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32
 #pragma unroll
   for (int c = 0; c < kNumProc; ++c) cnt[c] = 0u;
   for (int it = 0; it < kClassifyItems; ++it) {
-    const int i = tile0 + it * kBlock + threadIdx.x;
+    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
     const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
 #pragma unroll
     for (int p = 0; p < kNumProc; ++p) cnt[p] += (uint32_t)__popcll(__ballot(c == p + 1));
@@ -828,7 +830,7 @@ __device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32
   if (threadIdx.x < kNumProc) {
     const int p = threadIdx.x;
     uint32_t tot = 0;
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < kClassifyWaves; ++w) {
       const uint32_t v = sW[p][w];
       sW[p][w] = tot;
       tot += v;
@@ -840,7 +842,7 @@ __device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32
 #pragma unroll
   for (int p = 0; p < kNumProc; ++p) pos[p] = sBase[p] + sW[p][wave];
   for (int it = 0; it < kClassifyItems; ++it) {
-    const int i = tile0 + it * kBlock + threadIdx.x;
+    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
     const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
 #pragma unroll
     for (int p = 0; p < kNumProc; ++p) {
@@ -851,7 +853,7 @@ __device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_classify_proc(int n, const uint8_t* __restrict__ code,
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_proc(int n, const uint8_t* __restrict__ code,
                                                           uint32_t* __restrict__ pqueue, uint32_t* __restrict__ counters) {
   ClassifyProcTile(n, code, pqueue, counters, blockIdx.x);
 }
@@ -907,7 +909,7 @@ template <typename Key> HD int FindBatch(const BatchDev* __restrict__ d, int nba
 }
 
 // bcounters[4b + 0..2]: e-/e+, gammas, rejected tracks of batch b
-__global__ void __launch_bounds__(kBlock) k_classify_batched(const BatchDev* __restrict__ d, int nbatch,
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_batched(const BatchDev* __restrict__ d, int nbatch,
                                                              uint32_t* __restrict__ queue, uint32_t* __restrict__ bcounters) {
   const uint32_t t = blockIdx.x;
   const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
@@ -982,7 +984,7 @@ __global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables
                                : StepGammaSelect(T, seed, in, out, res, st + qoff, n, i);
 }
 
-__global__ void __launch_bounds__(kBlock) k_classify_proc_batched(const BatchDev* __restrict__ d, int nbatch,
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_proc_batched(const BatchDev* __restrict__ d, int nbatch,
                                                                   const uint8_t* __restrict__ code,
                                                                   uint32_t* __restrict__ pqueue,
                                                                   uint32_t* __restrict__ pcounters) {
