@@ -183,7 +183,7 @@ def _timed_showers(ctx, su, nevents: int, seed: int):
                      "iterations": g["iterations"], "max_in_flight": g["max_in_flight"]}
 
 
-def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=None):
+def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=None, woodcock: bool = True):
     """10 GeV e- into 50 x (2.3 mm Pb + 5.7 mm lAr) at 0.7 mm: GPU showers of
     `nevents` events (wall clock of the whole driver, host syncs included), and
     the oracle's CPU harness on the first `cpu_events` events for the per-slab
@@ -206,12 +206,14 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=Non
                     "layer_rms_MeV": [round(float(v), 4) for v in e32.std(0)]})
         out["fp32_table_form"] = o32
         del g32
-    gw, _, ow = _timed_showers(ctx, _slab_setup(0.2), nevents, seed)
-    ew = gw["edep_event"] / 1e9
-    ow.update({"woodcock_min_ekin_MeV": 0.2, "mean_edep_total_MeV": float(ew.sum(1).mean()),
-               "vs_normal_tracking": _chi2_means(ev, ew), "speedup_vs_normal": ow["events_per_s"] / out["events_per_s"]})
-    out["woodcock_fp64"] = ow
-    del gw
+    if woodcock:
+        gw, _, ow = _timed_showers(ctx, _slab_setup(0.2), nevents, seed)
+        ew = gw["edep_event"] / 1e9
+        ow.update({"woodcock_min_ekin_MeV": 0.2, "mean_edep_total_MeV": float(ew.sum(1).mean()),
+                   "vs_normal_tracking": _chi2_means(ev, ew),
+                   "speedup_vs_normal": ow["events_per_s"] / out["events_per_s"]})
+        out["woodcock_fp64"] = ow
+        del gw
     if cpu_events > 0:
         from oracle.oracle_py import Oracle
         cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -348,6 +350,10 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="headline FP64 step only (for profiling)")
     ap.add_argument("--shower-events", type=int, default=10000, help="config-5 events (0: skip)")
     ap.add_argument("--shower-cpu-events", type=int, default=100, help="config-5 events for the CPU harness")
+    ap.add_argument("--shower-forms", default="fp64,fp32,woodcock",
+                    help="config-5 runs: the FP64 form (always), the FP32-table form, FP64 with Woodcock tracking")
+    ap.add_argument("--lookups-only", action="store_true",
+                    help="config-1 lookups only, staged and from the table image (for profiling); prints their JSON object")
     ap.add_argument("--shower-only", action="store_true",
                     help="config-5 slab showers only, without the CPU harness (for profiling); prints their JSON object")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -365,10 +371,20 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU visible"
     dev = torch.device("cuda:0")
+    if args.lookups_only:
+        ctx1 = Context(Tables(SET1, 0.7), HEPEM_FP64)
+        q10m, ms10m = time_lookups(ctx1, 10_000_000, 20)
+        ctx1.set_lookup_staging(False)
+        g10m, gms10m = time_lookups(ctx1, 10_000_000, 20)
+        print(json.dumps({"config1_lookups": {"queries_per_s_10M": q10m, "ms_10M": ms10m,
+                                              "tables_in_image": {"queries_per_s_10M": g10m, "ms_10M": gms10m}}}),
+              flush=True)
+        return
     if args.shower_only:
         tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr
+        forms = args.shower_forms.split(",")
         c5 = run_config5(Context(tables_slab, HEPEM_FP64), tables_slab, args.seed, args.shower_events, 0,
-                         Context(tables_slab, HEPEM_FP32T))
+                         Context(tables_slab, HEPEM_FP32T) if "fp32" in forms else None, "woodcock" in forms)
         print(json.dumps({"config5_slab_shower": c5}), flush=True)
         return
     tables2 = Tables(SET2, 0.7)
@@ -463,9 +479,11 @@ def main():
     if not args.no_extras and args.shower_events > 0:
         log("config-5 slab showers")
         tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr
+        forms = args.shower_forms.split(",")
         line["config5_slab_shower"] = run_config5(Context(tables_slab, HEPEM_FP64), tables_slab, args.seed,
                                                   args.shower_events, 0 if args.no_cpu else args.shower_cpu_events,
-                                                  Context(tables_slab, HEPEM_FP32T))
+                                                  Context(tables_slab, HEPEM_FP32T) if "fp32" in forms else None,
+                                                  "woodcock" in forms)
     if not args.no_cpu:
         log("CPU baseline (oracle)")
         line["cpu_baseline"] = cpu_baseline(tables2, tracks, args.cpu_seconds, args.seed)
