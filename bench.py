@@ -361,8 +361,8 @@ def main():
                          "(reference kernel of known traffic for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by track batch with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by track batch, with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -550,22 +550,30 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
   const double safety = Safety(g, l, x);
   const uint64_t sid = ((uint64_t)s.rng_stream[i] << 32) | (uint32_t)s.id[i];
   const int ns1 = a.nstep[i] + 1;
+  // A gamma reads only the first interaction-length slot (its total) and
+  // neither the safety nor the MSC state: those fields of a gamma slot are not
+  // carried (the e-/e+ and gamma parts of the stack keep whole waves on one
+  // side of these branches).
   if (keep[i]) {
     const int8_t pt = s.ptype[i];
-    const int j = pt == 2 ? jG++ : jE++;
+    const bool isE = pt != 2;
+    const int j = isE ? jE++ : jG++;
     d.ptype[j] = pt;
     d.id[j] = s.id[i];
     d.rng_stream[j] = s.rng_stream[i];
     d.imc[j] = g.imc[l % g.nabs];
     d.ekin[j] = s.ekin[i];
     for (int c = 0; c < 3; ++c) d.dir[c * m + j] = s.dir[c * n + i];
-    d.safety[j] = safety;
     d.on_boundary[j] = s.on_boundary[i];
-    for (int k = 0; k < 4; ++k) d.num_ia_left[k * m + j] = s.num_ia_left[k * n + i];
-    d.msc_init_range[j] = s.msc_init_range[i];
-    d.msc_dyn_range_factor[j] = s.msc_dyn_range_factor[i];
-    d.msc_tlimit_min[j] = s.msc_tlimit_min[i];
-    d.msc_first_step[j] = s.msc_first_step[i];
+    d.num_ia_left[j] = s.num_ia_left[i];
+    if (isE) {
+      d.safety[j] = safety;
+      for (int k = 1; k < 4; ++k) d.num_ia_left[k * m + j] = s.num_ia_left[k * n + i];
+      d.msc_init_range[j] = s.msc_init_range[i];
+      d.msc_dyn_range_factor[j] = s.msc_dyn_range_factor[i];
+      d.msc_tlimit_min[j] = s.msc_tlimit_min[i];
+      d.msc_first_step[j] = s.msc_first_step[i];
+    }
     d.rng_ctr[j] = s.rng_ctr[i];
     d.rng_gauss[j] = s.rng_gauss[i];
     d.rng_has_gauss[j] = s.rng_has_gauss[i];
@@ -579,20 +587,24 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
     if (!((mask >> sl) & 1u)) continue;
     const uint64_t cid = ChildStream(sid, ns1, sl);
     const int8_t pt = res.sec_ptype[sl * n + i];
-    const int j = pt == 2 ? jG++ : jE++;
+    const bool isE = pt != 2;
+    const int j = isE ? jE++ : jG++;
     d.ptype[j] = pt;
     d.id[j] = (int32_t)(uint32_t)cid;
     d.rng_stream[j] = (uint32_t)(cid >> 32);
     d.imc[j] = g.imc[l % g.nabs];
     d.ekin[j] = res.sec_ekin[sl * n + i];
     for (int c = 0; c < 3; ++c) d.dir[c * m + j] = res.sec_dir[(c * 2 + sl) * n + i];
-    d.safety[j] = safety;
     d.on_boundary[j] = 0;
-    for (int k = 0; k < 4; ++k) d.num_ia_left[k * m + j] = -1.0;
-    d.msc_init_range[j] = 1.0e+21;
-    d.msc_dyn_range_factor[j] = 0.04;
-    d.msc_tlimit_min[j] = 1.0E-7;
-    d.msc_first_step[j] = 1;
+    d.num_ia_left[j] = -1.0;
+    if (isE) {
+      d.safety[j] = safety;
+      for (int k = 1; k < 4; ++k) d.num_ia_left[k * m + j] = -1.0;
+      d.msc_init_range[j] = 1.0e+21;
+      d.msc_dyn_range_factor[j] = 0.04;
+      d.msc_tlimit_min[j] = 1.0E-7;
+      d.msc_first_step[j] = 1;
+    }
     d.rng_ctr[j] = 0u;
     d.rng_gauss[j] = 0.0;
     d.rng_has_gauss[j] = 0;
