@@ -58,6 +58,8 @@ def test_null_arguments_are_rejected():
     assert L.hepem_batch_plan_run(None, 1, None) == -1
     assert L.hepem_batch_plan_tune(None, 1, None) == -1
     assert L.hepem_batch_plan_destroy(None) == -1
+    assert L.hepem_ctx_set_lookup_staging(None, 1) == -1
+    assert L.hepem_ctx_set_process_sort(None, 1) == -1
     assert L.hepem_status_string(-3) == b"capacity exceeded"
     assert L.hepem_step_work_bytes(1000) >= 4000
 
