@@ -292,6 +292,34 @@ def time_lookups(ctx, n, reps, seed=2026):
     return n / (ms * 1e-3), ms
 
 
+def time_lookups_device(ctx, n, reps, seed=2026):
+    """time_lookups for large n with the queries drawn on the device (torch's
+    Philox generator; same distributions as lookup_queries)."""
+    import math
+    import torch
+    d = torch.device("cuda")
+    gen = torch.Generator(device=d)
+    gen.manual_seed(seed)
+    lo, hi = math.log(1e-3), math.log(1e8)
+    tle = torch.rand(n, generator=gen, device=d, dtype=torch.float64) * (hi - lo) + lo
+    te = torch.exp(tle)
+    timc = torch.randint(0, len(SET1), (n,), generator=gen, device=d, dtype=torch.int32)
+    out = torch.empty(5 * n, dtype=torch.float64, device=d)
+    for _ in range(3):
+        ctx.lookup(0, timc, te, tle, out)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        ctx.lookup(0, timc, te, tle, out)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del timc, te, tle, out
+    torch.cuda.empty_cache()
+    return n / (ms * 1e-3), ms
+
+
 def cpu_baseline(tables, tracks: dict, min_seconds: float, seed: int):
     """The path on the host cores: the reference's own run-time code compiled
     from its sources (oracle/_ref, kind "reference") when that library is
@@ -374,9 +402,11 @@ def main():
     if args.lookups_only:
         ctx1 = Context(Tables(SET1, 0.7), HEPEM_FP64)
         q10m, ms10m = time_lookups(ctx1, 10_000_000, 20)
+        q100m, ms100m = time_lookups_device(ctx1, 100_000_000, 10)
         ctx1.set_lookup_staging(False)
         g10m, gms10m = time_lookups(ctx1, 10_000_000, 20)
         print(json.dumps({"config1_lookups": {"queries_per_s_10M": q10m, "ms_10M": ms10m,
+                                              "queries_per_s_100M": q100m, "ms_100M": ms100m,
                                               "tables_in_image": {"queries_per_s_10M": g10m, "ms_10M": gms10m}}}),
               flush=True)
         return
@@ -449,12 +479,17 @@ def main():
         ctx1 = Context(tables1, HEPEM_FP64)
         q100k, ms100k = time_lookups(ctx1, 100_000, 200)
         q10m, ms10m = time_lookups(ctx1, 10_000_000, 20)
+        sweep = {}
+        for nq in (30_000_000, 100_000_000):
+            qs, mss = time_lookups_device(ctx1, nq, 10)
+            sweep[f"{nq // 1_000_000}M"] = {"queries_per_s": qs, "ms": mss, "achieved_GBs": 60.0 * nq / (mss * 1e-3) / 1e9}
         ctx1.set_lookup_staging(False)  # the same lookups with the tables read from the image
         g10m, gms10m = time_lookups(ctx1, 10_000_000, 20)
         ctx1.set_lookup_staging(True)
         line["config1_lookups"] = {"queries_per_s_100k": q100k, "ms_100k": ms100k,
                                    "queries_per_s_10M": q10m, "ms_10M": ms10m,
                                    "achieved_GBs_10M": 60.0 * 10_000_000 / (ms10m * 1e-3) / 1e9,
+                                   "sweep_device_drawn": sweep,
                                    "tables_in": "LDS",
                                    "tables_in_image": {"queries_per_s_10M": g10m, "ms_10M": gms10m}}
         del tin, tout, res
