@@ -1026,6 +1026,23 @@ __global__ void __launch_bounds__(kBlock) k_interact_batched(DevTables<R> T, uin
   else GammaInteract<R>(T, seed, out, res, n, i, r, c);
 }
 
+// the five config-1 quantities of query i, whose inputs are (mc, E, lnE)
+template <typename R>
+HD void LookupValues(const DevEl<R>& e, int nmc, int n, int i, int mc, double E, double lnE, double* __restrict__ out) {
+  const size_t sn = (size_t)n;  // 5 n entries may exceed the int range
+  if (mc < 0 || mc >= nmc) {
+    for (int k = 0; k < 5; ++k) out[k * sn + i] = __longlong_as_double(0x7FF8000000000000LL);
+    return;
+  }
+  const R lE = (R)lnE;
+  const R r = GetRestRange(e, mc, E, lE);
+  out[i] = (double)GetRestDEDX(e, mc, E, lE);
+  out[sn + i] = (double)r;
+  out[2 * sn + i] = GetInvRange(e, mc, (double)r, LogBin(lE, e.lossLogMin, e.lossILD, e.N));
+  out[3 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, true);
+  out[4 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, false);
+}
+
 // the five config-1 quantities of query i
 template <typename R>
 HD void LookupOne(const DevEl<R>& e, int nmc, int n, int i, const int32_t* __restrict__ imc, const double* __restrict__ ekin,
@@ -1115,8 +1132,29 @@ __global__ void __launch_bounds__(kBlock) k_lookup_lds(DevTables<R> T, int ptype
     e.xsData = static_cast<const R*>(StageArray(lds, off, e.xsData, X * sizeof(R)));
   }
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    LookupOne(e, nmc, n, (int)i, imc, ekin, lekin, out);
+  // the next query's inputs are loaded before this one is computed, so their
+  // HBM latency overlaps the table work
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int mc = imc[i];
+  double E = ekin[i], lnE = lekin[i];
+  while (true) {
+    const int64_t next = i + stride;
+    int mcN = 0;
+    double EN = 0.0, lnEN = 0.0;
+    if (next < n) {
+      mcN = imc[next];
+      EN = ekin[next];
+      lnEN = lekin[next];
+    }
+    LookupValues(e, nmc, n, (int)i, mc, E, lnE, out);
+    if (next >= n) break;
+    i = next;
+    mc = mcN;
+    E = EN;
+    lnE = lnEN;
+  }
 }
 
 __global__ void k_reset_state(hepem_tracks t) {
