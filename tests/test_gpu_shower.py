@@ -203,3 +203,21 @@ def test_shower_setup_errors_and_empty_run(ctx1):
     g = ctx1.slab_run(slab_setup(LAYERS, THICK, [0, 1], 0, 100.0, capacity=1024, woodcock_min_ekin=0.2), 0, 1,
                       per_event=True)
     assert g["steps"] == 0 and g["tracks"] == 0 and not g["edep_sum"].any() and g["leak_sum"] == 0.0
+
+
+@pytest.mark.parametrize("woodcock", [0.0, 0.2])
+def test_shower_independent_of_process_sort(tables1, woodcock):
+    """The driver's Perform calls give the same events with the process sort
+    on, off or automatic (the sort only changes which thread samples a slot)."""
+    from g4hepem_amd import HEPEM_FP64, Context
+    from g4hepem_amd._native import slab_setup
+    su = slab_setup(LAYERS, THICK, [0, 1], 0, 1000.0, capacity=1 << 21, events_per_chunk=40, woodcock_min_ekin=woodcock)
+    ctx = Context(tables1, HEPEM_FP64)
+    runs = []
+    for mode in ("auto", True, False):
+        ctx.set_process_sort(mode)
+        runs.append(ctx.slab_run(su, 40, 81, per_event=True))
+    for g in runs[1:]:
+        assert np.array_equal(g["edep_event"], runs[0]["edep_event"])
+        assert np.array_equal(g["leak_event"], runs[0]["leak_event"])
+        assert g["steps"] == runs[0]["steps"] and g["tracks"] == runs[0]["tracks"]
