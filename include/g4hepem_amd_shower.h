@@ -25,8 +25,12 @@
  * G4HepEmTrackingManager.cc:955-1062): steps of the densest absorber's mean
  * free path, each point accepted as an interaction with probability
  * Sigma(material there) / Sigma(densest), until an interaction or the stack's
- * exit (less 1e-3 mm) is reached; the interaction then happens in the slab
- * where it lies, with that slab's couple.
+ * exit (less 1e-3 mm) is reached. The interaction then happens in the slab
+ * where it lies, with that slab's couple (the densest material's own couple
+ * where it lies in that material); a gamma that reaches the exit leaves the
+ * stack in that step. A gamma within 1e-3 + 1e-6 mm of the exit is tracked
+ * normally. The Woodcock step replaces the gamma's HowFar proposal, so the one
+ * number HowFar drew from its stream for the interaction length goes unused.
  *
  * Every track owns the Philox stream selected by a 64-bit id (id, rng_stream)
  * derived from its parent's id, the parent's step number and the secondary
