@@ -495,6 +495,8 @@ struct WorkLayout {
   uint8_t* code;
   Staged* st;
   uint32_t* pqueue;
+  uint16_t* tlist;   // the tile-local process layout (aliases pqueue)
+  uint32_t* tinfo;   // per classification tile: process offsets and first chunks
   WorkLayout(void* w, int32_t n) {
     uint8_t* p = static_cast<uint8_t*>(w);
     const uint64_t m = (uint64_t)(n > 0 ? n : 0);
@@ -507,13 +509,18 @@ struct WorkLayout {
     st = reinterpret_cast<Staged*>(p);
     p += Al(sizeof(Staged) * m);
     pqueue = reinterpret_cast<uint32_t*>(p);
+    tlist = reinterpret_cast<uint16_t*>(p);
+    p += Al(4 * m * kNumProc);
+    tinfo = reinterpret_cast<uint32_t*>(p);
   }
 };
 }  // namespace
 
 extern "C" uint64_t hepem_step_work_bytes(int32_t n) {
   const uint64_t m = (uint64_t)(n > 0 ? n : 0);
-  return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(4 * m * kNumProc);
+  const uint64_t tiles = (m + kClassifyTile - 1) / kClassifyTile;
+  return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(4 * m * kNumProc) +
+         Al(4 * kTileInfo * tiles);
 }
 
 extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
@@ -559,18 +566,19 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
       hipLaunchKernelGGL(k_step<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue, w.counters);
   } else {
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
+    const int igrid = InteractGrid(n);
     if (fp64) {
       hipLaunchKernelGGL(k_step_select<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue,
                          w.counters, w.code, w.st);
-      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.pqueue, w.counters);
-      hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *out, *res, w.code, w.st, w.pqueue,
-                         w.counters);
+      hipLaunchKernelGGL(k_classify_tiles, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.tlist, w.tinfo);
+      hipLaunchKernelGGL(k_interact_tiles<double>, dim3(igrid), dim3(kBlock), 0, s, c->t64, seed, *out, *res, w.code, w.st,
+                         w.tlist, w.tinfo);
     } else {
       hipLaunchKernelGGL(k_step_select<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue,
                          w.counters, w.code, w.st);
-      hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.pqueue, w.counters);
-      hipLaunchKernelGGL(k_interact<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *out, *res, w.code, w.st, w.pqueue,
-                         w.counters);
+      hipLaunchKernelGGL(k_classify_tiles, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.tlist, w.tinfo);
+      hipLaunchKernelGGL(k_interact_tiles<float>, dim3(igrid), dim3(kBlock), 0, s, c->t32, seed, *out, *res, w.code, w.st,
+                         w.tlist, w.tinfo);
     }
   }
   HIP_OK(hipGetLastError());
@@ -643,21 +651,23 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
   } else if (!perform) {
     hipLaunchKernelGGL((k_howfar<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, queue, counters);
   } else if (SortsProcesses(c, n)) {
-    // choice of the interaction, process sort, final states (as hepem_step_run)
+    // choice of the interaction, then the tile-local process sort and final
+    // states (as hepem_step_run)
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
+    const int igrid = InteractGrid(n);
     if (fp64)
       hipLaunchKernelGGL((k_perform_select<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue,
                          counters, wl.code, wl.st);
     else
       hipLaunchKernelGGL((k_perform_select<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue,
                          counters, wl.code, wl.st);
-    hipLaunchKernelGGL(k_classify_proc, dim3(tiles), dim3(kClassifyBlock), 0, s, n, wl.code, wl.pqueue, counters);
+    hipLaunchKernelGGL(k_classify_tiles, dim3(tiles), dim3(kClassifyBlock), 0, s, n, wl.code, wl.tlist, wl.tinfo);
     if (fp64)
-      hipLaunchKernelGGL(k_interact<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *res, wl.code, wl.st, wl.pqueue,
-                         counters);
+      hipLaunchKernelGGL(k_interact_tiles<double>, dim3(igrid), dim3(kBlock), 0, s, c->t64, seed, *t, *res, wl.code, wl.st,
+                         wl.tlist, wl.tinfo);
     else
-      hipLaunchKernelGGL(k_interact<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *res, wl.code, wl.st, wl.pqueue,
-                         counters);
+      hipLaunchKernelGGL(k_interact_tiles<float>, dim3(igrid), dim3(kBlock), 0, s, c->t32, seed, *t, *res, wl.code, wl.st,
+                         wl.tlist, wl.tinfo);
   } else if (fp64) {
     hipLaunchKernelGGL((k_perform<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue, counters);
   } else {

@@ -509,9 +509,10 @@ __global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, 
 // interaction (one thread per particle-queue entry), records the chosen process
 // per slot and, for a pending interaction, the 64-byte record of what the
 // final-state sampling needs (at the slot: one line per track instead of ten
-// scattered fields); k_classify_proc orders the pending slots by process into
-// one region per process; k_interact samples the final states, so each wave
-// runs one sampler (only the waves that straddle two regions diverge).
+// scattered fields); k_classify_tiles orders each tile's pending slots by
+// process; k_interact_tiles samples the final states, so each wave runs one
+// sampler. (The batched step orders the pending slots of a batch into one
+// region per process, k_classify_proc_batched and k_interact_batched.)
 
 // processes of the second stage (0: none pending), in the low 4 bits of the
 // per-slot code; bit 4: the track is an e-, bit 5: a Gaussian deviate is cached
@@ -780,8 +781,7 @@ __global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t
 }
 
 // Stage 1 of the split Perform (as k_perform up to the choice of the
-// interaction). It also clears the process counters of the classification
-// that follows it on the stream.
+// interaction). It also clears the per-process counters of the work buffer.
 template <typename R, int KIND>
 __global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint64_t seed, hepem_tracks t,
                                                            hepem_howfar_state hs, hepem_step_out res,
@@ -853,32 +853,112 @@ __device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32
   }
 }
 
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_proc(int n, const uint8_t* __restrict__ code,
-                                                          uint32_t* __restrict__ pqueue, uint32_t* __restrict__ counters) {
-  ClassifyProcTile(n, code, pqueue, counters, blockIdx.x);
+// Process sort and final states, tile by tile (the single-batch two-stage
+// step and Perform).
+//
+// k_classify_tiles lays the pending slots of each tile of kClassifyTile
+// slots out process by process, within the tile's own range of `tlist` (tile
+// offsets, 16 bits), and writes the tile's process offsets and chunk counts
+// to `tinfo` (wave ballots and an LDS scan, as ClassifyProcTile, but no
+// atomics: the layout is deterministic).
+//
+// k_interact_tiles gives each wave one chunk of up to 64 entries of one
+// process. A tile's chunks are spread over kInteractParts blocks that go to
+// the same XCD one after another (blocks are dealt to the XCDs round-robin),
+// so all final-state writes to the tile's cache lines come from one XCD
+// within a short time and its L2 merges them into whole lines before they go
+// to HBM. With a global process-sorted queue each wave's writes land a few
+// slots per line and reach HBM as partial lines.
+constexpr int kXcds = 8;
+constexpr int kTileInfo = 2 * (kNumProc + 1);   // per tile: process offsets, first chunks
+constexpr int kInteractWaves = kBlock / 64;
+// chunks of a tile: at most one per 64 slots plus one partial chunk per process
+constexpr int kInteractParts = (kClassifyTile / 64 + kNumProc + kInteractWaves - 1) / kInteractWaves;
+
+inline int InteractGrid(int n) {
+  const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
+  return (tiles + kXcds - 1) / kXcds * kXcds * kInteractParts;
+}
+
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles(int n, const uint8_t* __restrict__ code,
+                                                                   uint16_t* __restrict__ tlist, uint32_t* __restrict__ tinfo) {
+  __shared__ uint32_t sW[kNumProc][kClassifyWaves];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int tile0 = blockIdx.x * kClassifyTile;
+  uint32_t cnt[kNumProc];
+#pragma unroll
+  for (int q = 0; q < kNumProc; ++q) cnt[q] = 0u;
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
+    const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
+#pragma unroll
+    for (int q = 0; q < kNumProc; ++q) cnt[q] += (uint32_t)__popcll(__ballot(c == q + 1));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < kNumProc; ++q) sW[q][wave] = cnt[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* info = tinfo + (size_t)blockIdx.x * kTileInfo;
+    uint32_t off = 0u, chunk = 0u;
+    for (int q = 0; q < kNumProc; ++q) {
+      info[q] = off;
+      info[kNumProc + 1 + q] = chunk;
+      const uint32_t first = off;
+      for (int w = 0; w < kClassifyWaves; ++w) {
+        const uint32_t v = sW[q][w];
+        sW[q][w] = off;
+        off += v;
+      }
+      chunk += (off - first + 63u) / 64u;
+    }
+    info[kNumProc] = off;
+    info[2 * kNumProc + 1] = chunk;
+  }
+  __syncthreads();
+  uint32_t pos[kNumProc];
+#pragma unroll
+  for (int q = 0; q < kNumProc; ++q) pos[q] = sW[q][wave];
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int local = it * kClassifyBlock + threadIdx.x;
+    const int i = tile0 + local;
+    const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
+#pragma unroll
+    for (int q = 0; q < kNumProc; ++q) {
+      const uint64_t m = __ballot(c == q + 1);
+      if (c == q + 1) tlist[tile0 + pos[q] + __popcll(m & below)] = (uint16_t)local;
+      pos[q] += (uint32_t)__popcll(m);
+    }
+  }
 }
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_interact(DevTables<R> T, uint64_t seed, hepem_tracks out, hepem_step_out res,
-                                                     const uint8_t* __restrict__ code, const Staged* __restrict__ st,
-                                                     const uint32_t* __restrict__ pqueue,
-                                                     const uint32_t* __restrict__ counters) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_interact_tiles(DevTables<R> T, uint64_t seed, hepem_tracks out, hepem_step_out res,
+                                                           const uint8_t* __restrict__ code, const Staged* __restrict__ st,
+                                                           const uint16_t* __restrict__ tlist,
+                                                           const uint32_t* __restrict__ tinfo) {
   const int n = out.n;
-  // entry j of the concatenation of the process regions
-  uint32_t first = 0;
-  int p = -1;
-#pragma unroll
-  for (int q = 0; q < kNumProc; ++q) {
-    const uint32_t c = counters[kProcCountBase + q + 1];
-    if (p < 0 && (uint32_t)j < first + c) p = q;
-    if (p < 0) first += c;
-  }
-  if (p < 0) return;
-  const int i = (int)pqueue[(size_t)p * n + (j - first)];
+  // block -> (tile, part): XCD x takes tiles x, x + kXcds, ..., each as
+  // kInteractParts consecutive blocks of its share
+  const int x = (int)(blockIdx.x % kXcds);
+  const int inXcd = (int)(blockIdx.x / kXcds);
+  const int tile = (inXcd / kInteractParts) * kXcds + x;
+  const int tile0 = tile * kClassifyTile;
+  if (tile0 >= n) return;
+  const uint32_t* info = tinfo + (size_t)tile * kTileInfo;
+  const uint32_t chunk = (uint32_t)((inXcd % kInteractParts) * kInteractWaves + (threadIdx.x >> 6));
+  if (chunk >= info[2 * kNumProc + 1]) return;
+  int q = 0;
+  while (q + 1 < kNumProc && chunk >= info[kNumProc + 2 + q]) ++q;
+  const uint32_t k = (chunk - info[kNumProc + 1 + q]) * 64u + (uint32_t)__lane_id();
+  if (k >= info[q + 1] - info[q]) return;
+  const int i = tile0 + (int)tlist[tile0 + info[q] + k];
   const uint8_t c = code[i];
   const Staged r = st[i];
-  if (p + 1 < kPConv) ElectronInteract<R>(T, seed, out, res, n, i, r, c);
+  if (q + 1 < kPConv) ElectronInteract<R>(T, seed, out, res, n, i, r, c);
   else GammaInteract<R>(T, seed, out, res, n, i, r, c);
 }
 
