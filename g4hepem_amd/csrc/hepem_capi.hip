@@ -494,8 +494,7 @@ struct WorkLayout {
   uint32_t* queue;
   uint8_t* code;
   Staged* st;
-  uint32_t* pqueue;
-  uint16_t* tlist;   // the tile-local process layout (aliases pqueue)
+  uint16_t* tlist;   // the tile-local process layout: per slot, a tile offset
   uint32_t* tinfo;   // per classification tile: process offsets and first chunks
   WorkLayout(void* w, int32_t n) {
     uint8_t* p = static_cast<uint8_t*>(w);
@@ -508,9 +507,8 @@ struct WorkLayout {
     p += Al(m);
     st = reinterpret_cast<Staged*>(p);
     p += Al(sizeof(Staged) * m);
-    pqueue = reinterpret_cast<uint32_t*>(p);
     tlist = reinterpret_cast<uint16_t*>(p);
-    p += Al(4 * m * kNumProc);
+    p += Al(2 * m);
     tinfo = reinterpret_cast<uint32_t*>(p);
   }
 };
@@ -519,8 +517,7 @@ struct WorkLayout {
 extern "C" uint64_t hepem_step_work_bytes(int32_t n) {
   const uint64_t m = (uint64_t)(n > 0 ? n : 0);
   const uint64_t tiles = (m + kClassifyTile - 1) / kClassifyTile;
-  return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(4 * m * kNumProc) +
-         Al(4 * kTileInfo * tiles);
+  return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(2 * m) + Al(4 * kTileInfo * tiles);
 }
 
 extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
@@ -725,8 +722,8 @@ struct hepem_batch_plan {
   bool processSort = true;
   uint8_t* dCode = nullptr;       // process code per global slot
   Staged* dStaged = nullptr;      // staged record per global slot
-  uint32_t* dPQueue = nullptr;    // per batch b: kNumProc regions of n_b entries from kNumProc*qoff_b
-  uint32_t* dPCounters = nullptr; // 32 words per batch: [kProcCountBase + c] pending slots of process c
+  uint16_t* dTList = nullptr;     // tile-local process layout, batch b's slots from qoff_b
+  uint32_t* dTInfo = nullptr;     // kTileInfo words per classification tile (tiles numbered across batches)
   int shape = kNumShapes - 1;     // block size ShapeBlock(shape)
   int grid = 0;                   // step-kernel blocks (nchunks[shape])
 };
@@ -741,10 +738,10 @@ void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_
   }
   hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
                      p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
-  hipLaunchKernelGGL(k_classify_proc_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dCode, p->dPQueue,
-                     p->dPCounters);
-  hipLaunchKernelGGL(k_interact_batched<R>, dim3(p->nchunks[2]), dim3(kBlock), 0, s, T, seed, p->dDescs, p->nbatch,
-                     p->nchunks[2], p->dCode, p->dStaged, p->dPQueue, p->dPCounters);
+  hipLaunchKernelGGL(k_classify_tiles_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dCode,
+                     p->dTList, p->dTInfo);
+  hipLaunchKernelGGL(k_interact_tiles_batched<R>, dim3(InteractGridTiles((int)p->ntiles)), dim3(kBlock), 0, s, T, seed, p->dDescs,
+                     p->nbatch, p->ntiles, p->dCode, p->dStaged, p->dTList, p->dTInfo);
 }
 template <typename R> void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
   switch (p->shape) {
@@ -760,8 +757,8 @@ void FreePlan(hepem_batch_plan* p) {
   if (p->dCounters) (void)hipFree(p->dCounters);
   if (p->dCode) (void)hipFree(p->dCode);
   if (p->dStaged) (void)hipFree(p->dStaged);
-  if (p->dPQueue) (void)hipFree(p->dPQueue);
-  if (p->dPCounters) (void)hipFree(p->dPCounters);
+  if (p->dTList) (void)hipFree(p->dTList);
+  if (p->dTInfo) (void)hipFree(p->dTInfo);
   delete p;
 }
 }  // namespace
@@ -808,8 +805,8 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
       hipMalloc(&p->dQueue, (q > 0 ? q : 1) * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&p->dCounters, (size_t)4 * nbatch * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&p->dCode, (q > 0 ? q : 1)) != hipSuccess || hipMalloc(&p->dStaged, (q > 0 ? q : 1) * sizeof(Staged)) != hipSuccess ||
-      hipMalloc(&p->dPQueue, (size_t)kNumProc * (q > 0 ? q : 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&p->dPCounters, (size_t)32 * nbatch * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&p->dTList, (q > 0 ? q : 1) * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&p->dTInfo, (size_t)kTileInfo * (tiles > 0 ? tiles : 1) * sizeof(uint32_t)) != hipSuccess) {
     FreePlan(p);
     return HEPEM_ERR_ALLOC;
   }
@@ -832,9 +829,6 @@ extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, vo
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const int nc = 4 * p->nbatch;
   hipLaunchKernelGGL(k_fill_u32, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dCounters, nc, 0u);
-  if (p->processSort)
-    hipLaunchKernelGGL(k_fill_u32, dim3((32 * p->nbatch + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dPCounters,
-                       32 * p->nbatch, 0u);
   if (p->total > 0) {
     hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue, p->dCounters);
     if (p->ctx->precision == HEPEM_FP64) LaunchStepBatched(p->ctx->t64, p, seed, s);

@@ -511,8 +511,8 @@ __global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, 
 // final-state sampling needs (at the slot: one line per track instead of ten
 // scattered fields); k_classify_tiles orders each tile's pending slots by
 // process; k_interact_tiles samples the final states, so each wave runs one
-// sampler. (The batched step orders the pending slots of a batch into one
-// region per process, k_classify_proc_batched and k_interact_batched.)
+// sampler. (The batched step does the same over the tiles of all its batches,
+// k_classify_tiles_batched and k_interact_tiles_batched.)
 
 // processes of the second stage (0: none pending), in the low 4 bits of the
 // per-slot code; bit 4: the track is an e-, bit 5: a Gaussian deviate is cached
@@ -528,7 +528,6 @@ enum ProcCode : uint8_t {
   kPPE,        // photoelectric
 };
 constexpr int kNumProc = 8;          // codes 1..8
-constexpr int kProcCountBase = 16;   // counters[16 + c]: slots pending process c
 constexpr uint8_t kCodeProc = 0x0F, kCodeElectron = 0x10, kCodeGauss = 0x20;
 
 // state handed from the selection to the final-state sampling of one slot
@@ -781,14 +780,13 @@ __global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t
 }
 
 // Stage 1 of the split Perform (as k_perform up to the choice of the
-// interaction). It also clears the per-process counters of the work buffer.
+// interaction).
 template <typename R, int KIND>
 __global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint64_t seed, hepem_tracks t,
                                                            hepem_howfar_state hs, hepem_step_out res,
                                                            const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
                                                            uint8_t* __restrict__ code, Staged* __restrict__ st) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x < kNumProc) counters[kProcCountBase + 1 + threadIdx.x] = 0u;
   const int n = t.n;
   if (j >= n) return;
   int i;
@@ -802,65 +800,14 @@ __global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint6
   code[i] = isE ? PerformElectronSelect<R>(T, seed, t, hs, res, st, n, i) : PerformGammaSelect<R>(T, seed, t, hs, res, st, n, i);
 }
 
-// Orders the pending slots by process: region c-1 of pqueue (n entries each)
-// receives the slots with process c; counters[kProcCountBase + c] counts them.
-// Tiles of kClassifyTile slots: wave ballots per process, an LDS scan, and one
-// atomic per tile and process.
-__device__ void ClassifyProcTile(int n, const uint8_t* __restrict__ code, uint32_t* __restrict__ pqueue,
-                                 uint32_t* __restrict__ counters, int tile) {
-  __shared__ uint32_t sW[kNumProc][kClassifyWaves], sBase[kNumProc];
-  const int lane = __lane_id();
-  const int wave = threadIdx.x >> 6;
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int tile0 = tile * kClassifyTile;
-  uint32_t cnt[kNumProc];
-#pragma unroll
-  for (int c = 0; c < kNumProc; ++c) cnt[c] = 0u;
-  for (int it = 0; it < kClassifyItems; ++it) {
-    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
-    const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
-#pragma unroll
-    for (int p = 0; p < kNumProc; ++p) cnt[p] += (uint32_t)__popcll(__ballot(c == p + 1));
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int p = 0; p < kNumProc; ++p) sW[p][wave] = cnt[p];
-  }
-  __syncthreads();
-  if (threadIdx.x < kNumProc) {
-    const int p = threadIdx.x;
-    uint32_t tot = 0;
-    for (int w = 0; w < kClassifyWaves; ++w) {
-      const uint32_t v = sW[p][w];
-      sW[p][w] = tot;
-      tot += v;
-    }
-    sBase[p] = tot ? atomicAdd(&counters[kProcCountBase + p + 1], tot) : 0u;
-  }
-  __syncthreads();
-  uint32_t pos[kNumProc];
-#pragma unroll
-  for (int p = 0; p < kNumProc; ++p) pos[p] = sBase[p] + sW[p][wave];
-  for (int it = 0; it < kClassifyItems; ++it) {
-    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
-    const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
-#pragma unroll
-    for (int p = 0; p < kNumProc; ++p) {
-      const uint64_t m = __ballot(c == p + 1);
-      if (c == p + 1) pqueue[(size_t)p * n + pos[p] + __popcll(m & below)] = (uint32_t)i;
-      pos[p] += (uint32_t)__popcll(m);
-    }
-  }
-}
-
 // Process sort and final states, tile by tile (the single-batch two-stage
 // step and Perform).
 //
 // k_classify_tiles lays the pending slots of each tile of kClassifyTile
 // slots out process by process, within the tile's own range of `tlist` (tile
 // offsets, 16 bits), and writes the tile's process offsets and chunk counts
-// to `tinfo` (wave ballots and an LDS scan, as ClassifyProcTile, but no
-// atomics: the layout is deterministic).
+// to `tinfo` (wave ballots and an LDS scan; no atomics, so the layout is
+// deterministic).
 //
 // k_interact_tiles gives each wave one chunk of up to 64 entries of one
 // process. A tile's chunks are spread over kInteractParts blocks that go to
@@ -875,18 +822,18 @@ constexpr int kInteractWaves = kBlock / 64;
 // chunks of a tile: at most one per 64 slots plus one partial chunk per process
 constexpr int kInteractParts = (kClassifyTile / 64 + kNumProc + kInteractWaves - 1) / kInteractWaves;
 
-inline int InteractGrid(int n) {
-  const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
-  return (tiles + kXcds - 1) / kXcds * kXcds * kInteractParts;
-}
+inline int InteractGridTiles(int tiles) { return (tiles + kXcds - 1) / kXcds * kXcds * kInteractParts; }
+inline int InteractGrid(int n) { return InteractGridTiles((n + kClassifyTile - 1) / kClassifyTile); }
 
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles(int n, const uint8_t* __restrict__ code,
-                                                                   uint16_t* __restrict__ tlist, uint32_t* __restrict__ tinfo) {
+// the process layout of tile `tile` of a batch of n slots (codes, tile list
+// and the tile's info words of that batch)
+__device__ void ClassifyTileLayout(int n, const uint8_t* __restrict__ code, uint16_t* __restrict__ tlist,
+                                   uint32_t* __restrict__ info, int tile) {
   __shared__ uint32_t sW[kNumProc][kClassifyWaves];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int tile0 = blockIdx.x * kClassifyTile;
+  const int tile0 = tile * kClassifyTile;
   uint32_t cnt[kNumProc];
 #pragma unroll
   for (int q = 0; q < kNumProc; ++q) cnt[q] = 0u;
@@ -902,7 +849,6 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles(int n, const 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t* info = tinfo + (size_t)blockIdx.x * kTileInfo;
     uint32_t off = 0u, chunk = 0u;
     for (int q = 0; q < kNumProc; ++q) {
       info[q] = off;
@@ -935,21 +881,29 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles(int n, const 
   }
 }
 
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles(int n, const uint8_t* __restrict__ code,
+                                                                   uint16_t* __restrict__ tlist, uint32_t* __restrict__ tinfo) {
+  ClassifyTileLayout(n, code, tlist, tinfo + (size_t)blockIdx.x * kTileInfo, (int)blockIdx.x);
+}
+
+// block -> (tile, chunk of this wave): XCD x takes tiles x, x + kXcds, ...,
+// each as kInteractParts consecutive blocks of its share; false past the last
+// tile
+__device__ inline bool TileChunk(uint32_t ntiles, uint32_t& tile, uint32_t& chunk) {
+  const uint32_t x = blockIdx.x % kXcds;
+  const uint32_t inXcd = blockIdx.x / kXcds;
+  tile = (inXcd / kInteractParts) * kXcds + x;
+  chunk = (inXcd % kInteractParts) * kInteractWaves + (threadIdx.x >> 6);
+  return tile < ntiles;
+}
+
+// final state of this lane's entry of chunk `chunk` of a tile whose first
+// slot is tile0 (codes, staged records and tile list of its batch)
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_interact_tiles(DevTables<R> T, uint64_t seed, hepem_tracks out, hepem_step_out res,
-                                                           const uint8_t* __restrict__ code, const Staged* __restrict__ st,
-                                                           const uint16_t* __restrict__ tlist,
-                                                           const uint32_t* __restrict__ tinfo) {
-  const int n = out.n;
-  // block -> (tile, part): XCD x takes tiles x, x + kXcds, ..., each as
-  // kInteractParts consecutive blocks of its share
-  const int x = (int)(blockIdx.x % kXcds);
-  const int inXcd = (int)(blockIdx.x / kXcds);
-  const int tile = (inXcd / kInteractParts) * kXcds + x;
-  const int tile0 = tile * kClassifyTile;
-  if (tile0 >= n) return;
-  const uint32_t* info = tinfo + (size_t)tile * kTileInfo;
-  const uint32_t chunk = (uint32_t)((inXcd % kInteractParts) * kInteractWaves + (threadIdx.x >> 6));
+__device__ void InteractTileChunk(const DevTables<R>& T, uint64_t seed, hepem_tracks& out, const hepem_step_out& res, int n,
+                                  const uint8_t* __restrict__ code, const Staged* __restrict__ st,
+                                  const uint16_t* __restrict__ tlist, const uint32_t* __restrict__ info, int tile0,
+                                  uint32_t chunk) {
   if (chunk >= info[2 * kNumProc + 1]) return;
   int q = 0;
   while (q + 1 < kNumProc && chunk >= info[kNumProc + 2 + q]) ++q;
@@ -960,6 +914,18 @@ __global__ void __launch_bounds__(kBlock) k_interact_tiles(DevTables<R> T, uint6
   const Staged r = st[i];
   if (q + 1 < kPConv) ElectronInteract<R>(T, seed, out, res, n, i, r, c);
   else GammaInteract<R>(T, seed, out, res, n, i, r, c);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_interact_tiles(DevTables<R> T, uint64_t seed, hepem_tracks out, hepem_step_out res,
+                                                           const uint8_t* __restrict__ code, const Staged* __restrict__ st,
+                                                           const uint16_t* __restrict__ tlist,
+                                                           const uint32_t* __restrict__ tinfo) {
+  const int n = out.n;
+  uint32_t tile, chunk;
+  if (!TileChunk((uint32_t)((n + kClassifyTile - 1) / kClassifyTile), tile, chunk)) return;
+  InteractTileChunk<R>(T, seed, out, res, n, code, st, tlist, tinfo + (size_t)tile * kTileInfo, (int)tile * kClassifyTile,
+                       chunk);
 }
 
 // ---- batched stepping ---------------------------------------------------------------------------
@@ -1029,12 +995,9 @@ __global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched(DevTables<R> T, 
 
 // Two-stage form of k_step_batched: stage 1 over every batch (block c takes
 // chunk c, as k_step_batched), recording the process code and staged record of
-// slot i of batch b at global index qoff_b + i; k_classify_proc_batched sorts
-// each batch's pending slots by process (region p of batch b at
-// pqueue[kNumProc*qoff_b + p*n_b], counts at pcounters[32*b + kProcCountBase + p + 1]);
-// k_interact_batched samples the final states, block c taking the entries
-// of chunk c of the 256-entry chunks of every batch (so the batch, and its
-// descriptor, is uniform over the block).
+// slot i of batch b at global index qoff_b + i; k_classify_tiles_batched and
+// k_interact_tiles_batched then sort and sample them tile by tile, as the
+// single-batch step does (tiles numbered across the batches).
 template <typename R, int S>
 __global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables<R> T, uint64_t seed,
                                                                        const BatchDev* __restrict__ d, int nbatch,
@@ -1064,46 +1027,32 @@ __global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables
                                : StepGammaSelect(T, seed, in, out, res, st + qoff, n, i);
 }
 
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_proc_batched(const BatchDev* __restrict__ d, int nbatch,
-                                                                  const uint8_t* __restrict__ code,
-                                                                  uint32_t* __restrict__ pqueue,
-                                                                  uint32_t* __restrict__ pcounters) {
+// The tile-local process sort over all batches of a plan: tiles are numbered
+// across the batches (BatchDev::tile0), each batch's tile list and codes sit
+// at its queue offset.
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles_batched(const BatchDev* __restrict__ d, int nbatch,
+                                                                           const uint8_t* __restrict__ code,
+                                                                           uint16_t* __restrict__ tlist,
+                                                                           uint32_t* __restrict__ tinfo) {
   const uint32_t t = blockIdx.x;
   const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
   const uint32_t qoff = d[b].qoff;
-  ClassifyProcTile(d[b].in.n, code + qoff, pqueue + (size_t)kNumProc * qoff, pcounters + 32 * b, (int)(t - d[b].tile0));
+  ClassifyTileLayout(d[b].in.n, code + qoff, tlist + qoff, tinfo + (size_t)t * kTileInfo, (int)(t - d[b].tile0));
 }
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_interact_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
-                                                             int nbatch, uint32_t nchunks, const uint8_t* __restrict__ code,
-                                                             const Staged* __restrict__ st,
-                                                             const uint32_t* __restrict__ pqueue,
-                                                             const uint32_t* __restrict__ pcounters) {
-  constexpr int S = 2;  // 256-entry chunks
-  const uint32_t cidx = blockIdx.x;
-  if (cidx >= nchunks) return;
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, cidx, [](const BatchDev& x) { return x.chunk0[S]; }));
+__global__ void __launch_bounds__(kBlock) k_interact_tiles_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
+                                                                   int nbatch, uint32_t ntiles, const uint8_t* __restrict__ code,
+                                                                   const Staged* __restrict__ st,
+                                                                   const uint16_t* __restrict__ tlist,
+                                                                   const uint32_t* __restrict__ tinfo) {
+  uint32_t tile, chunk;
+  if (!TileChunk(ntiles, tile, chunk)) return;
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, tile, [](const BatchDev& x) { return x.tile0; }));
   hepem_tracks out = d[b].out;
-  const hepem_step_out res = d[b].res;
-  const int n = out.n;
   const uint32_t qoff = d[b].qoff;
-  const uint32_t* cnt = pcounters + 32 * b;
-  const int j = (int)(cidx - d[b].chunk0[S]) * kBlock + (int)threadIdx.x;
-  uint32_t first = 0;
-  int p = -1;
-#pragma unroll
-  for (int q = 0; q < kNumProc; ++q) {
-    const uint32_t c = cnt[kProcCountBase + q + 1];
-    if (p < 0 && (uint32_t)j < first + c) p = q;
-    if (p < 0) first += c;
-  }
-  if (p < 0) return;
-  const int i = (int)pqueue[(size_t)kNumProc * qoff + (size_t)p * n + (j - first)];
-  const uint8_t c = code[qoff + i];
-  const Staged r = st[qoff + i];
-  if (p + 1 < kPConv) ElectronInteract<R>(T, seed, out, res, n, i, r, c);
-  else GammaInteract<R>(T, seed, out, res, n, i, r, c);
+  InteractTileChunk<R>(T, seed, out, d[b].res, out.n, code + qoff, st + qoff, tlist + qoff, tinfo + (size_t)tile * kTileInfo,
+                       (int)(tile - d[b].tile0) * kClassifyTile, chunk);
 }
 
 // the five config-1 quantities of query i, whose inputs are (mc, E, lnE)
