@@ -331,8 +331,7 @@ struct hepem_ctx {
   uint64_t workBytes = 0;
   int lookupStaging = 1;  // config-1 lookups read the tables from LDS where they fit
   int lookupGrid[2][2] = {{0, 0}, {0, 0}};  // [FP32 form][particle]: resident blocks of the staged kernel (0: not yet known)
-  int processSort = HEPEM_SORT_AUTO;  // sort pending interactions by process (two stages): HEPEM_SORT_*;
-                                      // by default auto in the FP64 form, off in the FP32 form (measured faster)
+  int processSort = HEPEM_SORT_AUTO;  // sort pending interactions by process (two stages): HEPEM_SORT_*
   // per couple: e-, e+, gamma production thresholds and the region's apply-cuts flag
   std::vector<double> cuts;
   std::vector<uint8_t> applyCuts;
@@ -364,7 +363,7 @@ extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p
   if (!c) return HEPEM_ERR_ALLOC;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HEPEM_ERR_HIP; }
   c->precision = precision;
-  c->processSort = precision == HEPEM_FP64 ? HEPEM_SORT_AUTO : HEPEM_SORT_OFF;
+  c->processSort = HEPEM_SORT_AUTO;
   {
     const G4HepEmMatCutData* mc = d->fTheMatCutData;
     c->cuts.resize(3 * (size_t)mc->fNumMatCutData);
@@ -529,7 +528,8 @@ extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
 namespace {
 // whether a step or Perform over n slots sorts its pending interactions by process
 bool SortsProcesses(const hepem_ctx* c, int64_t n) {
-  return c->processSort == HEPEM_SORT_ON || (c->processSort == HEPEM_SORT_AUTO && n >= HEPEM_SORT_AUTO_MIN_TRACKS);
+  if (c->processSort != HEPEM_SORT_AUTO) return c->processSort == HEPEM_SORT_ON;
+  return c->precision == HEPEM_FP64 || n < HEPEM_SORT_AUTO_FP32_MAX_TRACKS;
 }
 }  // namespace
 

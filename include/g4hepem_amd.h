@@ -184,15 +184,15 @@ int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
                     hepem_step_out* res, void* work, void* hip_stream);
 /* hepem_step_run, batch plans and the split Perform calls (hepem_perform,
  * hepem_*_perform) in one fused kernel (HEPEM_SORT_OFF), with the process sort
- * (HEPEM_SORT_ON), or with the sort for calls of at least
- * HEPEM_SORT_AUTO_MIN_TRACKS slots (HEPEM_SORT_AUTO: below that the two extra
- * launches cost more than the sort saves). Results are identical; only the
- * speed differs. Default: HEPEM_SORT_AUTO for HEPEM_FP64, HEPEM_SORT_OFF for
- * HEPEM_FP32T (the faster choices on MI355X). */
+ * (HEPEM_SORT_ON), or with the faster of the two for the call as measured on
+ * MI355X (HEPEM_SORT_AUTO, the default): the HEPEM_FP64 form sorts at every
+ * size, the HEPEM_FP32T form sorts calls of fewer than
+ * HEPEM_SORT_AUTO_FP32_MAX_TRACKS slots. Results are identical; only the
+ * speed differs. */
 #define HEPEM_SORT_OFF  0
 #define HEPEM_SORT_ON   1
 #define HEPEM_SORT_AUTO 2
-#define HEPEM_SORT_AUTO_MIN_TRACKS 32768
+#define HEPEM_SORT_AUTO_FP32_MAX_TRACKS 65536
 int  hepem_ctx_set_process_sort(hepem_ctx* ctx, int mode);
 /* A speed switch of hepem_lookup_eloss_xsec, like the process sort above:
  * the particle's loss and cross-section tables staged in LDS (on = 1, the
