@@ -120,3 +120,19 @@ def test_batched_graph_replay(ctx64, dev):
                 assert same_bits(got_r[f], eager[k][1][f]), (rep, k, f)
             assert same_bits(got_t["ekin"], eager[k][0]["ekin"]), (rep, k)
         assert plan.rejected(2) == 1
+
+
+@pytest.mark.parametrize("sizes", [[1, 63, 64, 65, 1000, 4097, 30000, 0, 257],
+                                   [70000, 5, 20000]],
+                         ids=["two-stage", "fused"])
+def test_batched_equals_unbatched_fp32(tables2, dev, sizes):
+    """The FP32-table form of a batch plan (the automatic sort choice takes the
+    two-stage kernels below 65536 tracks in all and the fused kernel above) is
+    bit-identical to hepem_step on each batch."""
+    from g4hepem_amd import HEPEM_FP32T, BatchPlan, Context
+    ctx32 = Context(tables2, HEPEM_FP32T)
+    host, batches = make_batches(sizes, dev, seed0=990)
+    plan = BatchPlan(ctx32, batches)
+    plan.run(SEED)
+    torch.cuda.synchronize()
+    check_against_unbatched(ctx32, host, batches, dev, "fp32")
