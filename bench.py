@@ -197,6 +197,7 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=Non
     mean, rms = ev.mean(0), ev.std(0)
     out.update({"mean_edep_total_MeV": float(ev.sum(1).mean()), "mean_leak_MeV": float(g["leak_event"].mean() / 1e9),
                 "layer_mean_MeV": [round(float(v), 4) for v in mean], "layer_rms_MeV": [round(float(v), 4) for v in rms]})
+    head32 = None
     if ctx32 is not None:
         g32, _, o32 = _timed_showers(ctx32, su, nevents, seed)
         e32 = g32["edep_event"] / 1e9
@@ -205,6 +206,7 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=Non
                     "layer_mean_MeV": [round(float(v), 4) for v in e32.mean(0)],
                     "layer_rms_MeV": [round(float(v), 4) for v in e32.std(0)]})
         out["fp32_table_form"] = o32
+        head32 = g32["edep_event"][:cpu_events].copy() if cpu_events > 0 else None
         del g32
     if woodcock:
         gw, _, ow = _timed_showers(ctx, _slab_setup(0.2), nevents, seed)
@@ -226,6 +228,9 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=Non
         out["cpu_harness"] = {"events": cpu_events, "wall_s": cwall, "track_steps_per_s": steps / cwall,
                               "chi2": float((((a.mean(0) - b.mean(0)) ** 2)[m] / v[m]).sum()), "ndf": int(m.sum()),
                               "identical_events": float(np.all(g["edep_event"][:cpu_events] == dep, axis=1).mean())}
+        if head32 is not None:
+            # the FP32-table form's first events against the CPU harness's (FP64) same events
+            out["cpu_harness"]["fp32_table_form_vs_cpu"] = _chi2_means(head32 / 1e9, dep / 1e9)
     return out
 
 
