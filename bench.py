@@ -382,7 +382,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline FP64 step only (for profiling)")
     ap.add_argument("--shower-events", type=int, default=10000, help="config-5 events (0: skip)")
-    ap.add_argument("--shower-cpu-events", type=int, default=100, help="config-5 events for the CPU harness")
+    ap.add_argument("--shower-cpu-events", type=int, default=1000, help="config-5 events for the CPU harness")
     ap.add_argument("--shower-forms", default="fp64,fp32,woodcock",
                     help="config-5 runs: the FP64 form (always), the FP32-table form, FP64 with Woodcock tracking")
     ap.add_argument("--lookups-only", action="store_true",
