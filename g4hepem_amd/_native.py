@@ -112,6 +112,7 @@ def lib() -> C.CDLL:
     i32, u64, vp, dp, ip = C.c_int32, C.c_uint64, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
     L.hepem_tables_build.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.c_double, C.POINTER(vp), C.POINTER(vp)]
     L.hepem_tables_free.argtypes = [vp, vp]
+    L.hepem_tables_set_nuclear.argtypes = [vp, C.c_double, C.c_double]
     L.hepem_tables_serialize.argtypes = [vp, vp, vp, C.POINTER(u64)]
     L.hepem_tables_from_json.argtypes = [C.c_char_p, u64, C.POINTER(vp), C.POINTER(vp)]
     L.hepem_tables_to_json.argtypes = [vp, vp, vp, C.POINTER(u64)]
@@ -142,7 +143,7 @@ def lib() -> C.CDLL:
         getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), vp, vp]
     for name in ("hepem_perform", "hepem_electron_perform", "hepem_gamma_perform"):
         getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), C.POINTER(StepOut), vp, vp]
-    for name in ("hepem_tables_build", "hepem_tables_free", "hepem_tables_serialize", "hepem_tables_from_json",
+    for name in ("hepem_tables_build", "hepem_tables_free", "hepem_tables_set_nuclear", "hepem_tables_serialize", "hepem_tables_from_json",
                  "hepem_tables_to_json", "hepem_ctx_create",
                  "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_step",
                  "hepem_step_prepare", "hepem_step_run", "hepem_step_rejected", "hepem_ctx_set_process_sort",
@@ -195,6 +196,13 @@ class Tables:
         check("hepem_tables_from_json", lib().hepem_tables_from_json(text, len(text), C.byref(d), C.byref(p)))
         t.data, t.pars = d, p
         return t
+
+    def set_nuclear(self, enuc_scale: float, gnuc_scale: float) -> "Tables":
+        """Fill the e-/e+ nuclear and gamma-nuclear cross sections (scales in
+        units of 1/X0 times a fixed energy shape; 0 clears them), so that the
+        nuclear branches of the stepping are reached. Returns self."""
+        check("hepem_tables_set_nuclear", lib().hepem_tables_set_nuclear(self.data, float(enuc_scale), float(gnuc_scale)))
+        return self
 
     def to_json(self) -> str:
         """The tables as a G4HepEmState document in the reference's JSON schema."""

@@ -928,6 +928,67 @@ extern "C" int hepem_tables_build(const char* const* names, int nmat, double cut
   return HEPEM_OK;
 }
 
+// Lepto- and photo-nuclear macroscopic cross sections in units of the
+// inverse radiation length of the material, for tables that exercise the
+// nuclear branches. e-/e+: slowly rising above the 100 MeV grid start.
+// gamma: a giant-dipole bump at 13 MeV (threshold 5 MeV) over a plateau.
+namespace {
+double ENucShape(double E) { return E < 100.0 ? 0.0 : 1.0 + 0.25 * std::log(E / 100.0); }
+double GNucShape(double E) {
+  if (E <= 5.0) return 0.0;
+  const double t = (E - 5.0) * (E - 5.0);
+  const double onset = t / (t + 4.0);
+  const double bump = 25.0 / ((E - 13.0) * (E - 13.0) + 25.0);
+  return onset * (0.2 + bump);
+}
+}  // namespace
+
+extern "C" int hepem_tables_set_nuclear(G4HepEmData* hd, double enuc_scale, double gnuc_scale) {
+  if (!hd || !hd->fTheMaterialData || !hd->fTheElectronData || !hd->fThePositronData || !hd->fTheGammaData)
+    return HEPEM_ERR_INVALID;
+  if (!(enuc_scale >= 0.0) || !(gnuc_scale >= 0.0) || !std::isfinite(enuc_scale) || !std::isfinite(gnuc_scale))
+    return HEPEM_ERR_INVALID;
+  const G4HepEmMaterialData* md = hd->fTheMaterialData;
+  const int nmat = md->fNumMaterialData;
+  for (G4HepEmElectronData* ed : {hd->fTheElectronData, hd->fThePositronData}) {
+    const int n = ed->fENucEnergyGridSize;
+    if (ed->fNumMaterials != nmat || n < 2 || !ed->fENucEnergyGrid || !ed->fENucMacXsecData) return HEPEM_ERR_INVALID;
+    std::vector<double> y(n), sd(n);
+    for (int im = 0; im < nmat; ++im) {
+      const double invX0 = 1.0 / md->fMaterialData[im].fRadiationLength;
+      for (int i = 0; i < n; ++i) y[i] = enuc_scale * invX0 * ENucShape(ed->fENucEnergyGrid[i]);
+      SplineSD(n, ed->fENucEnergyGrid, y.data(), sd.data());
+      double* d = ed->fENucMacXsecData + (size_t)im * 2 * n;
+      for (int i = 0; i < n; ++i) {
+        d[2 * i] = y[i];
+        d[2 * i + 1] = sd[i];
+      }
+    }
+  }
+  // window 2 stores [E, (total, total''), (conv, ''), (compton, ''), (PE, '')]
+  // per node; the gamma-nuclear part is what the total holds beyond the sum
+  // of the other three
+  G4HepEmGammaData* gd = hd->fTheGammaData;
+  if (gd->fNumMaterials != nmat || !gd->fMacXsecData || gd->fEGridSize2 < 2) return HEPEM_ERR_INVALID;
+  const int n2 = gd->fEGridSize2;
+  std::vector<double> x(n2), y(n2), sd(n2);
+  for (int im = 0; im < nmat; ++im) {
+    const double invX0 = 1.0 / md->fMaterialData[im].fRadiationLength;
+    double* d = gd->fMacXsecData + (size_t)im * gd->fDataPerMat + gd->fNumData0 + gd->fNumData1;
+    for (int i = 0; i < n2; ++i) {
+      const double* r = d + 9 * i;
+      x[i] = r[0];
+      y[i] = r[3] + r[5] + r[7] + gnuc_scale * invX0 * GNucShape(r[0]);
+    }
+    SplineSD(n2, x.data(), y.data(), sd.data());
+    for (int i = 0; i < n2; ++i) {
+      d[9 * i + 1] = y[i];
+      d[9 * i + 2] = sd[i];
+    }
+  }
+  return HEPEM_OK;
+}
+
 extern "C" int hepem_tables_free(G4HepEmData* hd, G4HepEmParameters* pars) {
   if (pars) { std::free(pars->fParametersPerRegion); std::free(pars); }
   if (!hd) return HEPEM_OK;

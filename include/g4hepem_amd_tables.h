@@ -30,6 +30,21 @@ extern "C" {
 int hepem_tables_build(const char* const* material_names, int nmat, double prod_cut_mm,
                        struct G4HepEmData** out_data, struct G4HepEmParameters** out_pars);
 
+/* Fill the lepto-nuclear (e-/e+) and photo-nuclear macroscopic cross
+ * sections of existing tables. The offline builder leaves them zero, as
+ * Geant4 does without hadronic physics; tables in which they are non-zero
+ * reach the nuclear branches of the run-time code:
+ *   e-/e+: GetMacXSecNuclear and the winner-3 step limit and delta check
+ *          (G4HepEm/G4HepEmRun/include/G4HepEmElectronManager.icc:85,418-421,534-541);
+ *   gamma: the total cross section of window 2 exceeds the sum of
+ *          conversion, Compton and PE, so SampleInteraction can return 3
+ *          (G4HepEmGammaManager.icc:199-214).
+ * Cross sections are enuc_scale (gnuc_scale) times a fixed smooth energy
+ * shape in units of 1/X0 of the material: e-/e+ 1 + ln(E/100 MeV)/4 above
+ * 100 MeV; gamma a 13 MeV giant-dipole bump over a plateau above 5 MeV.
+ * Spline second derivatives are recomputed. Scales of 0 clear them. */
+int hepem_tables_set_nuclear(struct G4HepEmData* data, double enuc_scale, double gnuc_scale);
+
 /* Free everything hepem_tables_build allocated. */
 int hepem_tables_free(struct G4HepEmData* data, struct G4HepEmParameters* pars);
 

@@ -15,9 +15,13 @@ and the per-track Philox streams, and stores inputs and outputs:
 * one step of tracks with 64-bit stream selectors (nonzero rng_stream);
 * (golden_geom_v1.npz) a 3-step chain with a geometry between HowFar and
   Perform: each track's geometrical step is cut at a given distance, a cut
-  step ends on a boundary and the next step starts from there.
+  step ends on a boundary and the next step starts from there;
+* (golden_nuc_v1.npz) the 4-material set with non-zero e-/e+ nuclear and
+  gamma-nuclear cross sections (Tables.set_nuclear(0.5, 0.2)): electron and
+  gamma probes, one step in two energy regimes and a 3-step chain, in which
+  e-/e+ winner 3 and gamma process 3 occur.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [all|nuc]
 """
 from __future__ import annotations
 
@@ -36,6 +40,8 @@ from oracle.oracle_py import Ref  # noqa: E402
 
 OUT = Path(__file__).resolve().parent / "golden_v1.npz"
 OUT_GEOM = Path(__file__).resolve().parent / "golden_geom_v1.npz"
+OUT_NUC = Path(__file__).resolve().parent / "golden_nuc_v1.npz"
+NUC_SCALES = (0.5, 0.2)
 SET1 = ["G4_Pb", "G4_lAr"]
 SET2 = ["G4_PbWO4", "G4_Pb", "G4_lAr", "G4_Si"]
 CUT = 0.7
@@ -140,5 +146,45 @@ def geometry_vectors(ref, t2, n=1024, nsteps=3):
     print(f"wrote {OUT_GEOM} ({OUT_GEOM.stat().st_size / 1024:.0f} KiB)")
 
 
+def nuclear_vectors(ref):
+    """Tables with nuclear cross sections: probes, steps and a short chain."""
+    t = Tables(SET2, CUT).set_nuclear(*NUC_SCALES)
+    g = {"digest_nuc": np.array(t.digest())}
+    for pt, name in ((0, "e"), (1, "p")):
+        imc, e, le = lookup_queries(512, 800 + pt, len(SET2), lnemin=math.log(10.0), lnemax=math.log(1e8), margin=0.03)
+        g[f"pe_{name}_imc"], g[f"pe_{name}_ekin"], g[f"pe_{name}_lekin"] = imc, e, le
+        g[f"pe_{name}_out"] = np.stack([ref.probe_electron(t, pt, w, imc, e, le) for w in range(6)])
+    imc, e, le = lookup_queries(2048, 810, len(SET2), lnemin=math.log(1e-4), lnemax=math.log(1e8), margin=0.01)
+    u = np.random.Generator(np.random.Philox(811)).random(2048)
+    xs, pe, pid = ref.probe_gamma(t, imc, e, le, u)
+    g.update(gm_imc=imc, gm_ekin=e, gm_lekin=le, gm_u=u, gm_xs=xs, gm_pe=pe, gm_pid=pid)
+    for r, (lo, hi) in enumerate(((math.log(0.1), math.log(1e4)), (math.log(1e2), math.log(1e6)))):
+        tr = step_tracks(1024, 820 + r, len(SET2), lnemin=lo, lnemax=hi, first_id=80000 + 10000 * r)
+        to, res = ref.step(t, STEP_SEED + 4, tr, 8)
+        for k, v in tr.items():
+            g[f"st{r}_in_{k}"] = v
+        for k, v in to.items():
+            g[f"st{r}_out_{k}"] = v
+        for k, v in res.items():
+            g[f"st{r}_res_{k}"] = v
+    tr = step_tracks(512, 830, len(SET2), lnemin=math.log(1e2), lnemax=math.log(1e5), first_id=100000)
+    tr["on_boundary"][:] = (np.random.Generator(np.random.Philox(831)).random(512) < 0.2).astype(np.uint8)
+    for k, v in tr.items():
+        g[f"ch_in_{k}"] = v
+    cur = tr
+    for s in range(3):
+        cur, res = ref.step(t, STEP_SEED + 5, cur, 8)
+        for k, v in res.items():
+            g[f"ch{s}_res_{k}"] = v
+        for k in ("ekin", "dir", "num_ia_left", "rng_ctr", "rng_gauss", "rng_has_gauss", "msc_init_range",
+                  "msc_dyn_range_factor", "msc_tlimit_min", "msc_first_step"):
+            g[f"ch{s}_out_{k}"] = cur[k]
+    np.savez_compressed(OUT_NUC, **g)
+    print(f"wrote {OUT_NUC} ({OUT_NUC.stat().st_size / 1024:.0f} KiB); nuclear set {t.digest()[:16]}")
+
+
 if __name__ == "__main__":
-    main()
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what == "all":
+        main()
+    nuclear_vectors(Ref())
