@@ -3,6 +3,7 @@
 // and the launch wrappers of the lookup and stepping kernels.
 #include "hepem_kernels.hh"
 #include "hepem_internal.hh"
+#include "tables/hepem_validate.hh"
 
 #include <hip/hip_runtime.h>
 
@@ -315,7 +316,7 @@ bool DataComplete(const G4HepEmData* d, const G4HepEmParameters* p) {
     if (c.fHepEmMatIndex < 0 || c.fHepEmMatIndex >= d->fTheMaterialData->fNumMaterialData) return false;
     if (c.fG4RegionIndex < 0 || c.fG4RegionIndex >= p->fNumRegions) return false;
   }
-  return d->fTheElementData->fMaxZet >= 1 && d->fTheElementData->fMaxZet <= 120;
+  return HepemTablesValid(d);
 }
 
 }  // namespace
@@ -484,7 +485,9 @@ bool ResOk(const hepem_step_out* r, int n) {
 
 // work layout (each part at 256-byte alignment): 64 counters | particle queue
 // (n x u32) | process code per slot (n x u8) | staged record per slot
-// (n x 64 bytes) | process queue (kNumProc regions of n x u32)
+// (n x 64 bytes) | tile list (n x u16: each pending slot's offset within its
+// tile, laid out by process) | tile info (kTileInfo words per tile of
+// kClassifyTile slots)
 namespace {
 constexpr uint64_t kCountersBytes = 256;
 uint64_t Al(uint64_t b) { return (b + 255) & ~255ull; }

@@ -21,6 +21,7 @@
 
 #include "g4hepem_amd.h"
 #include "g4hepem_amd_tables.h"
+#include "hepem_validate.hh"
 
 namespace {
 
@@ -959,6 +960,8 @@ extern "C" int hepem_tables_from_json(const char* text, uint64_t len, G4HepEmDat
       return false;
     });
     if (!in.atEnd() || !state || !any || !data || !pars) throw BadJson{};
+    // every index the tables hold must point inside them
+    if (!HepemTablesValid(data)) throw BadJson{};
   } catch (...) {
     hepem_tables_free(data, pars);
     return HEPEM_ERR_INVALID;

@@ -163,8 +163,9 @@ int  hepem_lookup_eloss_xsec(const hepem_ctx* ctx, int ptype, int32_t n,
 
 /* ---- stepping ------------------------------------------------------------
  * One step of every track of the batch. `out` may alias `in`. `work` must be
- * at least hepem_step_work_bytes(n) bytes of device memory (counters and the
- * particle-type queue);
+ * at least hepem_step_work_bytes(n) bytes of device memory (step counters,
+ * the particle-type queue, a process code and a staged record per slot, and
+ * the tile-local process layout of the pending interactions);
  * pass NULL to let the context use its own buffer (then calls on one context
  * must not overlap). */
 int  hepem_step(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,

@@ -19,7 +19,7 @@
  * initialisers (grid sizes); here they are plain `int` members at the same
  * offsets and the table builder writes the reference's values into them.
  * The offsets are checked member by member against the reference headers
- * by oracle/ref/layout_check.cc.
+ * by the static_asserts of oracle/ref/ref_harness.cc.
  *
  * Units follow Geant4's internal system: MeV, mm, 1/mm, MeV/mm.
  * The header has no includes beyond <stdbool.h> in C so that it can be

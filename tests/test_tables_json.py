@@ -90,6 +90,80 @@ def test_malformed_documents_rejected(tables1, case):
         Tables.from_json(bad)
 
 
+INDEX_CASES = ["ioni_block_extent", "brem_block_extent", "brem_selector_start", "brem_selector_elements", "sb_rows",
+               "z_above_max", "electron_materials", "gamma_cut_index", "conv_selector_start", "gamma_block_size"]
+
+
+def corrupt_indices(doc: dict, case: str) -> None:
+    """One index-bearing value of a document (4-material set) pointed outside
+    the array it indexes, or a count made inconsistent with the others."""
+    d = doc["fData"]
+    el, sb, gm = d["fTheElectronData"], d["fTheSBTableData"], d["fTheGammaData"]
+    xs = el["fResMacXSecData"]
+    s1 = el["fResMacXSecStartIndexPerMatCut"][1]
+    if case == "ioni_block_extent":
+        xs[s1] = float(len(xs))                        # ioni block runs past the array
+    elif case == "brem_block_extent":
+        last = el["fResMacXSecStartIndexPerMatCut"][-1]
+        xs[last + 5 + 3 * int(xs[last])] = 1.0e6       # brem block behind the last ioni block
+    elif case == "brem_selector_start":
+        el["fElemSelectorBremSBStartIndexPerMatCut"][0] = len(el["fElemSelectorBremSBData"]) + 10
+    elif case == "brem_selector_elements":
+        st = el["fElemSelectorBremRBStartIndexPerMatCut"][0]
+        el["fElemSelectorBremRBData"][st + 1] = 7.0    # more elements than PbWO4 has
+    elif case == "sb_rows":
+        z = d["fTheMaterialData"]["fMaterialData"][1]["fElementVect"][0]
+        st = sb["fSBStartTablesStartPerZ"][z]
+        sb["fSBTableData"][st + 2] = 64.0               # rows up to the last energy: past the data
+        sb["fSBTableData"][st + 1] = 0.0
+    elif case == "z_above_max":
+        d["fTheMaterialData"]["fMaterialData"][2]["fElementVect"][0] = 150
+    elif case == "electron_materials":
+        el["fNumMaterials"] = el["fNumMaterials"] + 1
+        el["fENucMacXsecData"] += el["fENucMacXsecData"][: 2 * len(el["fENucEnergyGrid"])]
+        el["fTr1MacXSecData"] += el["fTr1MacXSecData"][: 2 * len(el["fELossEnergyGrid"])]
+    elif case == "gamma_cut_index":
+        sb["fGammaCutIndices"][0] = 999
+    elif case == "conv_selector_start":
+        gm["fElemSelectorConvStartIndexPerMat"][0] = len(gm["fElemSelectorConvData"])
+    elif case == "gamma_block_size":
+        gm["fNumData1"] = gm["fNumData1"] + 3
+    else:
+        raise ValueError(case)
+
+
+@pytest.mark.parametrize("case", INDEX_CASES)
+def test_out_of_range_indices_rejected(tables2, case):
+    """Documents whose sizes fit together but whose stored indices point
+    outside their arrays are refused at load (nothing is read out of bounds
+    later, on the host when the context packs the tables or on the device)."""
+    doc = json.loads(tables2.to_json())
+    corrupt_indices(doc, case)
+    with pytest.raises(HepEmError):
+        Tables.from_json(json.dumps(doc))
+
+
+def test_context_refuses_inconsistent_tables():
+    """hepem_ctx_create applies the same check to tables handed over in memory
+    (before it looks for a device, so this runs without a GPU)."""
+    from g4hepem_amd.data_layout import view
+    t = Tables(["G4_PbWO4", "G4_Pb", "G4_lAr", "G4_Si"], 0.7)
+    h = C.c_void_p()
+    assert lib().hepem_ctx_create(t.data, t.pars, 0, C.byref(h)) in (0, -5)
+    if h.value:
+        lib().hepem_ctx_destroy(h)
+    m = view(t).fTheMaterialData.contents.fMaterialData[1]
+    z = m.fElementVect[0]
+    m.fElementVect[0] = 121
+    assert lib().hepem_ctx_create(t.data, t.pars, 0, C.byref(h)) == -1
+    m.fElementVect[0] = z
+    e = view(t).fTheElectronData.contents
+    s = e.fResMacXSecStartIndexPerMatCut[2]
+    e.fResMacXSecStartIndexPerMatCut[2] = e.fResMacXSecNumData - 3
+    assert lib().hepem_ctx_create(t.data, t.pars, 0, C.byref(h)) == -1
+    e.fResMacXSecStartIndexPerMatCut[2] = s
+
+
 @needs_ref
 def test_reference_writer_loads_here(tables2):
     ref = RefJson()
