@@ -63,6 +63,14 @@ class Context:
         N.check("hepem_step", N.lib().hepem_step(self.h, C.c_uint64(seed), C.byref(si), C.byref(so), C.byref(rr), wp,
                                                   _stream_handle(stream)))
 
+    def step_noalloc(self, seed: int, tin: TrackSoA, tout: TrackSoA, res: StepOutSoA, stream=None) -> None:
+        """hepem_step_noalloc: the step with the library's fixed per-GPU work
+        array (at most NOALLOC_MAX_TRACKS tracks; calls on one GPU must not
+        overlap)."""
+        si, so, rr = tin.struct(), tout.struct(), res.struct()
+        N.check("hepem_step_noalloc", N.lib().hepem_step_noalloc(self.h, C.c_uint64(seed), C.byref(si), C.byref(so),
+                                                                  C.byref(rr), _stream_handle(stream)))
+
     def step_prepare(self, tin: TrackSoA, work, stream=None) -> None:
         si = tin.struct()
         N.check("hepem_step_prepare", N.lib().hepem_step_prepare(self.h, C.byref(si), work.data_ptr(),

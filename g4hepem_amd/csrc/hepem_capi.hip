@@ -9,6 +9,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -321,6 +322,52 @@ bool DataComplete(const G4HepEmData* d, const G4HepEmParameters* p) {
 
 }  // namespace
 
+// ---- the work buffer of the allocation-free step (hepem_step_noalloc) --------------------------
+// One array per GPU, in the code object's device memory: the step's counters,
+// particle-type queue, process codes, staged records and tile layout for up to
+// HEPEM_NOALLOC_MAX_TRACKS slots (about 17.7 MiB), held until the process ends.
+// In host code `scratch` names a host-side stand-in: only the address that
+// hipGetSymbolAddress returns for the current device may reach a kernel.
+// work layout (each part at 256-byte alignment): 64 counters | particle queue
+// (n x u32) | process code per slot (n x u8) | staged record per slot
+// (n x 64 bytes) | tile list (n x u16: each pending slot's offset within its
+// tile, laid out by process) | tile info (kTileInfo words per tile of
+// kClassifyTile slots)
+namespace {
+constexpr uint64_t kCountersBytes = 256;
+constexpr uint64_t Al(uint64_t b) { return (b + 255) & ~255ull; }
+// bytes of the work buffer of a step over m slots
+constexpr uint64_t WorkBytes(uint64_t m) {
+  return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(2 * m) +
+         Al(4 * kTileInfo * ((m + kClassifyTile - 1) / kClassifyTile));
+}
+}  // namespace
+
+namespace {
+constexpr uint64_t kNoAllocScratchBytes = WorkBytes(HEPEM_NOALLOC_MAX_TRACKS);
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+void* gScratchPtr[kMaxDevices] = {};
+}  // namespace
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+// Device address of `scratch` on device `dev`, which must be the current
+// device; looked up once per device and kept.
+int ScratchAddress(int dev, void** ptr) {
+  *ptr = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return HEPEM_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratchPtr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HEPEM_ERR_HIP;
+    gScratchPtr[dev] = p;
+  }
+  *ptr = gScratchPtr[dev];
+  return HEPEM_OK;
+}
+}  // namespace
+
 struct hepem_ctx {
   int device = 0;
   int precision = HEPEM_FP64;
@@ -330,6 +377,8 @@ struct hepem_ctx {
   DevTables<float> t32;
   void* work = nullptr;       // internal work buffer for callers passing work == NULL
   uint64_t workBytes = 0;
+  void* scratch = nullptr;    // device address of `scratch` on this context's device (hepem_step_noalloc)
+  int scratchStatus = HEPEM_OK;
   int lookupStaging = 1;  // config-1 lookups read the tables from LDS where they fit
   int lookupGrid[2][2] = {{0, 0}, {0, 0}};  // [FP32 form][particle]: resident blocks of the staged kernel (0: not yet known)
   int processSort = HEPEM_SORT_AUTO;  // sort pending interactions by process (two stages): HEPEM_SORT_*
@@ -363,6 +412,7 @@ extern "C" int hepem_ctx_create(const G4HepEmData* d, const G4HepEmParameters* p
   hepem_ctx* c = new (std::nothrow) hepem_ctx();
   if (!c) return HEPEM_ERR_ALLOC;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return HEPEM_ERR_HIP; }
+  c->scratchStatus = ScratchAddress(c->device, &c->scratch);
   c->precision = precision;
   c->processSort = HEPEM_SORT_AUTO;
   {
@@ -483,14 +533,8 @@ bool ResOk(const hepem_step_out* r, int n) {
 }
 }  // namespace
 
-// work layout (each part at 256-byte alignment): 64 counters | particle queue
-// (n x u32) | process code per slot (n x u8) | staged record per slot
-// (n x 64 bytes) | tile list (n x u16: each pending slot's offset within its
-// tile, laid out by process) | tile info (kTileInfo words per tile of
-// kClassifyTile slots)
+// the parts of a work buffer, in the layout WorkBytes describes
 namespace {
-constexpr uint64_t kCountersBytes = 256;
-uint64_t Al(uint64_t b) { return (b + 255) & ~255ull; }
 struct WorkLayout {
   uint32_t* counters;
   uint32_t* queue;
@@ -516,11 +560,7 @@ struct WorkLayout {
 };
 }  // namespace
 
-extern "C" uint64_t hepem_step_work_bytes(int32_t n) {
-  const uint64_t m = (uint64_t)(n > 0 ? n : 0);
-  const uint64_t tiles = (m + kClassifyTile - 1) / kClassifyTile;
-  return kCountersBytes + Al(4 * m) + Al(m) + Al(sizeof(Staged) * m) + Al(2 * m) + Al(4 * kTileInfo * tiles);
-}
+extern "C" uint64_t hepem_step_work_bytes(int32_t n) { return WorkBytes((uint64_t)(n > 0 ? n : 0)); }
 
 extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
   if (!c || (mode != HEPEM_SORT_OFF && mode != HEPEM_SORT_ON && mode != HEPEM_SORT_AUTO)) return HEPEM_ERR_INVALID;
@@ -613,6 +653,19 @@ extern "C" int hepem_step(const hepem_ctx* c, uint64_t seed, const hepem_tracks*
   rc = hepem_step_prepare(c, in, work, stream);
   if (rc != HEPEM_OK) return rc;
   return hepem_step_run(c, seed, in, out, res, work, stream);
+}
+
+extern "C" int hepem_step_noalloc(const hepem_ctx* c, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
+                                  hepem_step_out* res, void* stream) {
+  if (!c || !TracksOk(in) || !TracksOk(out) || !ResOk(res, in->n) || out->n != in->n) return HEPEM_ERR_INVALID;
+  if (in->n > HEPEM_NOALLOC_MAX_TRACKS) return HEPEM_ERR_CAPACITY;
+  if (in->n == 0) return HEPEM_OK;
+  if (c->scratchStatus != HEPEM_OK) return c->scratchStatus;
+  // the same kernels as hepem_step; prepare clears the counters with a fill
+  // kernel, so the scratch's previous contents do not matter
+  const int rc = hepem_step_prepare(c, in, c->scratch, stream);
+  if (rc != HEPEM_OK) return rc;
+  return hepem_step_run(c, seed, in, out, res, c->scratch, stream);
 }
 
 namespace {

@@ -172,6 +172,25 @@ int  hepem_step(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
                 hepem_tracks* out, hepem_step_out* res, void* work, void* hip_stream);
 uint64_t hepem_step_work_bytes(int32_t n);
 
+/* hepem_step with no work buffer and no allocation of any kind. No counterpart
+ * in the reference: this entry point is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc. The step's counters, particle-type queue
+ * and staged interactions live in a fixed array that the library holds in
+ * each GPU's memory from load until the process ends
+ * (hepem_step_work_bytes(HEPEM_NOALLOC_MAX_TRACKS) bytes, about 17.7 MiB);
+ * its device address is looked up when a context is created on that GPU.
+ * Batches above HEPEM_NOALLOC_MAX_TRACKS tracks return HEPEM_ERR_CAPACITY.
+ * Every call on one GPU uses that GPU's one array: calls that overlap on a
+ * GPU (other streams, threads or contexts) race and must be serialised by the
+ * caller. Each call clears the counters it needs with a kernel (no memset),
+ * so what an earlier call left in the array does not matter. Results are
+ * bit-identical to hepem_step on the same inputs. The number of rejected
+ * tracks is not reported (hepem_step_rejected needs a work buffer). */
+#define HEPEM_NOALLOC_MAX_TRACKS 262144
+int  hepem_step_noalloc(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
+                        hepem_step_out* res, void* hip_stream);
+
 /* The two phases of hepem_step, for callers that time them separately:
  * hepem_step_prepare clears the step counters and compacts the track slots
  * by particle type into the queue held in `work` (e-/e+ first); hepem_step_run
