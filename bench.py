@@ -132,10 +132,32 @@ def config3_batches(seed: int, nbatch: int = 64):
     return out
 
 
+def verify_batches(ctx, batches, seed) -> dict:
+    """After the timed replays, outside the timed region: every batch's outputs
+    from the plan against hepem_step on the same batch and seed, bit for bit
+    (compared on the device)."""
+    import torch
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_outputs_differ
+    dev = batches[0][0]["ekin"].device
+    work = ctx.work_buffer(max(b[0].n for b in batches), dev)
+    bad = {}
+    for k, (tin, tout, res) in enumerate(batches):
+        t_ref, r_ref = TrackSoA(tin.n, "torch", dev), StepOutSoA(tin.n, "torch", dev)
+        ctx.step(seed, tin, t_ref, r_ref, work)
+        diff = step_outputs_differ(tin["ptype"], tout, res, t_ref, r_ref)
+        if diff:
+            bad[k] = diff
+        del t_ref, r_ref
+    torch.cuda.synchronize()
+    return {"bit_identical_to_unbatched": not bad, "batches_checked": len(batches),
+            "mismatches": {str(k): v for k, v in list(bad.items())[:4]}}
+
+
 def time_batched(ctx, batches, seed, reps):
     """A batch plan over `batches` (device SoA triples): tune the launch shape,
     capture one run into a hipGraph and time `reps` replays with HIP events on
-    the replay stream. Returns (ms per run, (block, grid))."""
+    the replay stream; then check the last replay's outputs against hepem_step
+    on each batch. Returns (ms per run, (block, grid), check)."""
     import torch
     from g4hepem_amd import BatchPlan
     plan = BatchPlan(ctx, batches)
@@ -158,13 +180,20 @@ def time_batched(ctx, batches, seed, reps):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
     plan.close()
-    return ms, shape
+    return ms, shape, verify_batches(ctx, batches, seed)
 
 
 def _chi2_means(a, b):
-    v = a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b)
-    m = v > 0
-    return {"chi2": float((((a.mean(0) - b.mean(0)) ** 2)[m] / v[m]).sum()), "ndf": int(m.sum())}
+    """Per-slab agreement of two runs over the same events (rows: events,
+    columns: slabs). The runs share seeds and streams, so their events are
+    paired and correlated: the statistic is the paired one, per slab the mean
+    of the per-event differences over its standard error, squared and summed
+    over the slabs whose differences vary. Independent runs give the same
+    expectation (ndf)."""
+    d = a - b
+    se2 = d.var(0, ddof=1) / len(d)
+    m = se2 > 0
+    return {"chi2": float(((d.mean(0) ** 2)[m] / se2[m]).sum()), "ndf": int(m.sum()), "test": "paired"}
 
 
 def _slab_setup(woodcock_min_ekin: float = 0.0):
@@ -223,10 +252,8 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=Non
         dep, leak, steps, _ = Oracle().slab_shower(tables, su, cpu_events, seed, nthreads=max(1, min(int(cores), 64)))
         cwall = time.perf_counter() - t0
         a, b = g["edep_event"][:cpu_events] / 1e9, dep / 1e9
-        v = a.var(0, ddof=1) / len(a) + b.var(0, ddof=1) / len(b)
-        m = v > 0
         out["cpu_harness"] = {"events": cpu_events, "wall_s": cwall, "track_steps_per_s": steps / cwall,
-                              "chi2": float((((a.mean(0) - b.mean(0)) ** 2)[m] / v[m]).sum()), "ndf": int(m.sum()),
+                              **_chi2_means(a, b),
                               "identical_events": float(np.all(g["edep_event"][:cpu_events] == dep, axis=1).mean())}
         if head32 is not None:
             # the FP32-table form's first events against the CPU harness's (FP64) same events
@@ -234,7 +261,7 @@ def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=Non
     return out
 
 
-def fp32_errors(r64: dict, t64: dict, r32: dict, t32: dict) -> dict:
+def fp32_errors(r64: dict, t64: dict, r32: dict, t32: dict, ptype: np.ndarray) -> dict:
     """Config 4's error report: the FP32-table step against the FP64 step on
     the same tracks and streams. Integer decisions as agreement fractions. A
     track that drew another number of random numbers (a rejection test that
@@ -257,9 +284,24 @@ def fp32_errors(r64: dict, t64: dict, r32: dict, t32: dict) -> dict:
         return float(abs(a[m].sum() - b[m].sum()) / abs(b[m].sum()))
 
     d64, d32 = t64["dir"].reshape(3, n), t32["dir"].reshape(3, n)
-    ddir = np.abs(d64 - d32).max(axis=0)[same]
+    ddir_all = np.abs(d64 - d32).max(axis=0)
+    ddir = ddir_all[same]
     s64, s32 = r64["sec_ekin"].reshape(2, n)[0], r32["sec_ekin"].reshape(2, n)[0]
     has64, has32 = r64["nsec"] >= 1, r32["nsec"] >= 1
+
+    # the same errors per particle kind and winner process, to trace the tails to a branch
+    is_g = ptype == 2
+    by_regime = {}
+    for kind, km in (("e-/e+", ~is_g), ("gamma", is_g)):
+        for w in np.unique(r64["winner"][km]):
+            m = same & km & (r64["winner"] == w)
+            if not m.any():
+                continue
+            key = f"{kind} winner {int(w)}"
+            by_regime[key] = {"tracks": int(m.sum()), "edep": rel(r32["edep"], r64["edep"], m),
+                              "post_step_ekin": rel(t32["ekin"], t64["ekin"], m),
+                              "true_step": rel(r32["true_step"], r64["true_step"], m),
+                              "post_step_dir_max_abs": float(ddir_all[m].max())}
     return {
         "winner_agreement": float((r64["winner"] == r32["winner"]).mean()),
         "nsec_agreement": float((r64["nsec"] == r32["nsec"]).mean()),
@@ -274,6 +316,7 @@ def fp32_errors(r64: dict, t64: dict, r32: dict, t32: dict) -> dict:
         "secondary0_ekin": rel(s32, s64, same & has64),
         "total_edep_rel": total(r32["edep"], r64["edep"]),
         "total_secondary0_ekin_rel": float(abs(s32[has32].sum() - s64[has64].sum()) / s64[has64].sum()),
+        "by_regime": by_regime,
     }
 
 
@@ -477,7 +520,8 @@ def main():
         ms32, run32, _ = time_steps(ctx32, tin, tout, res, work, args.steps, args.warmup, args.seed)
         line["fp32_table_form"] = {"value": n / (ms32 * 1e-3), "unit": "track-steps/s", "ms_per_step": ms32,
                                    "kernel_ms": run32, "speedup_vs_fp64": ms64 / ms32,
-                                   "errors_vs_fp64": fp32_errors(r64, t64, res.to_numpy(), tout.to_numpy())}
+                                   "errors_vs_fp64": fp32_errors(r64, t64, res.to_numpy(), tout.to_numpy(),
+                                                                 tracks["ptype"])}
         del r64, t64
         log("config-1 lookups")
         tables1 = Tables(SET1, 0.7)
@@ -505,14 +549,14 @@ def main():
         dev_batches = [(TrackSoA.from_numpy(b, "torch", dev), TrackSoA(len(b["ekin"]), "torch", dev),
                         StepOutSoA(len(b["ekin"]), "torch", dev)) for b in host_batches]
         del host_batches
-        ms_b64, shape64 = time_batched(ctx64, dev_batches, args.seed, max(3, args.steps // 4))
-        ms_b32, shape32 = time_batched(ctx32, dev_batches, args.seed, max(3, args.steps // 4))
+        ms_b64, shape64, chk64 = time_batched(ctx64, dev_batches, args.seed, max(3, args.steps // 4))
+        ms_b32, shape32, chk32 = time_batched(ctx32, dev_batches, args.seed, max(3, args.steps // 4))
         line["config3_batched"] = {
             "batches": len(dev_batches), "tracks": ntot, "launch": "hipGraph replay of a tuned batch plan",
             "fp64": {"value": ntot / (ms_b64 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b64,
-                     "block": shape64[0]},
+                     "block": shape64[0], **chk64},
             "fp32_table_form": {"value": ntot / (ms_b32 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b32,
-                                "block": shape32[0]},
+                                "block": shape32[0], **chk32},
             "speedup_fp32_vs_fp64": ms_b64 / ms_b32,
         }
         del dev_batches

@@ -124,6 +124,7 @@ def lib() -> C.CDLL:
     L.hepem_status_string.restype = C.c_char_p
     L.hepem_lookup_eloss_xsec.argtypes = [vp, C.c_int, i32, vp, vp, vp, vp, vp]
     L.hepem_step.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(Tracks), C.POINTER(StepOut), vp, vp]
+    L.hepem_lookup_gamma_xsec.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]
     L.hepem_step_noalloc.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(Tracks), C.POINTER(StepOut), vp]
     L.hepem_step_work_bytes.argtypes = [i32]
     L.hepem_step_work_bytes.restype = u64
@@ -148,7 +149,7 @@ def lib() -> C.CDLL:
         getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), C.POINTER(StepOut), vp, vp]
     for name in ("hepem_tables_build", "hepem_tables_free", "hepem_tables_set_nuclear", "hepem_tables_serialize", "hepem_tables_from_json",
                  "hepem_tables_to_json", "hepem_ctx_create",
-                 "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_step", "hepem_step_noalloc",
+                 "hepem_ctx_destroy", "hepem_ctx_table_bytes", "hepem_lookup_eloss_xsec", "hepem_lookup_gamma_xsec", "hepem_step", "hepem_step_noalloc",
                  "hepem_step_prepare", "hepem_step_run", "hepem_step_rejected", "hepem_ctx_set_process_sort",
                  "hepem_ctx_set_lookup_staging",
                  "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",

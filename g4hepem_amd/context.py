@@ -57,6 +57,21 @@ class Context:
                                                 out.data_ptr(), _stream_handle(stream)))
         return out.view(5, n)
 
+    def lookup_gamma(self, imc, ekin, lekin, urnd, stream=None):
+        """hepem_lookup_gamma_xsec on device tensors: returns (total
+        macroscopic cross section, photoelectric cross section held by the
+        track, selected interaction) tensors."""
+        import torch
+        n = imc.numel()
+        xs = torch.empty(n, dtype=torch.float64, device=imc.device)
+        pe = torch.empty(n, dtype=torch.float64, device=imc.device)
+        pid = torch.empty(n, dtype=torch.int32, device=imc.device)
+        N.check("hepem_lookup_gamma_xsec",
+                N.lib().hepem_lookup_gamma_xsec(self.h, n, imc.data_ptr(), ekin.data_ptr(), lekin.data_ptr(),
+                                                urnd.data_ptr(), xs.data_ptr(), pe.data_ptr(), pid.data_ptr(),
+                                                _stream_handle(stream)))
+        return xs, pe, pid
+
     def step(self, seed: int, tin: TrackSoA, tout: TrackSoA, res: StepOutSoA, work=None, stream=None) -> None:
         si, so, rr = tin.struct(), tout.struct(), res.struct()
         wp = work.data_ptr() if work is not None else None

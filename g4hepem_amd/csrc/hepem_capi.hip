@@ -513,6 +513,24 @@ extern "C" int hepem_lookup_eloss_xsec(const hepem_ctx* c, int ptype, int32_t n,
   return HEPEM_OK;
 }
 
+extern "C" int hepem_lookup_gamma_xsec(const hepem_ctx* c, int32_t n, const int32_t* imc, const double* ekin,
+                                       const double* lekin, const double* urnd, double* out_xs, double* out_pe,
+                                       int32_t* out_pid, void* stream) {
+  if (!c || n < 0) return HEPEM_ERR_INVALID;
+  if (n == 0) return HEPEM_OK;
+  if (!imc || !ekin || !lekin || !urnd || !out_xs || !out_pe || !out_pid) return HEPEM_ERR_INVALID;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int grid = (n + kBlock - 1) / kBlock;
+  if (c->precision == HEPEM_FP64)
+    hipLaunchKernelGGL(k_gamma_lookup<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, n, imc, ekin, lekin, urnd, out_xs,
+                       out_pe, out_pid);
+  else
+    hipLaunchKernelGGL(k_gamma_lookup<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, n, imc, ekin, lekin, urnd, out_xs,
+                       out_pe, out_pid);
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+
 extern "C" int hepem_ctx_set_lookup_staging(hepem_ctx* c, int on) {
   if (!c || (on != 0 && on != 1)) return HEPEM_ERR_INVALID;
   c->lookupStaging = on;

@@ -1101,6 +1101,40 @@ __global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, in
   LookupOne(T.el[ptype], T.nmc, n, i, imc, ekin, lekin, out);
 }
 
+// Gamma macroscopic cross sections and the interaction they select, one query
+// per thread, as the reference's TestMacXSecDataKernel drives
+// G4HepEmGammaManager::GetTotalMacXSec and SampleInteraction
+// (testing/GammaXSections/src/GammaMacXSecs.cu:20-31): the total cross section,
+// the photoelectric one the track holds after SampleInteraction, and the
+// process chosen for the uniform draw u. An invalid couple gives NaN and -99.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_gamma_lookup(DevTables<R> T, int n, const int32_t* __restrict__ imc,
+                                                         const double* __restrict__ ekin, const double* __restrict__ lekin,
+                                                         const double* __restrict__ urnd, double* __restrict__ out_xs,
+                                                         double* __restrict__ out_pe, int32_t* __restrict__ out_pid) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int mc = imc[i];
+  if (mc < 0 || mc >= T.nmc) {
+    out_xs[i] = out_pe[i] = __longlong_as_double(0x7FF8000000000000LL);
+    out_pid[i] = -99;
+    return;
+  }
+  Trk<R> t;
+  t.ekin = ekin[i];
+  t.lekin = (R)lekin[i];
+  t.imc = mc;
+  t.winner = -1;
+  for (int k = 0; k < 4; ++k) t.mfp[k] = t.nial[k] = (R)0.0;
+  R pe = (R)0.0;
+  const R tot = GetTotalMacXSec(T, t, pe);
+  t.mfp[0] = (tot > (R)0.) ? (R)1. / tot : (R)kALargeValue;
+  SampleInteraction(T, t, pe, (R)urnd[i]);
+  out_xs[i] = (double)tot;
+  out_pe[i] = (double)pe;
+  out_pid[i] = t.winner;
+}
+
 // LDS staging of the config-1 lookups. The loss grid and data and the
 // cross-section blocks of every couple (the doubles, and in the FP32 form
 // the floats too) are a few kB per couple; each block copies them to LDS

@@ -92,6 +92,39 @@ class HowFarSoA(SoA):
         return h
 
 
+def step_outputs_differ(ptype, ta: TrackSoA, ra: StepOutSoA, tb: TrackSoA, rb: StepOutSoA) -> list:
+    """Names of the outputs of a step -- results and updated track state -- in
+    which two runs over the same batch differ bit for bit, compared on the
+    device. `ptype` is the batch's particle-type tensor. Not compared, because a
+    step leaves them unwritten: secondary slots at or beyond nsec, and the MSC
+    state of gammas."""
+    import torch
+    n = ta.n
+    bad = []
+    nsec = ra["nsec"]
+    el = ptype != GAMMA
+
+    def bits(x):
+        return x.view(torch.int64) if x.dtype == torch.float64 else x
+
+    for name, (_, k) in OUT_FIELDS.items():
+        a, b = bits(ra[name]), bits(rb[name])
+        if name.startswith("sec_"):
+            used = torch.stack([nsec > s for s in range(2)]).repeat(k // 2, 1).reshape(-1)
+            a, b = a[used], b[used]
+        if not torch.equal(a, b):
+            bad.append(name)
+    for name, (_, k) in TRACK_FIELDS.items():
+        a, b = bits(ta[name]), bits(tb[name])
+        if name.startswith("msc"):
+            m = el.repeat(k)
+            a, b = a[m], b[m]
+        if not torch.equal(a, b):
+            bad.append("state." + name)
+    assert n == tb.n
+    return bad
+
+
 def reset_state_numpy(t: dict, n: int) -> None:
     """Start-of-track state (G4HepEmTrack/G4HepEmMSCTrackData ReSet, no cached Gaussian)."""
     t["num_ia_left"][:] = -1.0

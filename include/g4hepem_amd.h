@@ -161,6 +161,19 @@ int  hepem_lookup_eloss_xsec(const hepem_ctx* ctx, int ptype, int32_t n,
                              const int32_t* imc, const double* ekin, const double* lekin,
                              double* out, void* hip_stream);
 
+/* ---- gamma cross-section lookups -------------------------------------------
+ * For n gammas (imc, E, lnE) and uniform draws u: out_xs[i] the total
+ * macroscopic cross section (G4HepEmGammaManager::GetTotalMacXSec,
+ * G4HepEm/G4HepEmRun/include/G4HepEmGammaManager.icc:108-153), out_pid[i] the
+ * interaction SampleInteraction (.icc:187-219) selects for u with the mean
+ * free path 1/out_xs[i], and out_pe[i] the photoelectric cross section the
+ * track holds afterwards -- the batched form of the reference test kernel
+ * TestMacXSecDataKernel (testing/GammaXSections/src/GammaMacXSecs.cu:20-31).
+ * All pointers are device pointers. */
+int  hepem_lookup_gamma_xsec(const hepem_ctx* ctx, int32_t n, const int32_t* imc, const double* ekin,
+                             const double* lekin, const double* urnd, double* out_xs, double* out_pe,
+                             int32_t* out_pid, void* hip_stream);
+
 /* ---- stepping ------------------------------------------------------------
  * One step of every track of the batch. `out` may alias `in`. `work` must be
  * at least hepem_step_work_bytes(n) bytes of device memory (step counters,

@@ -42,6 +42,7 @@ def test_null_arguments_are_rejected():
     assert L.hepem_step_prepare(None, C.byref(t), None, None) == -1
     assert L.hepem_step_run(None, 1, C.byref(t), C.byref(t), C.byref(r), None, None) == -1
     assert L.hepem_step_noalloc(None, 1, C.byref(t), C.byref(t), C.byref(r), None) == -1
+    assert L.hepem_lookup_gamma_xsec(None, 10, None, None, None, None, None, None, None, None) == -1
     assert L.hepem_tracks_reset_state(None, None) == -1
     assert L.hepem_tracks_reset_state(C.byref(t), None) == 0   # empty batch: nothing to do
     t5 = Tracks()
