@@ -75,6 +75,13 @@ def slab_setup(nlayers: int, thickness, imc, primary: int, ekin0: float, capacit
     return s
 
 
+class SubstepState(C.Structure):
+    """hepem_substep_state: TrackElectron's loop state of a batch of e-/e+."""
+    _fields_ = [("n", C.c_int32), ("discrete_winner", P), ("step_left", P), ("true_path", P), ("geom_path", P),
+                ("mean_eloss", P), ("pre_ekin", P), ("displacement", P), ("active", P), ("more", P), ("stopped", P),
+                ("left_world", P), ("nsub", P)]
+
+
 class BatchDesc(C.Structure):
     _fields_ = [("inp", Tracks), ("out", Tracks), ("res", StepOut)]
 
@@ -92,6 +99,13 @@ OUT_FIELDS = {
     "true_step": ("float64", 1), "geom_step": ("float64", 1), "edep": ("float64", 1),
     "displacement": ("float64", 3), "winner": ("int32", 1), "stopped": ("uint8", 1),
     "nsec": ("int32", 1), "sec_ptype": ("int8", 2), "sec_ekin": ("float64", 2), "sec_dir": ("float64", 6),
+}
+
+SUBSTEP_FIELDS = {
+    "discrete_winner": ("int32", 1), "step_left": ("float64", 1), "true_path": ("float64", 1),
+    "geom_path": ("float64", 1), "mean_eloss": ("float64", 1), "pre_ekin": ("float64", 1),
+    "displacement": ("float64", 3), "active": ("uint8", 1), "more": ("uint8", 1), "stopped": ("uint8", 1),
+    "left_world": ("uint8", 1), "nsub": ("int32", 1),
 }
 
 HOWFAR_FIELDS = {
@@ -142,6 +156,11 @@ def lib() -> C.CDLL:
     L.hepem_batch_plan_shape.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     L.hepem_batch_plan_rejected.argtypes = [vp, i32, C.POINTER(C.c_uint32), vp]
     L.hepem_tracks_reset_state.argtypes = [C.POINTER(Tracks), vp]
+    for name in ("hepem_electron_step_begin", "hepem_electron_substep_howfar", "hepem_electron_substep_along"):
+        getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), C.POINTER(SubstepState), vp, vp]
+    L.hepem_electron_step_end.argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), C.POINTER(SubstepState),
+                                          C.POINTER(StepOut), vp, vp]
+    L.hepem_electron_substep_active.argtypes = [vp, C.POINTER(C.c_uint32), vp]
     split = ("hepem_howfar", "hepem_electron_howfar", "hepem_gamma_howfar")
     for name in split:
         getattr(L, name).argtypes = [vp, u64, C.POINTER(Tracks), C.POINTER(HowFarState), vp, vp]
@@ -155,7 +174,9 @@ def lib() -> C.CDLL:
                  "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",
                  "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform",
                  "hepem_batch_plan_create", "hepem_batch_plan_destroy", "hepem_batch_plan_run", "hepem_batch_plan_tune",
-                 "hepem_batch_plan_shape", "hepem_batch_plan_rejected", "hepem_slab_run"):
+                 "hepem_batch_plan_shape", "hepem_batch_plan_rejected", "hepem_slab_run", "hepem_electron_step_begin",
+                 "hepem_electron_substep_howfar", "hepem_electron_substep_along", "hepem_electron_substep_active",
+                 "hepem_electron_step_end"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -114,6 +114,34 @@ class Context:
         N.check(name, getattr(N.lib(), name)(self.h, C.c_uint64(seed), C.byref(st), C.byref(sh), C.byref(rr), wp,
                                                _stream_handle(stream)))
 
+    # TrackElectron's MSC sub-steps driven by the caller (hepem_electron_step_begin ...)
+    def electron_step_begin(self, seed: int, t: TrackSoA, hs: HowFarSoA, ss, work, stream=None) -> None:
+        self._substep("hepem_electron_step_begin", seed, t, hs, ss, work, stream)
+
+    def electron_substep_howfar(self, seed: int, t: TrackSoA, hs: HowFarSoA, ss, work, stream=None) -> None:
+        self._substep("hepem_electron_substep_howfar", seed, t, hs, ss, work, stream)
+
+    def electron_substep_along(self, seed: int, t: TrackSoA, hs: HowFarSoA, ss, work, stream=None) -> None:
+        self._substep("hepem_electron_substep_along", seed, t, hs, ss, work, stream)
+
+    @staticmethod
+    def electron_substep_active(work, stream=None) -> int:
+        v = C.c_uint32()
+        N.check("hepem_electron_substep_active",
+                N.lib().hepem_electron_substep_active(work.data_ptr(), C.byref(v), _stream_handle(stream)))
+        return v.value
+
+    def electron_step_end(self, seed: int, t: TrackSoA, hs: HowFarSoA, ss, res: StepOutSoA, work, stream=None) -> None:
+        st, sh, sss, rr = t.struct(), hs.struct(), ss.struct(), res.struct()
+        N.check("hepem_electron_step_end",
+                N.lib().hepem_electron_step_end(self.h, C.c_uint64(seed), C.byref(st), C.byref(sh), C.byref(sss),
+                                                C.byref(rr), work.data_ptr(), _stream_handle(stream)))
+
+    def _substep(self, name, seed, t, hs, ss, work, stream):
+        st, sh, sss = t.struct(), hs.struct(), ss.struct()
+        N.check(name, getattr(N.lib(), name)(self.h, C.c_uint64(seed), C.byref(st), C.byref(sh), C.byref(sss),
+                                               work.data_ptr(), _stream_handle(stream)))
+
     def slab_run(self, setup: N.SlabSetup, nevents: int, seed: int, stream=None, per_event: bool = False) -> dict:
         """hepem_slab_run: showers in a stack of slabs. Returns per-slab sums
         over events of the deposit and its square, run totals and, with

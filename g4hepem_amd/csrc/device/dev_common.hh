@@ -358,6 +358,7 @@ struct DevTables {
   const double* regSafetyFac;
   const int* regMinimal;
   const int* regFluct;
+  const int* regMultiStep;  // fIsMultipleStepsInMSCTrans: MSC-limited sub-steps within one step
 };
 
 }  // namespace hepem

@@ -286,7 +286,7 @@ template <typename R> void PackTables(Packer& P, DevTables<R>& T, const G4HepEmD
   T.displace = p->fIsMSCDisplacement ? 1 : 0;
   const int nreg = p->fNumRegions;
   std::vector<double> fr(nreg), dr(nreg), ll(nreg), rf(nreg), sf(nreg);
-  std::vector<int> mn(nreg), fl(nreg);
+  std::vector<int> mn(nreg), fl(nreg), ms(nreg);
   for (int r = 0; r < nreg; ++r) {
     const G4HepEmRegionParmeters& rp = p->fParametersPerRegion[r];
     fr[r] = rp.fFinalRange;
@@ -296,6 +296,7 @@ template <typename R> void PackTables(Packer& P, DevTables<R>& T, const G4HepEmD
     sf[r] = rp.fMSCSafetyFactor;
     mn[r] = rp.fIsMSCMinimalStepLimit ? 1 : 0;
     fl[r] = rp.fIsELossFluctuation ? 1 : 0;
+    ms[r] = rp.fIsMultipleStepsInMSCTrans ? 1 : 0;
   }
   P.add(T.regFinalRange, fr);
   P.add(T.regDRover, dr);
@@ -304,6 +305,7 @@ template <typename R> void PackTables(Packer& P, DevTables<R>& T, const G4HepEmD
   P.add(T.regSafetyFac, sf);
   P.add(T.regMinimal, mn);
   P.add(T.regFluct, fl);
+  P.add(T.regMultiStep, ms);
 }
 
 bool DataComplete(const G4HepEmData* d, const G4HepEmParameters* p) {
@@ -772,6 +774,81 @@ extern "C" int hepem_gamma_howfar(const hepem_ctx* c, uint64_t seed, hepem_track
 extern "C" int hepem_gamma_perform(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
                                    hepem_step_out* res, void* work, void* stream) {
   return SplitCall<kOnlyGammas>(c, seed, t, hs, res, work, stream, true);
+}
+
+namespace {
+bool SubstepOk(const hepem_substep_state* ss, int n) {
+  return ss && ss->n == n &&
+         (n == 0 || (ss->discrete_winner && ss->step_left && ss->true_path && ss->geom_path && ss->mean_eloss && ss->pre_ekin &&
+                     ss->displacement && ss->active && ss->more && ss->stopped && ss->left_world && ss->nsub));
+}
+
+// the sub-step calls: 0 begin, 1 MSC HowFar, 2 along, 3 end
+int SubstepCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs, hepem_substep_state* ss,
+                hepem_step_out* res, void* work, void* stream, int phase) {
+  if (!c || !TracksOk(t) || !HowFarOk(hs, t ? t->n : -1) || !SubstepOk(ss, t ? t->n : -1) ||
+      (phase == 3 && !ResOk(res, t->n)))
+    return HEPEM_ERR_INVALID;
+  const int n = t->n;
+  int rc = EnsureWork(c, n > 0 ? n : 1, work);
+  if (rc != HEPEM_OK) return rc;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t* counters = static_cast<uint32_t*>(work);
+  if (phase == 0) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters, 64, 0u);
+  if (phase == 2) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters + 4, 1, 0u);
+  if (n == 0) {
+    HIP_OK(hipGetLastError());
+    return HEPEM_OK;
+  }
+  const int grid = (n + kBlock - 1) / kBlock;
+  const bool fp64 = c->precision == HEPEM_FP64;
+  switch (phase) {
+    case 0:
+      if (fp64) hipLaunchKernelGGL(k_el_begin<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *ss, counters);
+      else hipLaunchKernelGGL(k_el_begin<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *ss, counters);
+      break;
+    case 1:
+      if (fp64) hipLaunchKernelGGL(k_el_substep_howfar<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *ss);
+      else hipLaunchKernelGGL(k_el_substep_howfar<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *ss);
+      break;
+    case 2:
+      if (fp64)
+        hipLaunchKernelGGL(k_el_substep_along<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *ss, counters);
+      else
+        hipLaunchKernelGGL(k_el_substep_along<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *ss, counters);
+      break;
+    default:
+      if (fp64) hipLaunchKernelGGL(k_el_end<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *ss, *res);
+      else hipLaunchKernelGGL(k_el_end<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *ss, *res);
+      break;
+  }
+  HIP_OK(hipGetLastError());
+  return HEPEM_OK;
+}
+}  // namespace
+
+extern "C" int hepem_electron_step_begin(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                                         hepem_substep_state* ss, void* work, void* stream) {
+  return SubstepCall(c, seed, t, hs, ss, nullptr, work, stream, 0);
+}
+extern "C" int hepem_electron_substep_howfar(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                                             hepem_substep_state* ss, void* work, void* stream) {
+  return SubstepCall(c, seed, t, hs, ss, nullptr, work, stream, 1);
+}
+extern "C" int hepem_electron_substep_along(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                                            hepem_substep_state* ss, void* work, void* stream) {
+  return SubstepCall(c, seed, t, hs, ss, nullptr, work, stream, 2);
+}
+extern "C" int hepem_electron_step_end(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                                       hepem_substep_state* ss, hepem_step_out* res, void* work, void* stream) {
+  return SubstepCall(c, seed, t, hs, ss, res, work, stream, 3);
+}
+extern "C" int hepem_electron_substep_active(const void* work, uint32_t* nactive, void* stream) {
+  if (!work || !nactive) return HEPEM_ERR_INVALID;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_OK(hipMemcpyAsync(nactive, static_cast<const uint32_t*>(work) + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return HEPEM_OK;
 }
 
 extern "C" int hepem_step_rejected(const void* work, uint32_t* nrej, void* stream) {

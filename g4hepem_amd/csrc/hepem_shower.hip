@@ -2,17 +2,17 @@
 //
 // The track stack is a pair of structure-of-arrays buffers of `capacity`
 // slots. One stepping round over the n tracks in flight:
-//   hepem_howfar   (library C-ABI); each track's isotropic safety (distance
-//                  to the nearer plane) was set when it was stacked
-//   k_sh_geometry  cut the geometrical step at the next plane along x; the
-//                  post-step boundary flag; end point of the straight move;
-//                  with Woodcock tracking on, a gamma above its energy limit
-//                  is moved through the stack to its next interaction point
-//                  or out of the stack instead
-//   hepem_perform  (library C-ABI)
-//   k_sh_post      MSC displacement, slab crossing, scoring, leakage, the
-//                  number of stack slots each track needs next round and
-//                  their exclusive scan within the block
+//   k_sh_step      one step of every track: an e-/e+ takes the whole
+//                  G4HepEmTrackingManager::TrackElectron step (MSC sub-steps
+//                  and their displacements, device/dev_tracking.hh) with the
+//                  slab as its geometry; a gamma HowFar, the cut at the next
+//                  plane along x (or, with Woodcock tracking on and above its
+//                  energy limit, a step through the stack to its next
+//                  interaction point or out of it), then the boundary update or
+//                  SelectInteraction + Perform
+//   k_sh_post      slab crossing, leakage, scoring, the number of stack slots
+//                  each track needs next round and their exclusive scan within
+//                  the block
 //   k_scan_sums    exclusive scans of the block totals (one block)
 //   k_sh_scatter   survivors and their secondaries into the other buffer:
 //                  e-/e+ first, then gammas, each in slot order of the parents
@@ -29,6 +29,8 @@
 #include <vector>
 
 #include "device/dev_physics.hh"
+#include "device/dev_soa.hh"
+#include "device/dev_tracking.hh"
 #include "g4hepem_amd_shower.h"
 #include "hepem_internal.hh"
 
@@ -91,6 +93,11 @@ struct Stack {
   int32_t* event;     // deposit row of the event (events resident at once)
   int32_t* nstep;     // steps taken by the track
 };
+
+__global__ void k_sh_fill32(uint32_t* p, int n, uint32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 __global__ void k_sh_fill64(int64_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,35 +214,32 @@ __device__ inline void WalkToSlab(const Slabs& g, double x, int& l, int& a, doub
 
 constexpr int kWdtMaxIter = 1 << 20;  // delta steps of one Woodcock step (then: taken to the exit)
 
-// Woodcock tracking of gamma slot i through the stack
+// Woodcock tracking of a gamma through the stack
 // (G4HepEmWoodcockHelper::KeepTracking, G4HepEm/G4HepEm/src/G4HepEmWoodcockHelper.cc:145-300,
-// and its driver, G4HepEmTrackingManager.cc:955-1062). The distance to the
-// stack's exit less 1e-3 mm is eaten up in steps sampled with the densest
-// material's total cross section Sigma_w; a point in another material is an
-// interaction with probability Sigma/Sigma_w, one in the densest material
-// always. On an interaction the slot moves there with the couple, total mean
-// free path and PE cross section of the point; on reaching the exit it ends
-// on the stack's boundary. Either way the number of interaction lengths left
-// is reset and the step adds nothing to it. Returns false (nothing written) if
-// the gamma is within 1e-3 + 1e-6 mm of the exit (FindWDTVolume, :141-142).
+// and its driver, G4HepEmTrackingManager.cc:955-1062), after its HowFar. The
+// distance to the stack's exit less 1e-3 mm is eaten up in steps sampled with
+// the densest material's total cross section Sigma_w; a point in another
+// material is an interaction with probability Sigma/Sigma_w, one in the densest
+// material always. On an interaction the gamma moves there (layer l, position
+// xe) with the couple, total mean free path and PE cross section of the point;
+// on reaching the exit it ends on the stack's boundary (side sd). Either way
+// the number of interaction lengths left is reset and the step adds nothing to
+// it. Returns false (nothing changed) if the gamma is within 1e-3 + 1e-6 mm of
+// the exit (FindWDTVolume, :141-142).
 template <typename R>
-__device__ bool WoodcockStep(const hepem::DevTables<R>& T, uint64_t seed, Stack& s, const Slabs& g,
-                             hepem_howfar_state& hs, double* xend, int8_t* side, int i, int l, double x, double ux) {
+__device__ bool WoodcockStep(const hepem::DevTables<R>& T, const Slabs& g, hepem::Trk<R>& gt, hepem::Rng& rng, R& pe, int& l,
+                             double x, double ux, double& xe, int& sd) {
   using namespace hepem;
   const double distOut = ux > 0.0 ? (g.xmax - x) / ux : (ux < 0.0 ? (0.0 - x) / ux : 1.0e300);
   double dist = distOut - 1.0E-3 > 0.0 ? distOut - 1.0E-3 : 0.0;
   if (!(dist >= 1.0E-6)) return false;
   constexpr R kBig = std::numeric_limits<R>::max();
-  const hepem_tracks& t = s.t;
-  Rng rng;
-  rng.init(seed, (uint32_t)t.id[i], t.rng_stream ? t.rng_stream[i] : 0u, t.rng_ctr[i], t.rng_has_gauss[i] != 0,
-           t.rng_gauss[i]);
-  Trk<R> gt;
-  gt.isEl = false;
-  gt.setE(t.ekin[i]);
-  gt.imc = g.wdtImc;
+  Trk<R> xt;  // cross-section evaluations at the gamma's energy
+  xt.isEl = false;
+  xt.setE(gt.ekin);
+  xt.imc = g.wdtImc;
   R wpe = (R)0.0;
-  const R wmx = GetTotalMacXSec(T, gt, wpe);
+  const R wmx = GetTotalMacXSec(T, xt, wpe);
   const R wmfp = wmx > (R)0.0 ? (R)1.0 / wmx : kBig;
   // total and PE cross sections of the other absorbers' couples at this energy
   // (the energy stays the same through the step), ahead of the loop so that
@@ -245,8 +249,8 @@ __device__ bool WoodcockStep(const hepem::DevTables<R>& T, uint64_t seed, Stack&
   for (int k = 0; k < HEPEM_SLAB_MAX_ABS; ++k) {
     mxa[k] = pea[k] = (R)0.0;
     if (k < g.nabs && !((g.wdtMask >> k) & 1u)) {
-      gt.imc = g.imc[k];
-      mxa[k] = GetTotalMacXSec(T, gt, pea[k]);
+      xt.imc = g.imc[k];
+      mxa[k] = GetTotalMacXSec(T, xt, pea[k]);
     }
   }
   R mfp = wmfp, pint = wpe;
@@ -284,58 +288,191 @@ __device__ bool WoodcockStep(const hepem::DevTables<R>& T, uint64_t seed, Stack&
       break;
     }
   }
-  t.rng_ctr[i] = rng.ctr;
-  t.num_ia_left[i] = -1.0;
-  hs.geom_step[i] = 0.0;
+  gt.nial[0] = (R)-1.0;
+  gt.gstep = (R)0.0;
   if (boundary) {
-    const int sd = ux < 0.0 ? -1 : 1;
-    s.layer[i] = sd > 0 ? g.nbins - 1 : 0;
-    t.on_boundary[i] = 1;
-    side[i] = (int8_t)sd;
-    xend[i] = sd > 0 ? g.xmax : 0.0;
+    sd = ux < 0.0 ? -1 : 1;
+    l = sd > 0 ? g.nbins - 1 : 0;
+    gt.onBoundary = true;
+    xe = sd > 0 ? g.xmax : 0.0;
   } else {
-    s.layer[i] = lp;
-    t.imc[i] = imcp;
-    hs.mfp[i] = (double)mfp;
-    hs.pe_xsec[i] = (double)pint;
-    t.on_boundary[i] = 0;
-    side[i] = 0;
-    xend[i] = x + L * ux;
+    l = lp;
+    gt.imc = imcp;
+    gt.mfp[0] = mfp;
+    pe = pint;
+    gt.onBoundary = false;
+    sd = 0;
+    xe = x + L * ux;
   }
   return true;
 }
 
-template <typename R, bool kWoodcock>
-__global__ void k_sh_geometry(Stack s, Slabs g, hepem_howfar_state hs, double* xend, int8_t* side,
-                              hepem::DevTables<R> T, uint64_t seed) {
+// One step of the e-/e+ slots [0, ne), the track state updated in place: the
+// whole G4HepEmTrackingManager::TrackElectron step with the slab as the
+// caller's geometry (device/dev_tracking.hh), MSC sub-steps while the region
+// asks for them, each MSC displacement applied in the slab, the stack's outer
+// planes as the world's boundary. xend = post-step position; side = +1/-1 when
+// the step ended on the upper or lower plane of its slab, 0 inside. The
+// discrete interaction's final state is left to k_sh_electron_final: proc =
+// the process to sample (0 ionisation, 1 brem, 2 annihilation in flight) or -1.
+//
+// Steps take from one to a few tens of sub-steps, so a lane keeps a track only
+// for its sub-steps: each pass of the loop below does one sub-step of every
+// lane's track, and lanes whose track finished take the next slots of a queue
+// shared by the grid (one atomic per wave and refill). A wave therefore runs
+// sub-steps of many tracks side by side instead of waiting for its longest
+// chain. The grid is one resident round of blocks; every wave leaves the loop
+// once the queue is exhausted and its lanes' tracks are done (each step's
+// sub-steps are bounded by kMaxSubsteps).
+template <typename R>
+__global__ void __launch_bounds__(kB) k_sh_electron(Stack s, Slabs g, hepem_step_out res, double* __restrict__ xend,
+                                                    int8_t* __restrict__ side, int8_t* __restrict__ proc,
+                                                    hepem::DevTables<R> T, uint64_t seed, int ne,
+                                                    uint32_t* __restrict__ queue) {
+  using namespace hepem;
+  const int n = s.t.n;
+  hepem_tracks& t = s.t;
+  const int lane = __lane_id();
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int i = -1;          // slot of this lane's track; -1: none
+  bool drained = false;  // the queue has no slots left (uniform over the wave)
+  ElState<R> el;
+  Rng rng;
+  StepLoop<R> L;
+  SlabPos p;
+  while (true) {
+    const uint64_t need = drained ? 0ull : __ballot(i < 0);
+    if (need) {
+      const int leader = __ffsll((unsigned long long)need) - 1;
+      uint32_t base = 0u;
+      if (lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(need));
+      base = __shfl(base, leader, 64);
+      if (base + (uint32_t)__popcll(need) >= (uint32_t)ne) drained = true;
+      if (i < 0) {
+        const uint32_t j = base + (uint32_t)__popcll(need & below);
+        if (j < (uint32_t)ne) {
+          i = (int)j;
+          const int l = s.layer[i];
+          InitElectron(t, n, i, seed, el, rng);
+          p = SlabPos{s.x[i], g.plane[l], g.plane[l + 1], l == 0, l == g.nbins - 1, 0};
+          SlabBegin(T, el, L, rng, p);
+        }
+      }
+    }
+    if (!__ballot(i >= 0)) break;
+    if (i >= 0 && SlabSubstep(T, el, L, rng, p)) {
+      SecList<R> sl;
+      sl.ns = 0;
+      proc[i] = (int8_t)LoopFinishSelect(T, el, L, sl, rng);
+      StoreElectronState(t, n, i, el, rng);
+      t.ekin[i] = el.t.ekin;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) t.dir[c * n + i] = (double)el.t.dir[c];
+      t.on_boundary[i] = p.side != 0 ? 1 : 0;
+      res.true_step[i] = (double)L.truePath;
+      res.edep[i] = el.t.edep;
+      res.winner[i] = el.t.winner;
+      WriteSecondaries(res, n, i, sl);
+      xend[i] = p.x;
+      side[i] = (int8_t)p.side;
+      i = -1;
+    }
+  }
+}
+
+// The final states of the discrete interactions k_sh_electron selected: the
+// e-/e+ slot's post-step energy and direction, RNG state and secondaries.
+template <typename R>
+__global__ void __launch_bounds__(kB) k_sh_electron_final(Stack s, hepem_step_out res, const int8_t* __restrict__ proc,
+                                                          hepem::DevTables<R> T, uint64_t seed, int ne) {
+  using namespace hepem;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const int p = proc[i];
+  if (p < 0) return;
+  const int n = s.t.n;
+  hepem_tracks& t = s.t;
+  Trk<R> e;
+  e.isEl = t.ptype[i] == 0;
+  e.setE(t.ekin[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) e.dir[c] = (R)t.dir[c * n + i];
+  e.imc = t.imc[i];
+  Rng rng;
+  rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
+  SecList<R> sl;
+  sl.ns = 0;
+  ElectronFinalState(T, e, p, sl, rng);
+  t.ekin[i] = e.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c * n + i] = (double)e.dir[c];
+  StoreRng(t, i, rng);
+  WriteSecondaries(res, n, i, sl);
+}
+
+// One step of the gamma slots [ne, n), as G4HepEmTrackingManager::TrackGamma
+// takes it (G4HepEmTrackingManager.cc:893-1145): HowFar, the cut at the next
+// plane along x (or, with Woodcock tracking on and above its energy limit, a
+// Woodcock step through the stack), then the boundary update or
+// SelectInteraction + Perform. xend and side as k_sh_electron; layer = the slab
+// the step's deposit is scored in (changed only by a Woodcock step).
+template <typename R, bool kWoodcock>
+__global__ void __launch_bounds__(kB) k_sh_gamma(Stack s, Slabs g, hepem_step_out res, double* __restrict__ xend,
+                                                 int8_t* __restrict__ side, hepem::DevTables<R> T, uint64_t seed, int ne) {
+  using namespace hepem;
+  const int i = ne + blockIdx.x * blockDim.x + threadIdx.x;
   const int n = s.t.n;
   if (i >= n) return;
-  const int l = s.layer[i];
-  const double x = s.x[i], ux = s.t.dir[i];
-  if (kWoodcock && s.t.ptype[i] == 2 && hs.winner[i] != -99 && s.t.ekin[i] > g.wdtMin &&
-      WoodcockStep(T, seed, s, g, hs, xend, side, i, l, x, ux))
-    return;
-  const double gs = hs.geom_step[i];
-  double dist = 1.0e300;
+  hepem_tracks& t = s.t;
+  int l = s.layer[i];
+  const double x = s.x[i];
+  SecList<R> sl;
+  sl.ns = 0;
+  Trk<R> gt;
+  gt.isEl = false;
+  LoadTrack(t, n, i, gt);
+  Rng rng;
+  rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
+  R pe = GammaHowFar(T, gt, rng);
+  const double ux = (double)gt.dir[0];
+  double xe;
   int sd = 0;
-  if (ux > 0.0) {
-    dist = (g.plane[l + 1] - x) / ux;
-    sd = 1;
-  } else if (ux < 0.0) {
-    dist = (g.plane[l] - x) / ux;
-    sd = -1;
-  }
-  if (hs.winner[i] != -99 && dist < gs) {
-    hs.geom_step[i] = dist;
-    s.t.on_boundary[i] = 1;
-    side[i] = (int8_t)sd;
-    xend[i] = sd > 0 ? g.plane[l + 1] : g.plane[l];
+  if (!(kWoodcock && gt.ekin > g.wdtMin && WoodcockStep(T, g, gt, rng, pe, l, x, ux, xe, sd))) {
+    const double gs = (double)gt.gstep;
+    double dist = 1.0e300;
+    if (ux > 0.0) {
+      dist = (g.plane[l + 1] - x) / ux;
+      sd = 1;
+    } else if (ux < 0.0) {
+      dist = (g.plane[l] - x) / ux;
+      sd = -1;
+    }
+    if (dist < gs) {
+      gt.gstep = (R)dist;
+      gt.onBoundary = true;
+      xe = sd > 0 ? g.plane[l + 1] : g.plane[l];
+    } else {
+      gt.onBoundary = false;
+      sd = 0;
+      xe = fmin(fmax(x + gs * ux, g.plane[l]), g.plane[l + 1]);
+    }
   } else {
-    s.t.on_boundary[i] = 0;
-    side[i] = 0;
-    xend[i] = fmin(fmax(x + gs * ux, g.plane[l]), g.plane[l + 1]);
+    s.layer[i] = l;
+    t.imc[i] = gt.imc;
   }
+  GammaPerform(T, gt, pe, sl, rng);
+  t.ekin[i] = gt.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c * n + i] = (double)gt.dir[c];
+  t.num_ia_left[i] = (double)gt.nial[0];
+  StoreRng(t, i, rng);
+  t.on_boundary[i] = gt.onBoundary ? 1 : 0;
+  res.true_step[i] = (double)gt.gstep;
+  res.edep[i] = gt.edep;
+  res.winner[i] = gt.winner;
+  WriteSecondaries(res, n, i, sl);
+  xend[i] = xe;
+  side[i] = (int8_t)sd;
 }
 
 // After Perform: position, slab crossing, scoring and next-round slot counts.
@@ -363,45 +500,25 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
     const int l0 = s.layer[i];  // the pre-step slab: scoring and the couple's cuts
     int l = l0;
     double edep = res.edep[i];
-    double x = xend[i];
+    const double x = xend[i];
     const int sd = side[i];
-    const bool rejected = res.winner[i] == -99;
     const bool isE = t.ptype[i] != 2;
-    if (sd == 0 && isE && !rejected) {
-      // MSC displacement (G4HepEmTrackingManager.cc:527-566) with the slab safety
-      const double d0 = res.displacement[i], d1 = res.displacement[n + i], d2 = res.displacement[2 * n + i];
-      const double dLength2 = d0 * d0 + d1 * d1 + d2 * d2;
-      if (dLength2 > kGeomMinLength * kGeomMinLength) {
-        const double dispR = sqrt(dLength2);
-        const double postSafety = 0.99 * Safety(g, l, x);
-        if (postSafety > 0.0 && dispR <= postSafety) {
-          x += d0;
-        } else if (dispR < postSafety) {
-          x += d0;
-        } else if (postSafety > kGeomMinLength) {
-          x += d0 * (postSafety / dispR);
-        }
-      }
-    }
     const double ekin = t.ekin[i];
-    bool alive = !rejected && ekin > 0.0;
-    bool left = false;
-    if (alive && sd != 0) {
+    bool alive = ekin > 0.0;
+    int nsec = res.nsec[i];
+    if (sd != 0 && (l + sd < 0 || l + sd >= g.nbins)) {
+      // leaves the stack: its kinetic energy, the annihilation energy of an e+
+      // in flight and anything its last step created are lost
+      double e = ekin + (t.ptype[i] == 1 && ekin > 0.0 ? 2.0 * kElectronMass : 0.0);
+      for (int sl = 0; sl < nsec; ++sl) e += res.sec_ekin[sl * n + i];
+      atomicAdd(&leak[ev], (unsigned long long)llrint(e * kFix));
+      alive = false;
+      nsec = 0;
+    } else if (alive && sd != 0) {
       l += sd;
-      if (l < 0 || l >= g.nbins) {
-        // leaves the stack: its kinetic energy (and an e+'s annihilation energy) is lost
-        const double e = ekin + (t.ptype[i] == 1 ? 2.0 * kElectronMass : 0.0);
-        atomicAdd(&leak[ev], (unsigned long long)llrint(e * kFix));
-        alive = false;
-        left = true;
-        l -= sd;  // keep the index in range for the record
-      }
     }
     s.x[i] = x;
     s.layer[i] = l;
-    // a track that crossed a plane had no interaction, so a track leaving the
-    // stack has no secondaries; the guard keeps that an invariant of the stack
-    const int nsec = (rejected || left) ? 0 : res.nsec[i];
     // StackSecondaries (G4HepEmTrackingManager.cc:1274-1313): where the
     // couple's region applies cuts, a secondary below its production
     // threshold is deposited at the step instead of stacked (an e+ with its
@@ -522,6 +639,7 @@ __global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t*
     *nsecTotal += secs;
     // the host reads the next round's track count while the scatter runs
     hostCount[0] = carryE + carryG + k;
+    hostCount[2] = elen;  // its e-/e+ part
     __threadfence_system();
     hostCount[1] = tag;
   }
@@ -630,8 +748,8 @@ struct Arena {
   }
 };
 
-// Two words of pinned, device-mapped, coherent host memory: the next round's
-// track count and the tag of the round that wrote it. Freed on every exit
+// Three words of pinned, device-mapped, coherent host memory: the next round's
+// track count, the tag of the round that wrote it and the count's e-/e+ part. Freed on every exit
 // path (after the stream has drained).
 struct HostCount {
   uint32_t* p = nullptr;  // host address
@@ -640,6 +758,7 @@ struct HostCount {
     if (hipHostMalloc(reinterpret_cast<void**>(&p), 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return false;
     p[0] = 0u;
     p[1] = 0u;
+    p[2] = 0u;
     return hipHostGetDevicePointer(reinterpret_cast<void**>(&d), p, 0) == hipSuccess;
   }
   ~HostCount() {
@@ -658,6 +777,7 @@ struct HostCount {
     }
   }
   uint32_t count() const { return __atomic_load_n(&p[0], __ATOMIC_ACQUIRE); }
+  uint32_t electrons() const { return __atomic_load_n(&p[2], __ATOMIC_ACQUIRE); }
 };
 
 // Synchronises the stream when the run ends, on every path, before the
@@ -719,25 +839,22 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
 
   Arena A;
   Stack s[2];
-  hepem_howfar_state hs;
-  hepem_step_out res;
+  hepem_step_out res;  // the fields the stepping kernel writes and the post kernel reads
+  res.geom_step = nullptr;
+  res.displacement = nullptr;
+  res.stopped = nullptr;
   double *dPlanes, *xend;
-  int8_t* side;
-  uint8_t *keep, *smask, *work;
-  uint32_t *offs, *bsum, *bsec, *total;  // offs: e-/e+ then gamma block offsets (2 x cap); bsum: 2 x nblk
+  int8_t *side, *proc;
+  uint8_t *keep, *smask;
+  uint32_t *offs, *bsum, *bsec, *total, *equeue;  // offs: e-/e+ then gamma block offsets (2 x cap); bsum: 2 x nblk
   unsigned long long *dep, *leak, *nsec;
   const int nblk = (cap + kB - 1) / kB;
-  if (!AllocStack(A, s[0], cap) || !AllocStack(A, s[1], cap) || !A.get(hs.geom_step, cap) || !A.get(hs.range, cap) ||
-      !A.get(hs.mfp, 4 * (size_t)cap) || !A.get(hs.winner, cap) || !A.get(hs.msc_lambtr1, cap) ||
-      !A.get(hs.msc_true_step, cap) || !A.get(hs.msc_zpath, cap) || !A.get(hs.msc_par1, cap) || !A.get(hs.msc_par2, cap) ||
-      !A.get(hs.msc_par3, cap) || !A.get(hs.msc_active, cap) || !A.get(hs.msc_displace, cap) || !A.get(hs.pe_xsec, cap) ||
-      !A.get(res.true_step, cap) || !A.get(res.geom_step, cap) || !A.get(res.edep, cap) ||
-      !A.get(res.displacement, 3 * (size_t)cap) || !A.get(res.winner, cap) || !A.get(res.stopped, cap) ||
-      !A.get(res.nsec, cap) || !A.get(res.sec_ptype, 2 * (size_t)cap) || !A.get(res.sec_ekin, 2 * (size_t)cap) ||
-      !A.get(res.sec_dir, 6 * (size_t)cap) || !A.get(dPlanes, planes.size()) || !A.get(xend, cap) || !A.get(side, cap) ||
-      !A.get(keep, cap) || !A.get(smask, cap) || !A.get(offs, 2 * (size_t)cap) || !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) ||
-      !A.get(dep, (size_t)span * nbins) || !A.get(leak, (size_t)span) || !A.get(nsec, 1) ||
-      !A.get(work, hepem_step_work_bytes(cap)))
+  if (!AllocStack(A, s[0], cap) || !AllocStack(A, s[1], cap) || !A.get(res.true_step, cap) || !A.get(res.edep, cap) ||
+      !A.get(res.winner, cap) || !A.get(res.nsec, cap) || !A.get(res.sec_ptype, 2 * (size_t)cap) ||
+      !A.get(res.sec_ekin, 2 * (size_t)cap) || !A.get(res.sec_dir, 6 * (size_t)cap) || !A.get(dPlanes, planes.size()) ||
+      !A.get(xend, cap) || !A.get(side, cap) || !A.get(proc, cap) || !A.get(keep, cap) || !A.get(smask, cap) || !A.get(offs, 2 * (size_t)cap) ||
+      !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) || !A.get(equeue, 1) || !A.get(dep, (size_t)span * nbins) ||
+      !A.get(leak, (size_t)span) || !A.get(nsec, 1))
     return HEPEM_ERR_ALLOC;
   HostCount hc;  // next round's track count, published by k_scan_sums
   if (!hc.alloc()) return HEPEM_ERR_ALLOC;
@@ -787,6 +904,18 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   }
   const bool woodcock = g.wdtMin > 0.0;
   const bool fp64 = HepemPrecision(ctx) == HEPEM_FP64;
+  // one resident round of blocks of the e-/e+ stepping kernel
+  int residentE = 0;
+  {
+    int dev = 0, cus = 0, perCu = 0;
+    SH_OK(hipGetDevice(&dev));
+    SH_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (fp64)
+      SH_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_sh_electron<double>, kB, 0));
+    else
+      SH_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_sh_electron<float>, kB, 0));
+    residentE = cus * perCu > 0 ? cus * perCu : 1024;
+  }
 
   std::vector<Kahan> ks(nbins), ks2(nbins);
   Kahan kleak, ktot;
@@ -805,6 +934,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
     hipLaunchKernelGGL(k_sh_fill64, dim3((ns + kB - 1) / kB), dim3(kB), 0, st, reinterpret_cast<int64_t*>(leak), ns);
     int next = 0;  // next event of this span to stack
     int n = 0;     // tracks in flight
+    int ne = 0;    // the e-/e+ among them (the first ne slots)
     int cur = 0;
     int rounds = 0;
     int sinceInject = 0;  // rounds stepped since primaries were last stacked
@@ -819,6 +949,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
         r->tracks += k;
         next += k;
         n = k;
+        ne = su->primary != 2 ? k : 0;
         rounds = 0;
         sinceInject = 0;
       }
@@ -830,21 +961,32 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       Stack& a = s[cur];
       Stack& b = s[cur ^ 1];
       a.t.n = n;
-      hs.n = n;
       const int grid = (n + kB - 1) / kB;
-      int rc = hepem_howfar(ctx, seed, &a.t, &hs, work, stream);
-      if (rc != HEPEM_OK) return rc;
-      if (woodcock && !fp64)
-        hipLaunchKernelGGL((k_sh_geometry<float, true>), dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side,
-                           HepemTables32(ctx), seed);
-      else if (woodcock)
-        hipLaunchKernelGGL((k_sh_geometry<double, true>), dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side,
-                           HepemTables64(ctx), seed);
-      else
-        hipLaunchKernelGGL((k_sh_geometry<double, false>), dim3(grid), dim3(kB), 0, st, a, g, hs, xend, side,
-                           HepemTables64(ctx), seed);
-      rc = hepem_perform(ctx, seed, &a.t, &hs, &res, work, stream);
-      if (rc != HEPEM_OK) return rc;
+      // the e-/e+ part of the stack comes first
+      const int ge = (ne + kB - 1) / kB, gg = (n - ne + kB - 1) / kB;
+      if (ge > 0) {
+        const int gq = ge < residentE ? ge : residentE;
+        hipLaunchKernelGGL(k_sh_fill32, dim3(1), dim3(64), 0, st, equeue, 1, 0u);
+        if (fp64) {
+          hipLaunchKernelGGL(k_sh_electron<double>, dim3(gq), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables64(ctx),
+                             seed, ne, equeue);
+          hipLaunchKernelGGL(k_sh_electron_final<double>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables64(ctx), seed, ne);
+        } else {
+          hipLaunchKernelGGL(k_sh_electron<float>, dim3(gq), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables32(ctx),
+                             seed, ne, equeue);
+          hipLaunchKernelGGL(k_sh_electron_final<float>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables32(ctx), seed, ne);
+        }
+      }
+      if (gg > 0) {
+        if (fp64 && woodcock)
+          hipLaunchKernelGGL((k_sh_gamma<double, true>), dim3(gg), dim3(kB), 0, st, a, g, res, xend, side, HepemTables64(ctx), seed, ne);
+        else if (fp64)
+          hipLaunchKernelGGL((k_sh_gamma<double, false>), dim3(gg), dim3(kB), 0, st, a, g, res, xend, side, HepemTables64(ctx), seed, ne);
+        else if (woodcock)
+          hipLaunchKernelGGL((k_sh_gamma<float, true>), dim3(gg), dim3(kB), 0, st, a, g, res, xend, side, HepemTables32(ctx), seed, ne);
+        else
+          hipLaunchKernelGGL((k_sh_gamma<float, false>), dim3(gg), dim3(kB), 0, st, a, g, res, xend, side, HepemTables32(ctx), seed, ne);
+      }
       hipLaunchKernelGGL(k_sh_post, dim3(grid), dim3(kB), 0, st, a, g, res, xend, side, dep, leak, keep, smask, offs, offs + cap,
                          bsum, bsum + nblk, bsec);
       // primaries of the next events join once the stack runs low, and no
@@ -876,6 +1018,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       if (growRounds < 0 && (int)m < n) growRounds = sinceInject + 1;  // the stack shrank for the first time
       sinceInject = k > 0 ? 0 : sinceInject + 1;
       n = (int)m;
+      ne = (int)hc.electrons();
       cur ^= 1;
     }
     SH_OK(hipMemcpyAsync(hdep.data(), dep, (size_t)ns * nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));

@@ -11,8 +11,8 @@ import math
 
 import numpy as np
 
-from ._native import (HOWFAR_FIELDS, OUT_FIELDS, TRACK_FIELDS, HowFarState, StepOut, Tracks, ELECTRON, POSITRON,
-                      GAMMA)
+from ._native import (HOWFAR_FIELDS, OUT_FIELDS, SUBSTEP_FIELDS, TRACK_FIELDS, HowFarState, StepOut, SubstepState,
+                      Tracks, ELECTRON, POSITRON, GAMMA)
 
 
 def _alloc(kind: str, dtype: str, count: int, device=None):
@@ -86,6 +86,17 @@ class HowFarSoA(SoA):
 
     def struct(self) -> HowFarState:
         h = HowFarState()
+        h.n = self.n
+        for name in self.FIELDS:
+            setattr(h, name, _ptr(self.a[name]))
+        return h
+
+
+class SubstepSoA(SoA):
+    FIELDS = SUBSTEP_FIELDS
+
+    def struct(self) -> SubstepState:
+        h = SubstepState()
         h.n = self.n
         for name in self.FIELDS:
             setattr(h, name, _ptr(self.a[name]))

@@ -136,6 +136,28 @@ typedef struct hepem_howfar_state {
   double*   pe_xsec;           /* gamma: photoelectric macroscopic cross section  */
 } hepem_howfar_state;
 
+/* What one e-/e+ step carries across its MSC sub-steps when the caller drives
+ * G4HepEmTrackingManager::TrackElectron's loop itself
+ * (G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:427-606): the locals of that
+ * loop, per track. Written by the hepem_electron_step_* / _substep_* calls;
+ * the caller's geometry sets left_world. */
+typedef struct hepem_substep_state {
+  int32_t   n;
+  int32_t*  discrete_winner;   /* the process HowFarToDiscreteInteraction chose (iDProc) */
+  double*   step_left;         /* what is left of the discrete step (stepLimitLeft) */
+  double*   true_path;         /* summed true sub-step lengths (totalTruePathLength) */
+  double*   geom_path;         /* summed geometrical sub-step lengths              */
+  double*   mean_eloss;        /* summed mean energy losses (totalEloss)           */
+  double*   pre_ekin;          /* kinetic energy at the step's pre-step point      */
+  double*   displacement;      /* [3][n] MSC displacement of the last sub-step (lab
+                                  frame; 0 when none was sampled)                  */
+  uint8_t*  active;            /* 1: the track takes the current sub-step          */
+  uint8_t*  more;              /* 1: another sub-step follows (continueStepping)   */
+  uint8_t*  stopped;           /* the track stopped in a sub-step                  */
+  uint8_t*  left_world;        /* set by the caller: the last move left the world  */
+  int32_t*  nsub;              /* sub-steps taken                                  */
+} hepem_substep_state;
+
 /* One batch of the batched launch. */
 typedef struct hepem_batch_desc {
   hepem_tracks   in;
@@ -274,6 +296,60 @@ int  hepem_gamma_howfar(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, he
                         void* work, void* hip_stream);
 int  hepem_gamma_perform(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
                          hepem_step_out* res, void* work, void* hip_stream);
+
+/* ---- TrackElectron's MSC sub-steps, for callers with their own geometry ----
+ * G4HepEmTrackingManager::TrackElectron (G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:
+ * 408-635) does not call HowFar and Perform: it drives the fine-grained
+ * G4HepEmElectronManager functions (G4HepEm/G4HepEmRun/include/G4HepEmElectronManager.hh:
+ * 90-206) and, where the region sets fIsMultipleStepsInMSCTrans, repeats the
+ * MSC part of the step in sub-steps until the discrete step is used up. A
+ * caller with that loop binds to these calls, on batches of e-/e+ (other slots
+ * are rejected: hs->winner = -99, ss->active = 0):
+ *
+ *   hepem_electron_step_begin       pre-step point: the interaction lengths
+ *                                   and HowFarToDiscreteInteraction (:427-443);
+ *                                   t->safety must be the pre-step safety (0
+ *                                   on a boundary) and stays so for the step;
+ *   loop while hepem_electron_substep_active reports tracks:
+ *     hepem_electron_substep_howfar HowFarToMSC of the active tracks
+ *                                   (:446-452): hs->geom_step = proposed move;
+ *     (caller)                      moves each active track at most
+ *                                   hs->geom_step, writes the length moved to
+ *                                   hs->geom_step and the post-move boundary
+ *                                   flag to t->on_boundary, sets
+ *                                   ss->left_world for a track that left the
+ *                                   world (:456-486);
+ *     hepem_electron_substep_along  UpdatePStepLength, UpdateNumIALeft,
+ *                                   ApplyMeanEnergyLoss, SampleMSC (:490-521)
+ *                                   and the next sub-step's set-up (:569-594);
+ *                                   ss->displacement is the MSC displacement
+ *                                   the caller applies off a boundary
+ *                                   (:523-566);
+ *   hepem_electron_step_end         the loss fluctuation on the summed mean
+ *                                   loss (:597-606), at-rest annihilation or
+ *                                   PerformDiscrete (:611-650, winner 3: the
+ *                                   interaction-length reset and the delta test,
+ *                                   the caller's nuclear final state); fills res
+ *                                   (true_step = summed true path, geom_step =
+ *                                   summed moves, displacement = 0).
+ *
+ * `work`: device memory of at least hepem_step_work_bytes(0) bytes for the
+ * loop's counters (NULL: the context's own buffer, whose count
+ * hepem_electron_substep_active cannot read).
+ * With sub-steps off in every region and no track leaving the world, begin +
+ * substep_howfar + substep_along + end equal hepem_electron_howfar +
+ * hepem_electron_perform with the same geometry bit for bit. */
+int  hepem_electron_step_begin(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                               hepem_substep_state* ss, void* work, void* hip_stream);
+int  hepem_electron_substep_howfar(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                                   hepem_substep_state* ss, void* work, void* hip_stream);
+int  hepem_electron_substep_along(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, hepem_howfar_state* hs,
+                                  hepem_substep_state* ss, void* work, void* hip_stream);
+/* Tracks that take another sub-step after the last hepem_electron_step_begin
+ * or hepem_electron_substep_along on `work`. Synchronises the stream. */
+int  hepem_electron_substep_active(const void* work, uint32_t* nactive, void* hip_stream);
+int  hepem_electron_step_end(const hepem_ctx* ctx, uint64_t seed, hepem_tracks* t, const hepem_howfar_state* hs,
+                             hepem_substep_state* ss, hepem_step_out* res, void* work, void* hip_stream);
 
 /* ---- many independent batches in one launch -------------------------------
  * A batch plan holds, in device memory, the descriptors of nbatch batches of

@@ -1366,6 +1366,238 @@ static int ElectronPerform(const struct G4HepEmData* d, const struct G4HepEmPara
   return 0;
 }
 
+/* ---- one e-/e+ step as G4HepEmTrackingManager::TrackElectron takes it ---------------
+   G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:408-635, with the caller's geometry a slab
+   lo <= x <= hi across the x axis. HowFarToDiscreteInteraction once; then, while MSC
+   limits the sub-step, the region sets fIsMultipleStepsInMSCTrans and no plane is
+   reached: HowFarToMSC, the straight move to the next plane at most, UpdatePStepLength,
+   UpdateNumIALeft, ApplyMeanEnergyLoss and SampleMSC with the MSC displacement applied
+   under the safety rule (:527-566); the next sub-step is what is left of the discrete
+   step (:569-594). Then the loss fluctuation on the summed mean loss from the step's
+   pre-step energy (:599-606), and at-rest annihilation or PerformDiscrete (:618-650,
+   with a lepto-nuclear process present: winner 3 resets its interaction length and
+   draws the delta test). A step that crosses a plane out of the world ends there
+   with no along-step physics of that sub-step (:463-472). */
+#define SLAB_MAX_SUBSTEPS 65536  /* the reference's loop is unbounded */
+
+typedef struct oslab {
+  double x, lo, hi;   /* pre-step position and planes */
+  int exit_lo, exit_hi;
+  int side, left;     /* out: the step ended on hi (+1) or lo (-1); left the world */
+  double geom_path;   /* out: summed geometrical sub-steps */
+  double true_path;   /* out: totalTruePathLength */
+  int nsub;           /* out: sub-steps */
+} oslab;
+
+static int ElectronSlabStep(const struct G4HepEmData* d, const struct G4HepEmParameters* p, otl* tl, oslab* g) {
+  oel* el = &tl->el;
+  otrack* t = &el->t;
+  const int isEl = t->charge < 0.0;
+  const int ireg = d->fTheMatCutData->fMatCutData[t->imc].fG4RegionIndex;
+  int cont = p->fParametersPerRegion[ireg].fIsMultipleStepsInMSCTrans;
+  double x = g->x;
+  {
+    const double saf = dmin(x - g->lo, g->hi - x);
+    t->safety = t->on_boundary ? 0.0 : dmax(saf, 0.0);
+  }
+  const double preStepEkin = t->ekin;
+  const double preStepLogEkin = GetLogEKin(t);
+  for (int ip = 0; ip < 4; ++ip)
+    if (t->nial[ip] <= 0.) t->nial[ip] = -HLog(flat(tl->rng));
+  HowFarToDiscreteInteraction(d, p, el);
+  const int iDProc = t->winner;
+  double stepLeft = el->pstep;
+  double totalTrue = 0.0, totalEloss = 0.0;
+  int stopped = 0;
+  el->pre_ekin = t->ekin;
+  el->pre_lekin = GetLogEKin(t);
+  g->side = 0; g->left = 0; g->geom_path = 0.0; g->nsub = 0;
+  for (int sub = 1;; ++sub) {
+    HowFarToMSC(d, p, el, tl->rng);
+    if (t->winner != -2) cont = 0;
+    const double physicalStep = t->gstep;
+    const double ux = t->dir[0];
+    double dist = DBL_MAX;
+    int sd = 0;
+    if (ux > 0.0) { dist = (g->hi - x) / ux; sd = 1; }
+    else if (ux < 0.0) { dist = (g->lo - x) / ux; sd = -1; }
+    const int limited = dist < physicalStep;
+    const double finalStep = limited ? dist : physicalStep;
+    if (limited) {
+      x = sd > 0 ? g->hi : g->lo;
+    } else {
+      x = x + finalStep * ux;
+      x = x < g->lo ? g->lo : (x > g->hi ? g->hi : x);
+    }
+    g->geom_path += finalStep;
+    g->nsub = sub;
+    if (limited) {
+      cont = 0;
+      g->side = sd;
+      if ((sd > 0 && g->exit_hi) || (sd < 0 && g->exit_lo)) { g->left = 1; break; }
+    }
+    t->gstep = finalStep;
+    t->on_boundary = limited;
+    if (finalStep > 0) {
+      do {
+        UpdatePStepLength(el);
+        const double pStep = el->pstep;
+        totalTrue += pStep;
+        if (pStep <= 0.0) break;
+        UpdateNumIALeft(el);
+        stopped = ApplyMeanEnergyLoss(d, p, el);
+        totalEloss += t->edep;
+        if (stopped) { cont = 0; break; }
+        SampleMSC(d, p, el, tl->rng);
+        if (!limited) {
+          const double d0 = el->msc.displ[0], d1 = el->msc.displ[1], d2 = el->msc.displ[2];
+          const double dl2 = d0 * d0 + d1 * d1 + d2 * d2;
+          if (dl2 > 5.0e-8 * 5.0e-8) {
+            const double dispR = sqrt(dl2);
+            const double ps = 0.99 * dmax(dmin(x - g->lo, g->hi - x), 0.0);
+            if (ps > 0.0 && dispR <= ps) x += d0;
+            else if (dispR < ps) x += d0;
+            else if (ps > 5.0e-8) x += d0 * (ps / dispR);
+          }
+        }
+      } while (0);
+      if (cont) {
+        t->edep = 0.0;
+        t->winner = iDProc;
+        el->pre_ekin = t->ekin;
+        el->pre_lekin = GetLogEKin(t);
+        const double pStep = el->pstep;
+        stepLeft -= pStep;
+        el->pstep = stepLeft;
+        t->gstep = stepLeft;
+        el->range -= pStep;
+      }
+    }
+    if (!cont || sub >= SLAB_MAX_SUBSTEPS) break;
+  }
+  g->x = x;
+  g->true_path = totalTrue;
+  t->edep = totalEloss;
+  if (!stopped) {
+    el->pre_ekin = preStepEkin;
+    el->pre_lekin = preStepLogEkin;
+    stopped = SampleLossFluctuations(d, p, el, tl->rng);
+  }
+  if (stopped) {
+    if (!isEl) AnnihilateAtRest(tl);
+  } else if (!g->left) {
+    PerformDiscrete(d, p, tl);
+  }
+  return stopped;
+}
+
+typedef struct eslab_job {
+  const struct G4HepEmData* d; const struct G4HepEmParameters* p; uint64_t seed;
+  hepem_tracks* t; hepem_step_out* res; double* x; const double* lo; const double* hi; const uint8_t* exitf;
+  int8_t* side; uint8_t* left; int32_t* nsub; int begin, end;
+} eslab_job;
+
+static void* eslab_worker(void* arg) {
+  const eslab_job* J = (const eslab_job*)arg;
+  hepem_tracks* T = J->t;
+  hepem_step_out* res = J->res;
+  const int n = T->n;
+  for (int i = J->begin; i < J->end; ++i) {
+    orng rng;
+    rng.key[0] = (uint32_t)J->seed;
+    rng.key[1] = (uint32_t)(J->seed >> 32);
+    rng.id = (uint32_t)T->id[i];
+    rng.hi = T->rng_stream ? T->rng_stream[i] : 0u;
+    rng.ctr = T->rng_ctr[i];
+    rng.has_gauss = T->rng_has_gauss[i];
+    rng.gauss = T->rng_gauss[i];
+    otl tl;
+    memset(&tl, 0, sizeof(tl));
+    tl.rng = &rng;
+    otrack* t = &tl.el.t;
+    TrackReset(t);
+    MscReset(&tl.el.msc);
+    t->charge = T->ptype[i] == HEPEM_ELECTRON ? -1.0 : 1.0;
+    tl.el.msc.init_range = T->msc_init_range[i];
+    tl.el.msc.dyn_range_factor = T->msc_dyn_range_factor[i];
+    tl.el.msc.tlimit_min = T->msc_tlimit_min[i];
+    tl.el.msc.first_step = T->msc_first_step[i];
+    SetEKin(t, T->ekin[i]);
+    for (int c = 0; c < 3; ++c) t->dir[c] = T->dir[c * n + i];
+    t->imc = T->imc[i];
+    t->on_boundary = T->on_boundary[i];
+    for (int k = 0; k < 4; ++k) t->nial[k] = T->num_ia_left[k * n + i];
+    oslab g;
+    g.x = J->x[i]; g.lo = J->lo[i]; g.hi = J->hi[i];
+    g.exit_lo = J->exitf[i] & 1; g.exit_hi = (J->exitf[i] >> 1) & 1;
+    const int stopped = ElectronSlabStep(J->d, J->p, &tl, &g);
+    T->ekin[i] = t->ekin;
+    for (int c = 0; c < 3; ++c) T->dir[c * n + i] = t->dir[c];
+    for (int k = 0; k < 4; ++k) T->num_ia_left[k * n + i] = t->nial[k];
+    T->safety[i] = t->safety;
+    T->on_boundary[i] = (uint8_t)t->on_boundary;
+    T->msc_init_range[i] = tl.el.msc.init_range;
+    T->msc_dyn_range_factor[i] = tl.el.msc.dyn_range_factor;
+    T->msc_tlimit_min[i] = tl.el.msc.tlimit_min;
+    T->msc_first_step[i] = (uint8_t)tl.el.msc.first_step;
+    T->rng_ctr[i] = rng.ctr;
+    T->rng_gauss[i] = rng.gauss;
+    T->rng_has_gauss[i] = (uint8_t)rng.has_gauss;
+    J->x[i] = g.x;
+    J->side[i] = (int8_t)g.side;
+    J->left[i] = (uint8_t)g.left;
+    J->nsub[i] = g.nsub;
+    res->true_step[i] = g.true_path;
+    res->geom_step[i] = g.geom_path;
+    res->edep[i] = t->edep;
+    for (int c = 0; c < 3; ++c) res->displacement[c * n + i] = 0.0;
+    res->winner[i] = t->winner;
+    res->stopped[i] = (uint8_t)(t->ekin <= 0.0);  /* stopped or killed in the step */
+    (void)stopped;
+    int ns = 0;
+    osec* list[4];
+    for (int k = 0; k < tl.nsec_el; ++k) list[ns++] = &tl.sec_el[k];
+    for (int k = 0; k < tl.nsec_gm; ++k) list[ns++] = &tl.sec_gm[k];
+    res->nsec[i] = ns;
+    for (int s = 0; s < 2; ++s) {
+      res->sec_ptype[s * n + i] = s < ns ? (int8_t)list[s]->ptype : -1;
+      res->sec_ekin[s * n + i] = s < ns ? list[s]->ekin : 0.0;
+      for (int c = 0; c < 3; ++c) res->sec_dir[(c * 2 + s) * n + i] = s < ns ? list[s]->dir[c] : 0.0;
+    }
+  }
+  return NULL;
+}
+
+int oracle_electron_slab_step(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed,
+                              hepem_tracks* t, hepem_step_out* res, double* x, const double* lo, const double* hi,
+                              const uint8_t* exitf, int8_t* side, uint8_t* left, int32_t* nsub, int nthreads) {
+  if (!d || !p || !t || !res || !x || !lo || !hi || !exitf || !side || !left || !nsub || t->n < 0) return -1;
+  const int n = t->n;
+  for (int i = 0; i < n; ++i)
+    if (t->ptype[i] == HEPEM_GAMMA || t->imc[i] < 0 || t->imc[i] >= d->fTheMatCutData->fNumMatCutData || !(t->ekin[i] > 0.0))
+      return -1;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if (nthreads > n) nthreads = n > 0 ? n : 1;
+  eslab_job jobs[256];
+  pthread_t th[256];
+  const int chunk = (n + nthreads - 1) / nthreads;
+  for (int k = 0; k < nthreads; ++k) {
+    eslab_job* J = &jobs[k];
+    J->d = d; J->p = p; J->seed = seed; J->t = t; J->res = res; J->x = x; J->lo = lo; J->hi = hi; J->exitf = exitf;
+    J->side = side; J->left = left; J->nsub = nsub;
+    J->begin = k * chunk < n ? k * chunk : n;
+    J->end = (k + 1) * chunk < n ? (k + 1) * chunk : n;
+  }
+  if (nthreads == 1) {
+    eslab_worker(&jobs[0]);
+    return 0;
+  }
+  for (int k = 0; k < nthreads; ++k) pthread_create(&th[k], NULL, eslab_worker, &jobs[k]);
+  for (int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+  return 0;
+}
+
 /* ---- G4HepEmGammaManager.icc --------------------------------------------------- */
 static double GetMacXSecPE(const struct G4HepEmData* d, int imat, double ekin) { /* :155-170 */
   const struct G4HepEmMatData* md = &d->fTheMaterialData->fMaterialData[imat];
@@ -2149,7 +2381,7 @@ static void sh_event(sh_job* J, int ev) {
     stk[top++] = t;
     J->tracks++;
   }
-  /* one-track SoA views for step_one */
+  /* one-track SoA views for the gamma steps (step_one) */
   int8_t a_pt; int32_t a_id, a_imc; uint32_t a_hi, a_ctr; double a_e, a_dir[3], a_saf, a_nial[4], a_mir, a_mdrf, a_mtm, a_g;
   uint8_t a_onb, a_mfs, a_hg;
   double r_ts, r_gs, r_ed, r_disp[3], r_sek[2], r_sdir[6]; int32_t r_w, r_ns; uint8_t r_st; int8_t r_spt[2];
@@ -2167,84 +2399,122 @@ static void sh_event(sh_job* J, int ev) {
     for (;;) {
       int l = t.layer;
       const double lo = P[l], hi = P[l + 1];
-      double saf = t.x - lo < hi - t.x ? t.x - lo : hi - t.x;
-      t.safety = saf > 0.0 ? saf : 0.0;
-      a_pt = t.ptype; a_id = t.id; a_hi = t.hi; a_imc = t.imc; a_e = t.ekin;
-      a_dir[0] = t.dir[0]; a_dir[1] = t.dir[1]; a_dir[2] = t.dir[2]; a_saf = t.safety; a_onb = t.onb;
-      for (int k = 0; k < 4; ++k) a_nial[k] = t.nial[k];
-      a_mir = t.mir; a_mdrf = t.mdrf; a_mtm = t.mtm; a_mfs = t.mfs; a_ctr = t.ctr; a_g = t.gauss; a_hg = t.hg;
-      /* geometry: distance to the next plane along the pre-step direction */
-      const double ux = t.dir[0];
-      int sd = 0;
-      glim = 1.0e300;
-      if (ux > 0.0) { glim = (hi - t.x) / ux; sd = 1; }
-      else if (ux < 0.0) { glim = (lo - t.x) / ux; sd = -1; }
-      W.x = t.x; W.layer = l; W.done = 0;
-      step_one(&SJ, 0);
-      J->steps++;
-      const int onb = a_onb;
-      double x;
-      if (W.done) {
-        /* Woodcock step: the pre-step point moves to where the step ended, an
-           interaction point or the stack's last slab before its exit */
-        if (W.boundary) {
-          sd = ux > 0.0 ? 1 : (ux < 0.0 ? -1 : 1);
-          l = sd > 0 ? nbins - 1 : 0;
-        } else {
-          l = W.layer_out;
+      double x;            /* post-step position */
+      int onb, sd;         /* post-step boundary flag; +1/-1: upper/lower plane */
+      int left = 0;        /* an e-/e+ that left the world (the stack) in the step */
+      double ed, ekin;     /* the step's deposit; post-step energy */
+      int ns = 0;
+      int8_t spt[2] = {-1, -1};
+      double sek[2] = {0., 0.}, sdir[6] = {0., 0., 0., 0., 0., 0.};
+      int l0 = l;          /* the pre-step slab: scoring and the couple's cuts */
+      if (t.ptype != 2) {
+        /* e-/e+: the whole TrackElectron step, sub-steps and MSC displacements included */
+        orng rng;
+        rng.key[0] = (uint32_t)J->seed; rng.key[1] = (uint32_t)(J->seed >> 32);
+        rng.id = (uint32_t)t.id; rng.hi = t.hi; rng.ctr = t.ctr; rng.has_gauss = t.hg; rng.gauss = t.gauss;
+        otl tl;
+        memset(&tl, 0, sizeof(tl));
+        tl.rng = &rng;
+        otrack* e = &tl.el.t;
+        TrackReset(e);
+        MscReset(&tl.el.msc);
+        e->charge = t.ptype == HEPEM_ELECTRON ? -1.0 : 1.0;
+        tl.el.msc.init_range = t.mir; tl.el.msc.dyn_range_factor = t.mdrf; tl.el.msc.tlimit_min = t.mtm;
+        tl.el.msc.first_step = t.mfs;
+        SetEKin(e, t.ekin);
+        for (int c = 0; c < 3; ++c) e->dir[c] = t.dir[c];
+        e->imc = t.imc;
+        e->on_boundary = t.onb;
+        for (int k = 0; k < 4; ++k) e->nial[k] = t.nial[k];
+        oslab g;
+        g.x = t.x; g.lo = lo; g.hi = hi; g.exit_lo = l == 0; g.exit_hi = l == nbins - 1;
+        ElectronSlabStep(J->d, J->p, &tl, &g);
+        x = g.x; sd = g.side; onb = sd != 0; left = g.left;
+        ed = e->edep; ekin = e->ekin;
+        for (int k = 0; k < tl.nsec_el + tl.nsec_gm && ns < 2; ++k, ++ns) {
+          const osec* sc = k < tl.nsec_el ? &tl.sec_el[k] : &tl.sec_gm[k - tl.nsec_el];
+          spt[ns] = (int8_t)sc->ptype; sek[ns] = sc->ekin;
+          for (int c = 0; c < 3; ++c) sdir[c * 2 + ns] = sc->dir[c];
         }
-      }
-      const int l0 = l;  /* the pre-step slab: scoring and the couple's cuts */
-      if (W.done && !W.boundary) {
-        x = W.x_out;
-      } else if (onb) {
-        x = sd > 0 ? P[l + 1] : P[l];
+        for (int c = 0; c < 3; ++c) t.dir[c] = e->dir[c];
+        for (int k = 0; k < 4; ++k) t.nial[k] = e->nial[k];
+        t.mir = tl.el.msc.init_range; t.mdrf = tl.el.msc.dyn_range_factor; t.mtm = tl.el.msc.tlimit_min;
+        t.mfs = (uint8_t)tl.el.msc.first_step;
+        t.ctr = rng.ctr; t.gauss = rng.gauss; t.hg = (uint8_t)rng.has_gauss;
       } else {
-        x = t.x + r_gs * ux;
-        x = x < lo ? lo : (x > hi ? hi : x);
-        if (t.ptype != 2) {
-          const double d0 = r_disp[0], d1 = r_disp[1], d2 = r_disp[2];
-          const double dl2 = d0 * d0 + d1 * d1 + d2 * d2;
-          if (dl2 > SH_GEOM_MIN * SH_GEOM_MIN) {
-            const double dispR = sqrt(dl2);
-            double ps = x - lo < hi - x ? x - lo : hi - x;
-            ps = 0.99 * (ps > 0.0 ? ps : 0.0);
-            if (ps > 0.0 && dispR <= ps) x += d0;
-            else if (dispR < ps) x += d0;
-            else if (ps > SH_GEOM_MIN) x += d0 * (ps / dispR);
+        /* gamma: HowFar, the cut at the next plane (or a Woodcock step), Perform */
+        double saf = t.x - lo < hi - t.x ? t.x - lo : hi - t.x;
+        t.safety = saf > 0.0 ? saf : 0.0;
+        a_pt = t.ptype; a_id = t.id; a_hi = t.hi; a_imc = t.imc; a_e = t.ekin;
+        a_dir[0] = t.dir[0]; a_dir[1] = t.dir[1]; a_dir[2] = t.dir[2]; a_saf = t.safety; a_onb = t.onb;
+        for (int k = 0; k < 4; ++k) a_nial[k] = t.nial[k];
+        a_mir = t.mir; a_mdrf = t.mdrf; a_mtm = t.mtm; a_mfs = t.mfs; a_ctr = t.ctr; a_g = t.gauss; a_hg = t.hg;
+        const double ux = t.dir[0];
+        sd = 0;
+        glim = 1.0e300;
+        if (ux > 0.0) { glim = (hi - t.x) / ux; sd = 1; }
+        else if (ux < 0.0) { glim = (lo - t.x) / ux; sd = -1; }
+        W.x = t.x; W.layer = l; W.done = 0;
+        step_one(&SJ, 0);
+        onb = a_onb;
+        if (W.done) {
+          /* Woodcock step: the pre-step point moves to where the step ended, an
+             interaction point or the stack's last slab before its exit */
+          if (W.boundary) {
+            sd = ux > 0.0 ? 1 : (ux < 0.0 ? -1 : 1);
+            l = sd > 0 ? nbins - 1 : 0;
+          } else {
+            l = W.layer_out;
           }
         }
-      }
-      const double ekin = a_e;
-      int alive = ekin > 0.0, left = 0;
-      if (alive && onb) {
-        l += sd;
-        if (l < 0 || l >= nbins) {
-          const double e = ekin + (t.ptype == 1 ? 2.0 * 0.51099895000 : 0.0);
-          J->leak[ev] += (int64_t)llrint(e * SH_FIX);
-          alive = 0; left = 1; l -= sd;
+        l0 = l;
+        if (W.done && !W.boundary) x = W.x_out;
+        else if (onb) x = sd > 0 ? P[l + 1] : P[l];
+        else {
+          x = t.x + r_gs * ux;
+          x = x < lo ? lo : (x > hi ? hi : x);
         }
+        if (!onb) sd = 0;
+        ed = r_ed; ekin = a_e;
+        ns = r_ns;
+        for (int k = 0; k < ns; ++k) {
+          spt[k] = r_spt[k]; sek[k] = r_sek[k];
+          for (int c = 0; c < 3; ++c) sdir[c * 2 + k] = r_sdir[c * 2 + k];
+        }
+        t.dir[0] = a_dir[0]; t.dir[1] = a_dir[1]; t.dir[2] = a_dir[2];
+        for (int k = 0; k < 4; ++k) t.nial[k] = a_nial[k];
+        t.ctr = a_ctr; t.gauss = a_g; t.hg = a_hg;
+      }
+      J->steps++;
+      int alive = ekin > 0.0;
+      if (left || (alive && onb && (l + sd < 0 || l + sd >= nbins))) {
+        /* leaves the stack: its kinetic energy, the annihilation energy of an e+
+           in flight and anything its last step created are lost */
+        double e = ekin + (t.ptype == 1 && ekin > 0.0 ? 2.0 * 0.51099895000 : 0.0);
+        for (int k = 0; k < ns; ++k) e += sek[k];
+        J->leak[ev] += (int64_t)llrint(e * SH_FIX);
+        alive = 0; left = 1; ns = 0;
+      } else if (alive && onb) {
+        l += sd;
       }
       const uint64_t sid = ((uint64_t)t.hi << 32) | (uint32_t)t.id;
       const int ns1 = t.nstep + 1;
-      const int ns = left ? 0 : r_ns;
       /* StackSecondaries (G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:1274-1313):
          where the couple's region applies cuts, a secondary below its production
          threshold is deposited at the step instead of stacked */
       int stacked[2] = {1, 1};
-      double ed = r_ed;
       {
         const struct G4HepEmMCCData* mcc = &J->d->fTheMatCutData->fMatCutData[J->su->imc[l0 % nabs]];
         const int apply = J->p->fParametersPerRegion[mcc->fG4RegionIndex].fIsApplyCuts;
         for (int s = 0; s < ns; ++s) {
           int cut = 0;
           if (apply) {
-            if (r_spt[s] == 0) cut = r_sek[s] < mcc->fSecElProdCutE;
-            else if (r_spt[s] == 1) cut = 0.51099895000 < mcc->fSecGamProdCutE && r_sek[s] < mcc->fSecPosProdCutE;
-            else cut = r_sek[s] < mcc->fSecGamProdCutE;
+            if (spt[s] == 0) cut = sek[s] < mcc->fSecElProdCutE;
+            else if (spt[s] == 1) cut = 0.51099895000 < mcc->fSecGamProdCutE && sek[s] < mcc->fSecPosProdCutE;
+            else cut = sek[s] < mcc->fSecGamProdCutE;
           }
           if (cut) {
-            ed += r_spt[s] == 1 ? r_sek[s] + 2.0 * 0.51099895000 : r_sek[s];
+            ed += spt[s] == 1 ? sek[s] + 2.0 * 0.51099895000 : sek[s];
             stacked[s] = 0;
           }
         }
@@ -2263,9 +2533,9 @@ static void sh_event(sh_job* J, int ev) {
         sh_track c;
         memset(&c, 0, sizeof(c));
         const uint64_t cid = sh_child_stream(sid, ns1, s);
-        c.ptype = r_spt[s]; c.id = (int32_t)(uint32_t)cid; c.hi = (uint32_t)(cid >> 32);
-        c.imc = J->su->imc[l % nabs]; c.ekin = r_sek[s];
-        for (int k = 0; k < 3; ++k) c.dir[k] = r_sdir[k * 2 + s];
+        c.ptype = spt[s]; c.id = (int32_t)(uint32_t)cid; c.hi = (uint32_t)(cid >> 32);
+        c.imc = J->su->imc[l % nabs]; c.ekin = sek[s];
+        for (int k = 0; k < 3; ++k) c.dir[k] = sdir[k * 2 + s];
         for (int k = 0; k < 4; ++k) c.nial[k] = -1.0;
         c.mir = 1.0e+21; c.mdrf = 0.04; c.mtm = 1.0E-7; c.mfs = 1;
         c.x = x; c.layer = l; c.nstep = 0;
@@ -2274,9 +2544,7 @@ static void sh_event(sh_job* J, int ev) {
       }
       if (!alive) break;
       /* the track goes on */
-      t.ekin = ekin; t.dir[0] = a_dir[0]; t.dir[1] = a_dir[1]; t.dir[2] = a_dir[2]; t.onb = (uint8_t)onb;
-      for (int k = 0; k < 4; ++k) t.nial[k] = a_nial[k];
-      t.mir = a_mir; t.mdrf = a_mdrf; t.mtm = a_mtm; t.mfs = a_mfs; t.ctr = a_ctr; t.gauss = a_g; t.hg = a_hg;
+      t.ekin = ekin; t.onb = (uint8_t)onb;
       t.x = x; t.layer = l; t.imc = J->su->imc[l % nabs]; t.nstep = ns1;
     }
   }

@@ -55,6 +55,21 @@ int oracle_step_geom(const struct G4HepEmData* d, const struct G4HepEmParameters
                      const hepem_tracks* in, hepem_tracks* out, hepem_step_out* res, const double* geom_limit,
                      int nthreads);
 
+/* One e-/e+ step of each track as G4HepEmTrackingManager::TrackElectron takes it
+ * (G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:408-635), with MSC sub-steps where
+ * the couple's region sets fIsMultipleStepsInMSCTrans, in a slab geometry: track i
+ * is at x[i] in the slab lo[i] <= x <= hi[i] across the x axis, and crossing lo[i]
+ * (bit 0 of exitf[i]) or hi[i] (bit 1) leaves the world. The pre-step safety is the
+ * distance to the nearer plane (0 on a boundary). Updates t in place and x[i] to the
+ * post-step position (MSC displacements applied); side[i] = +1/-1 if the step ended
+ * on hi/lo, else 0; left[i] = 1 if it left the world; nsub[i] = sub-steps taken.
+ * res: true_step = summed true path, geom_step = summed geometrical path,
+ * displacement = 0 (applied), the rest as oracle_step. Every track must be an
+ * e-/e+ with a valid couple and a positive energy. */
+int oracle_electron_slab_step(const struct G4HepEmData* d, const struct G4HepEmParameters* p, uint64_t seed,
+                              hepem_tracks* t, hepem_step_out* res, double* x, const double* lo, const double* hi,
+                              const uint8_t* exitf, int8_t* side, uint8_t* left, int32_t* nsub, int nthreads);
+
 /* CPU harness of hepem_slab_run: the same tracking loop, per event on up to
  * `nthreads` threads. dep: [nevents][nlayers*nabs] and leak: [nevents] in
  * 1e-9 MeV units; steps_tracks (optional): {track-steps, tracks}. */

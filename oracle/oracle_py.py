@@ -130,6 +130,35 @@ class _Base:
         return tout, res
 
 
+    def electron_slab_step(self, tables, seed, tracks: dict, x, lo, hi, exitf, nthreads: int = 1):
+        """One e-/e+ step of every track as G4HepEmTrackingManager::TrackElectron
+        takes it (MSC sub-steps where the region asks for them) in a slab
+        lo <= x <= hi across the x axis; bit 0/1 of exitf: crossing lo/hi
+        leaves the world. Returns (track state dict, results dict, geometry dict
+        with x, side, left, nsub)."""
+        n = len(tracks["ekin"])
+        t = {k: np.array(tracks[k], dtype=dt, copy=True) for k, (dt, _) in TRACK_FIELDS.items()}
+        res = {k: np.zeros(c * n, dtype=dt) for k, (dt, c) in OUT_FIELDS.items()}
+        geo = {"x": np.array(x, dtype=np.float64, copy=True), "side": np.zeros(n, np.int8),
+               "left": np.zeros(n, np.uint8), "nsub": np.zeros(n, np.int32)}
+        lo, hi = _arr(lo, np.float64), _arr(hi, np.float64)
+        ex = _arr(exitf, np.uint8)
+        st, rr = _Tracks(), _Out()
+        st.n = n
+        for k in TRACK_FIELDS:
+            setattr(st, k, t[k].ctypes.data)
+        for k in OUT_FIELDS:
+            setattr(rr, k, res[k].ctypes.data)
+        f = getattr(self.L, self.prefix + "electron_slab_step")
+        f.argtypes = [P, P, C.c_uint64, C.POINTER(_Tracks), C.POINTER(_Out), P, P, P, P, P, P, P, C.c_int]
+        f.restype = C.c_int
+        rc = f(tables.data, tables.pars, C.c_uint64(seed), C.byref(st), C.byref(rr), geo["x"].ctypes.data,
+               lo.ctypes.data, hi.ctypes.data, ex.ctypes.data, geo["side"].ctypes.data, geo["left"].ctypes.data,
+               geo["nsub"].ctypes.data, int(nthreads))
+        assert rc == 0, rc
+        return t, res, geo
+
+
 class Oracle(_Base):
     prefix = "oracle_"
     path = ORACLE_LIB
@@ -181,6 +210,23 @@ class Oracle(_Base):
 class Ref(_Base):
     prefix = "ref_"
     path = REF_LIB
+
+    def slab_shower(self, tables, setup, nevents: int, seed: int, nthreads: int = 1):
+        """Showers in the slab stack with every step taken by the reference's
+        own run-time code (TrackElectron's sub-step loop, TrackGamma,
+        StackSecondaries; no Woodcock tracking): per-event per-slab deposits
+        and leakage (1e-9 MeV units) and (steps, tracks)."""
+        nb = setup.nlayers * setup.nabs
+        dep = np.zeros(nevents * nb, dtype=np.int64)
+        leak = np.zeros(nevents, dtype=np.int64)
+        st = np.zeros(2, dtype=np.int64)
+        f = self.L.ref_slab_shower
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        f.restype = C.c_int
+        rc = f(tables.data, tables.pars, C.byref(setup), nevents, C.c_uint64(seed), dep.ctypes.data, leak.ctypes.data,
+               st.ctypes.data, nthreads)
+        assert rc == 0, rc
+        return dep.reshape(nevents, nb), leak, int(st[0]), int(st[1])
 
 
 def ref_available() -> bool:

@@ -56,6 +56,61 @@ def test_shower_matches_cpu_harness(tables1, ctx1, oracle):
     assert abs(steps - g["steps"]) < 0.02 * steps
 
 
+def paired_chi2(a, b):
+    """chi2 of the per-slab mean differences of paired events (rows = events,
+    the same seeds and streams on both sides)."""
+    d = a - b
+    se2 = d.var(0, ddof=1) / len(d)
+    m = se2 > 0
+    return float(((d.mean(0) ** 2)[m] / se2[m]).sum()), int(m.sum())
+
+
+@pytest.mark.skipif(not __import__("oracle.oracle_py", fromlist=["ref_available"]).ref_available(),
+                    reason="compiled reference (oracle/_ref) not shipped")
+def test_shower_10gev_matches_reference_harness(tables1, ctx1):
+    """200 events of the benchmark's 10 GeV e- showers against the same
+    showers stepped by the reference's own compiled code (TrackElectron's MSC
+    sub-step loop, TrackGamma, StackSecondaries: oracle/ref/ref_harness.cc)."""
+    from g4hepem_amd._native import slab_setup
+    from oracle.oracle_py import Ref
+    nev, e0 = 200, 1.0e4
+    su = slab_setup(LAYERS, THICK, [0, 1], 0, e0, capacity=1 << 23, events_per_chunk=200)
+    _print("shower 10 GeV e-: gpu")
+    g = ctx1.slab_run(su, nev, 31, per_event=True)
+    _print("shower 10 GeV e-: reference harness")
+    dep, leak, steps, tracks = Ref().slab_shower(tables1, su, nev, 31, nthreads=16)
+    for d, lk in ((g["edep_event"], g["leak_event"]), (dep, leak)):
+        assert np.abs((d.sum(1) + lk) / 1e9 - e0).max() < 1e-4
+    same = np.all(g["edep_event"] == dep, axis=1) & (g["leak_event"] == leak)
+    chi2, ndf = paired_chi2(g["edep_event"] / 1e9, dep / 1e9)
+    _print(f"  identical events {same.mean():.3f}, paired chi2/ndf {chi2:.1f}/{ndf}, steps {g['steps']} vs {steps}")
+    assert same.mean() > 0.2
+    assert chi2 < ndf + 5 * np.sqrt(2 * ndf)
+    assert abs(steps - g["steps"]) < 0.01 * steps and abs(tracks - g["tracks"]) < 0.01 * tracks
+
+
+def test_shower_substeps_off_matches_cpu_harness(tables1, oracle):
+    """With fIsMultipleStepsInMSCTrans off in every region (through the JSON
+    form) every e-/e+ step is one HowFar/Perform pass; GPU and CPU harness agree
+    on those tables too, and take more steps than with sub-steps."""
+    import json
+    from g4hepem_amd import HEPEM_FP64, Context
+    from g4hepem_amd._native import Tables, slab_setup
+    doc = json.loads(tables1.to_json())
+    for r in doc["fParameters"]["fParametersPerRegion"]:
+        r["fIsMultipleStepsInMSCTrans"] = False
+    t_off = Tables.from_json(json.dumps(doc))
+    nev, e0 = 100, 1000.0
+    su = slab_setup(LAYERS, THICK, [0, 1], 0, e0, capacity=1 << 21, events_per_chunk=64)
+    g = Context(t_off, HEPEM_FP64).slab_run(su, nev, 8, per_event=True)
+    dep, leak, steps, tracks = oracle.slab_shower(t_off, su, nev, 8, nthreads=16)
+    same = np.all(g["edep_event"] == dep, axis=1) & (g["leak_event"] == leak)
+    _print(f"  sub-steps off: identical events {same.mean():.3f}, steps {g['steps']} vs {steps}")
+    assert same.mean() > 0.5
+    on = Context(tables1, HEPEM_FP64).slab_run(su, nev, 8)
+    assert on["steps"] < g["steps"]
+
+
 @pytest.mark.parametrize("chunk,refill", [(7, 0), (3, 2000), (5, 1 << 30)])
 def test_shower_independent_of_chunking(ctx1, chunk, refill):
     """Per-event results do not depend on how many events are stacked together
