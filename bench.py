@@ -209,7 +209,7 @@ def _timed_showers(ctx, su, nevents: int, seed: int):
     wall = time.perf_counter() - t0
     return g, wall, {"events": nevents, "wall_s": wall, "events_per_s": nevents / wall,
                      "track_steps_per_s": g["steps"] / wall, "steps": g["steps"], "tracks": g["tracks"],
-                     "iterations": g["iterations"], "max_in_flight": g["max_in_flight"]}
+                     "iterations": g["iterations"], "max_in_flight": g["max_in_flight"], "electron_substeps": g["substeps"]}
 
 
 def run_config5(ctx, tables, seed: int, nevents: int, cpu_events: int, ctx32=None, woodcock: bool = True):

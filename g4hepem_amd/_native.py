@@ -60,7 +60,8 @@ class SlabSetup(C.Structure):
 
 class SlabResult(C.Structure):
     _fields_ = [("edep_sum", P), ("edep_sum2", P), ("edep_event", P), ("leak_event", P), ("leak_sum", C.c_double), ("edep_total", C.c_double),
-                ("steps", C.c_int64), ("tracks", C.c_int64), ("iterations", C.c_int32), ("max_in_flight", C.c_int32)]
+                ("steps", C.c_int64), ("tracks", C.c_int64), ("iterations", C.c_int32), ("max_in_flight", C.c_int32),
+                ("substeps", C.c_int64)]
 
 
 def slab_setup(nlayers: int, thickness, imc, primary: int, ekin0: float, capacity: int = 1 << 22,

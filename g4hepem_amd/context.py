@@ -158,7 +158,8 @@ class Context:
         N.check("hepem_slab_run", N.lib().hepem_slab_run(self.h, C.byref(setup), nevents, C.c_uint64(seed), C.byref(r),
                                                          _stream_handle(stream)))
         out = {"edep_sum": s1, "edep_sum2": s2, "leak_sum": r.leak_sum, "edep_total": r.edep_total,
-               "steps": r.steps, "tracks": r.tracks, "iterations": r.iterations, "max_in_flight": r.max_in_flight}
+               "steps": r.steps, "tracks": r.tracks, "iterations": r.iterations, "max_in_flight": r.max_in_flight,
+               "substeps": r.substeps}
         if per_event:
             out["edep_event"], out["leak_event"] = ev.reshape(nevents, nb), lk
         return out

@@ -86,18 +86,35 @@ __device__ inline int AbsorberCouple(const Slabs& g, int a) {
 }
 
 // one stack buffer
+// What an e-/e+ carries between the sub-steps of one TrackElectron step
+// (G4HepEmTrackingManager.cc:427-606), which run one per stepping round: the
+// step's pre-step part (k_sh_el_begin) sets it, each sub-step (k_sh_el_substep)
+// updates it, the step's end (k_sh_el_finish) reads it. Gamma slots do not
+// carry it.
+constexpr uint8_t kInStep = 1;   // between sub-steps of a step (else the next round begins a step)
+constexpr uint8_t kCont = 2;     // continueStepping: MSC may limit the next sub-step too
+constexpr uint8_t kStopped = 4;  // the step's last sub-step stopped the track
+constexpr uint8_t kLeft = 8;     // the step's last sub-step left the world
+struct ElLoop {
+  double* range;       // restricted range, reduced over the sub-steps
+  double* stepLeft;    // what is left of the discrete step
+  double* mfp;         // [4][n] macroscopic mean free paths of the step's pre-step point
+  double* preEkin;     // the step's pre-step energy
+  double* meanLoss;    // summed mean energy losses
+  int32_t* nsub;       // sub-steps taken in the step
+  int8_t* iDProc;      // the discrete process HowFarToDiscreteInteraction chose
+  int8_t* winner;      // the process that limited the last sub-step
+  uint8_t* flags;      // kInStep | kCont | kStopped | kLeft
+};
+
 struct Stack {
   hepem_tracks t;     // t.n = tracks in it (stride of the planar fields)
   double* x;
   int32_t* layer;     // slab index
   int32_t* event;     // deposit row of the event (events resident at once)
   int32_t* nstep;     // steps taken by the track
+  ElLoop lp;          // e-/e+ slots: the state between sub-steps
 };
-
-__global__ void k_sh_fill32(uint32_t* p, int n, uint32_t v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 __global__ void k_sh_fill64(int64_t* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,6 +156,7 @@ __global__ void k_sh_inject(Stack s, Slabs g, int ptype, double ekin, int e0, in
   s.layer[i] = 0;
   s.event[i] = ev0 + j;
   s.nstep[i] = 0;
+  s.lp.flags[i] = 0;
 }
 
 // isotropic safety of a point in slab l: distance to the nearer plane
@@ -307,80 +325,149 @@ __device__ bool WoodcockStep(const hepem::DevTables<R>& T, const Slabs& g, hepem
   return true;
 }
 
-// One step of the e-/e+ slots [0, ne), the track state updated in place: the
-// whole G4HepEmTrackingManager::TrackElectron step with the slab as the
-// caller's geometry (device/dev_tracking.hh), MSC sub-steps while the region
-// asks for them, each MSC displacement applied in the slab, the stack's outer
-// planes as the world's boundary. xend = post-step position; side = +1/-1 when
-// the step ended on the upper or lower plane of its slab, 0 inside. The
-// discrete interaction's final state is left to k_sh_electron_final: proc =
-// the process to sample (0 ionisation, 1 brem, 2 annihilation in flight) or -1.
+// The e-/e+ slots [0, ne) of G4HepEmTrackingManager::TrackElectron's step with
+// the slab as the caller's geometry (device/dev_tracking.hh), one sub-step
+// per stepping round: a slot that is not within a step begins one
+// (k_sh_el_begin), every slot takes one sub-step (k_sh_el_substep), and a slot
+// whose step thereby ended finishes it (k_sh_el_finish, k_sh_electron_final).
+// A slot within a step stays in the stack with its loop state, deposits
+// nothing and creates nothing until its step ends. Every round thus does one
+// uniform unit of work per e-/e+, whatever the length of its sub-step chain.
 //
-// Steps take from one to a few tens of sub-steps, so a lane keeps a track only
-// for its sub-steps: each pass of the loop below does one sub-step of every
-// lane's track, and lanes whose track finished take the next slots of a queue
-// shared by the grid (one atomic per wave and refill). A wave therefore runs
-// sub-steps of many tracks side by side instead of waiting for its longest
-// chain. The grid is one resident round of blocks; every wave leaves the loop
-// once the queue is exhausted and its lanes' tracks are done (each step's
-// sub-steps are bounded by kMaxSubsteps).
+// k_sh_el_begin: the pre-step safety (kept in t.safety for the step), the
+// interaction lengths and HowFarToDiscreteInteraction.
 template <typename R>
-__global__ void __launch_bounds__(kB) k_sh_electron(Stack s, Slabs g, hepem_step_out res, double* __restrict__ xend,
-                                                    int8_t* __restrict__ side, int8_t* __restrict__ proc,
-                                                    hepem::DevTables<R> T, uint64_t seed, int ne,
-                                                    uint32_t* __restrict__ queue) {
+__global__ void __launch_bounds__(kB) k_sh_el_begin(Stack s, Slabs g, hepem::DevTables<R> T, uint64_t seed, int ne) {
   using namespace hepem;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne || (s.lp.flags[i] & kInStep)) return;
   const int n = s.t.n;
   hepem_tracks& t = s.t;
-  const int lane = __lane_id();
-  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  int i = -1;          // slot of this lane's track; -1: none
-  bool drained = false;  // the queue has no slots left (uniform over the wave)
+  const int l = s.layer[i];
   ElState<R> el;
   Rng rng;
+  InitElectron(t, n, i, seed, el, rng);
   StepLoop<R> L;
-  SlabPos p;
-  while (true) {
-    const uint64_t need = drained ? 0ull : __ballot(i < 0);
-    if (need) {
-      const int leader = __ffsll((unsigned long long)need) - 1;
-      uint32_t base = 0u;
-      if (lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(need));
-      base = __shfl(base, leader, 64);
-      if (base + (uint32_t)__popcll(need) >= (uint32_t)ne) drained = true;
-      if (i < 0) {
-        const uint32_t j = base + (uint32_t)__popcll(need & below);
-        if (j < (uint32_t)ne) {
-          i = (int)j;
-          const int l = s.layer[i];
-          InitElectron(t, n, i, seed, el, rng);
-          p = SlabPos{s.x[i], g.plane[l], g.plane[l + 1], l == 0, l == g.nbins - 1, 0};
-          SlabBegin(T, el, L, rng, p);
-        }
-      }
-    }
-    if (!__ballot(i >= 0)) break;
-    if (i >= 0 && SlabSubstep(T, el, L, rng, p)) {
-      SecList<R> sl;
-      sl.ns = 0;
-      proc[i] = (int8_t)LoopFinishSelect(T, el, L, sl, rng);
-      StoreElectronState(t, n, i, el, rng);
-      t.ekin[i] = el.t.ekin;
+  SlabPos p{s.x[i], g.plane[l], g.plane[l + 1], l == 0, l == g.nbins - 1, 0};
+  SlabBegin(T, el, L, rng, p);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) t.dir[c * n + i] = (double)el.t.dir[c];
-      t.on_boundary[i] = p.side != 0 ? 1 : 0;
-      res.true_step[i] = (double)L.truePath;
-      res.edep[i] = el.t.edep;
-      res.winner[i] = el.t.winner;
-      WriteSecondaries(res, n, i, sl);
-      xend[i] = p.x;
-      side[i] = (int8_t)p.side;
-      i = -1;
-    }
+  for (int k = 0; k < 4; ++k) {
+    t.num_ia_left[k * n + i] = (double)el.t.nial[k];
+    s.lp.mfp[k * n + i] = (double)el.t.mfp[k];
   }
+  StoreRng(t, i, rng);
+  t.safety[i] = (double)el.t.safety;
+  s.lp.range[i] = (double)el.range;
+  s.lp.stepLeft[i] = (double)L.stepLeft;
+  s.lp.preEkin[i] = L.preEkin;
+  s.lp.meanLoss[i] = 0.0;
+  s.lp.nsub[i] = 0;
+  s.lp.iDProc[i] = (int8_t)L.iDProc;
+  s.lp.flags[i] = (uint8_t)(kInStep | (L.cont ? kCont : 0));
 }
 
-// The final states of the discrete interactions k_sh_electron selected: the
+// k_sh_el_substep: one sub-step of every slot (SlabSubstep: HowFarToMSC, the
+// move, the along-step part, the MSC displacement). xend = post-sub-step
+// position; side = +1/-1 when it ended on the upper or lower plane of its
+// slab, 0 inside. A slot still within its step gets proc = -1 and no deposit
+// or secondaries this round.
+template <typename R>
+__global__ void __launch_bounds__(kB) k_sh_el_substep(Stack s, Slabs g, hepem_step_out res, double* __restrict__ xend,
+                                                      int8_t* __restrict__ side, int8_t* __restrict__ proc,
+                                                      hepem::DevTables<R> T, uint64_t seed, int ne) {
+  using namespace hepem;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const int n = s.t.n;
+  hepem_tracks& t = s.t;
+  const int l = s.layer[i];
+  ElState<R> el;
+  Rng rng;
+  InitElectron(t, n, i, seed, el, rng);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) el.t.mfp[k] = (R)s.lp.mfp[k * n + i];
+  el.range = (R)s.lp.range[i];
+  el.pstep = (R)s.lp.stepLeft[i];
+  el.t.gstep = el.pstep;  // HowFarToMSC keeps it where MSC is off
+  el.preEkin = el.t.ekin;
+  el.preLEkin = el.t.logE();
+  StepLoop<R> L;
+  L.iDProc = s.lp.iDProc[i];
+  el.t.winner = L.iDProc;
+  L.stepLeft = el.pstep;
+  L.truePath = (R)0.;
+  L.meanLoss = s.lp.meanLoss[i];
+  L.preEkin = s.lp.preEkin[i];
+  L.preLEkin = (R)0.;
+  L.cont = (s.lp.flags[i] & kCont) != 0;
+  L.stopped = false;
+  L.left = false;
+  L.nsub = s.lp.nsub[i];
+  SlabPos p{s.x[i], g.plane[l], g.plane[l + 1], l == 0, l == g.nbins - 1, 0};
+  const bool done = SlabSubstep(T, el, L, rng, p);
+  StoreElectronState(t, n, i, el, rng);
+  t.ekin[i] = el.t.ekin;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) t.dir[c * n + i] = (double)el.t.dir[c];
+  t.on_boundary[i] = p.side != 0 ? 1 : 0;
+  s.lp.range[i] = (double)el.range;
+  s.lp.stepLeft[i] = (double)L.stepLeft;
+  s.lp.meanLoss[i] = L.meanLoss;
+  s.lp.nsub[i] = L.nsub;
+  s.lp.winner[i] = (int8_t)el.t.winner;
+  s.lp.flags[i] = done ? (uint8_t)((L.stopped ? kStopped : 0) | (L.left ? kLeft : 0)) : (uint8_t)(kInStep | kCont);
+  if (!done) {
+    res.edep[i] = 0.0;
+    res.nsec[i] = 0;
+    proc[i] = -1;
+  }
+  xend[i] = p.x;
+  side[i] = (int8_t)p.side;
+}
+
+// k_sh_el_finish: the step's end for the slots whose step ended this round:
+// the loss fluctuation on the summed mean loss, at-rest annihilation, and the
+// choice of the discrete interaction (LoopFinishSelect); proc = the process
+// whose final state k_sh_electron_final samples (0 ionisation, 1 brem,
+// 2 annihilation in flight) or -1.
+template <typename R>
+__global__ void __launch_bounds__(kB) k_sh_el_finish(Stack s, hepem_step_out res, int8_t* __restrict__ proc,
+                                                     hepem::DevTables<R> T, uint64_t seed, int ne) {
+  using namespace hepem;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const uint8_t f = s.lp.flags[i];
+  if (f & kInStep) return;
+  const int n = s.t.n;
+  hepem_tracks& t = s.t;
+  ElState<R> el;
+  Rng rng;
+  InitElectron(t, n, i, seed, el, rng);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) el.t.mfp[k] = (R)s.lp.mfp[k * n + i];
+  el.range = (R)s.lp.range[i];
+  el.t.winner = s.lp.winner[i];
+  StepLoop<R> L;
+  L.iDProc = s.lp.iDProc[i];
+  L.meanLoss = s.lp.meanLoss[i];
+  L.preEkin = s.lp.preEkin[i];
+  L.preLEkin = L.preEkin > 0. ? vlog((R)L.preEkin) : (R)-30;
+  L.stopped = (f & kStopped) != 0;
+  L.left = (f & kLeft) != 0;
+  L.cont = false;
+  SecList<R> sl;
+  sl.ns = 0;
+  proc[i] = (int8_t)LoopFinishSelect(T, el, L, sl, rng);
+  t.ekin[i] = el.t.ekin;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) t.num_ia_left[k * n + i] = (double)el.t.nial[k];
+  StoreRng(t, i, rng);
+  res.edep[i] = el.t.edep;
+  res.winner[i] = el.t.winner;
+  WriteSecondaries(res, n, i, sl);
+}
+
+// The final states of the discrete interactions k_sh_el_finish selected: the
 // e-/e+ slot's post-step energy and direction, RNG state and secondaries.
 template <typename R>
 __global__ void __launch_bounds__(kB) k_sh_electron_final(Stack s, hepem_step_out res, const int8_t* __restrict__ proc,
@@ -414,7 +501,7 @@ __global__ void __launch_bounds__(kB) k_sh_electron_final(Stack s, hepem_step_ou
 // takes it (G4HepEmTrackingManager.cc:893-1145): HowFar, the cut at the next
 // plane along x (or, with Woodcock tracking on and above its energy limit, a
 // Woodcock step through the stack), then the boundary update or
-// SelectInteraction + Perform. xend and side as k_sh_electron; layer = the slab
+// SelectInteraction + Perform. xend and side as k_sh_el_loop; layer = the slab
 // the step's deposit is scored in (changed only by a Woodcock step).
 template <typename R, bool kWoodcock>
 __global__ void __launch_bounds__(kB) k_sh_gamma(Stack s, Slabs g, hepem_step_out res, double* __restrict__ xend,
@@ -492,6 +579,9 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
   const int n = s.t.n;
   const bool valid = i < n;
   const hepem_tracks& t = s.t;
+  // the steps that ended this round (an e-/e+ within its sub-steps has not),
+  // summed with the block's stacked secondaries in bsec (bits 16 and up)
+  const uint32_t ended = (valid && (t.ptype[i] == 2 || !(s.lp.flags[i] & kInStep))) ? 1u : 0u;
   constexpr unsigned long long kNone = ~0ull;
   unsigned long long key = kNone, val = 0ull;
   uint32_t cE = 0u, cG = 0u, ns = 0u;
@@ -581,7 +671,7 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
     if (k != kNone) atomicAdd(&dep[k], hval[e]);
   }
   uint32_t xE, xG, tE, tG, tns;
-  BlockScan2<kB>(cE, cG, ns, xE, xG, tE, tG, tns);
+  BlockScan2<kB>(cE, cG, ns | (ended << 16), xE, xG, tE, tG, tns);
   if (valid) {
     offE[i] = xE;
     offG[i] = xG;
@@ -602,20 +692,24 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
 constexpr int kScanB = 1024;  // threads of the one-block scan
 __global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t* bsumG, const uint32_t* __restrict__ bsec, int nb,
                                                   int inject, int injectE, uint32_t* total, unsigned long long* nsecTotal,
-                                                  volatile uint32_t* hostCount, uint32_t tag) {
+                                                  unsigned long long* nstepsTotal, volatile uint32_t* hostCount, uint32_t tag) {
   __shared__ uint32_t carryE, carryG;
-  __shared__ unsigned long long secs;
+  __shared__ unsigned long long secs, steps;
   if (threadIdx.x == 0) {
     carryE = 0u;
     carryG = 0u;
     secs = 0ull;
+    steps = 0ull;
   }
   __syncthreads();
+  unsigned long long myEnded = 0ull;
   for (int base = 0; base < nb; base += kScanB) {
     const int i = base + threadIdx.x;
     const uint32_t a = i < nb ? bsumE[i] : 0u;
     const uint32_t b = i < nb ? bsumG[i] : 0u;
-    const uint32_t w = i < nb ? bsec[i] : 0u;
+    const uint32_t packed = i < nb ? bsec[i] : 0u;  // secondaries | steps ended << 16
+    const uint32_t w = packed & 0xFFFFu;
+    myEnded += packed >> 16;
     uint32_t xa, xb, ta, tb, tw;
     BlockScan2<kScanB>(a, b, w, xa, xb, ta, tb, tw);
     if (i < nb) {
@@ -630,7 +724,11 @@ __global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t*
     }
     __syncthreads();
   }
+  for (int o = 32; o > 0; o >>= 1) myEnded += __shfl_xor(myEnded, o, 64);
+  if (__lane_id() == 0) atomicAdd(&steps, myEnded);
+  __syncthreads();
   if (threadIdx.x == 0) {
+    *nstepsTotal += steps;
     const uint32_t k = (uint32_t)inject;
     const uint32_t elen = carryE + (injectE ? k : 0u);
     total[0] = carryE + carryG + k;
@@ -684,13 +782,29 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
     for (int c = 0; c < 3; ++c) d.dir[c * m + j] = s.dir[c * n + i];
     d.on_boundary[j] = s.on_boundary[i];
     d.num_ia_left[j] = s.num_ia_left[i];
+    bool inStep = false;
     if (isE) {
-      d.safety[j] = safety;
       for (int k = 1; k < 4; ++k) d.num_ia_left[k * m + j] = s.num_ia_left[k * n + i];
       d.msc_init_range[j] = s.msc_init_range[i];
       d.msc_dyn_range_factor[j] = s.msc_dyn_range_factor[i];
       d.msc_tlimit_min[j] = s.msc_tlimit_min[i];
       d.msc_first_step[j] = s.msc_first_step[i];
+      const uint8_t f = a.lp.flags[i];
+      inStep = (f & kInStep) != 0;
+      b.lp.flags[j] = inStep ? f : (uint8_t)0;
+      if (inStep) {
+        // within a step: its pre-step safety and the loop state go along
+        d.safety[j] = s.safety[i];
+        b.lp.range[j] = a.lp.range[i];
+        b.lp.stepLeft[j] = a.lp.stepLeft[i];
+        for (int k = 0; k < 4; ++k) b.lp.mfp[k * m + j] = a.lp.mfp[k * n + i];
+        b.lp.preEkin[j] = a.lp.preEkin[i];
+        b.lp.meanLoss[j] = a.lp.meanLoss[i];
+        b.lp.nsub[j] = a.lp.nsub[i];
+        b.lp.iDProc[j] = a.lp.iDProc[i];
+      } else {
+        d.safety[j] = safety;
+      }
     }
     d.rng_ctr[j] = s.rng_ctr[i];
     d.rng_gauss[j] = s.rng_gauss[i];
@@ -698,7 +812,7 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
     b.x[j] = x;
     b.layer[j] = l;
     b.event[j] = a.event[i];
-    b.nstep[j] = ns1;
+    b.nstep[j] = inStep ? a.nstep[i] : ns1;  // a step counts once it ends
   }
   const uint32_t mask = smask[i];
   for (int sl = 0; sl < 2; ++sl) {
@@ -717,6 +831,7 @@ __global__ void __launch_bounds__(kB) k_sh_scatter(Stack a, Stack b, Slabs g, he
     d.num_ia_left[j] = -1.0;
     if (isE) {
       d.safety[j] = safety;
+      b.lp.flags[j] = 0;
       for (int k = 1; k < 4; ++k) d.num_ia_left[k * m + j] = -1.0;
       d.msc_init_range[j] = 1.0e+21;
       d.msc_dyn_range_factor[j] = 0.04;
@@ -794,7 +909,10 @@ bool AllocStack(Arena& A, Stack& s, int cap) {
          A.get(t.safety, cap) && A.get(t.on_boundary, cap) && A.get(t.num_ia_left, 4 * (size_t)cap) &&
          A.get(t.msc_init_range, cap) && A.get(t.msc_dyn_range_factor, cap) && A.get(t.msc_tlimit_min, cap) &&
          A.get(t.msc_first_step, cap) && A.get(t.rng_ctr, cap) && A.get(t.rng_gauss, cap) && A.get(t.rng_has_gauss, cap) &&
-         A.get(t.rng_stream, cap) && A.get(s.x, cap) && A.get(s.layer, cap) && A.get(s.event, cap) && A.get(s.nstep, cap);
+         A.get(t.rng_stream, cap) && A.get(s.x, cap) && A.get(s.layer, cap) && A.get(s.event, cap) && A.get(s.nstep, cap) &&
+         A.get(s.lp.range, cap) && A.get(s.lp.stepLeft, cap) && A.get(s.lp.mfp, 4 * (size_t)cap) && A.get(s.lp.preEkin, cap) &&
+         A.get(s.lp.meanLoss, cap) && A.get(s.lp.nsub, cap) && A.get(s.lp.iDProc, cap) && A.get(s.lp.winner, cap) &&
+         A.get(s.lp.flags, cap);
 }
 
 // Kahan-compensated accumulator
@@ -846,15 +964,15 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   double *dPlanes, *xend;
   int8_t *side, *proc;
   uint8_t *keep, *smask;
-  uint32_t *offs, *bsum, *bsec, *total, *equeue;  // offs: e-/e+ then gamma block offsets (2 x cap); bsum: 2 x nblk
+  uint32_t *offs, *bsum, *bsec, *total;  // offs: e-/e+ then gamma block offsets (2 x cap); bsum: 2 x nblk
   unsigned long long *dep, *leak, *nsec;
   const int nblk = (cap + kB - 1) / kB;
   if (!AllocStack(A, s[0], cap) || !AllocStack(A, s[1], cap) || !A.get(res.true_step, cap) || !A.get(res.edep, cap) ||
       !A.get(res.winner, cap) || !A.get(res.nsec, cap) || !A.get(res.sec_ptype, 2 * (size_t)cap) ||
       !A.get(res.sec_ekin, 2 * (size_t)cap) || !A.get(res.sec_dir, 6 * (size_t)cap) || !A.get(dPlanes, planes.size()) ||
       !A.get(xend, cap) || !A.get(side, cap) || !A.get(proc, cap) || !A.get(keep, cap) || !A.get(smask, cap) || !A.get(offs, 2 * (size_t)cap) ||
-      !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) || !A.get(equeue, 1) || !A.get(dep, (size_t)span * nbins) ||
-      !A.get(leak, (size_t)span) || !A.get(nsec, 1))
+      !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) || !A.get(dep, (size_t)span * nbins) ||
+      !A.get(leak, (size_t)span) || !A.get(nsec, 2))
     return HEPEM_ERR_ALLOC;
   HostCount hc;  // next round's track count, published by k_scan_sums
   if (!hc.alloc()) return HEPEM_ERR_ALLOC;
@@ -904,28 +1022,17 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   }
   const bool woodcock = g.wdtMin > 0.0;
   const bool fp64 = HepemPrecision(ctx) == HEPEM_FP64;
-  // one resident round of blocks of the e-/e+ stepping kernel
-  int residentE = 0;
-  {
-    int dev = 0, cus = 0, perCu = 0;
-    SH_OK(hipGetDevice(&dev));
-    SH_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (fp64)
-      SH_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_sh_electron<double>, kB, 0));
-    else
-      SH_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_sh_electron<float>, kB, 0));
-    residentE = cus * perCu > 0 ? cus * perCu : 1024;
-  }
 
   std::vector<Kahan> ks(nbins), ks2(nbins);
   Kahan kleak, ktot;
   std::vector<unsigned long long> hdep((size_t)span * nbins), hleak((size_t)span);
   r->steps = 0;
   r->tracks = 0;
+  r->substeps = 0;
   r->iterations = 0;
   r->max_in_flight = 0;
-  unsigned long long hnsec = 0;
-  hipLaunchKernelGGL(k_sh_fill64, dim3(1), dim3(kB), 0, st, reinterpret_cast<int64_t*>(nsec), 1);
+  unsigned long long hnsec[2] = {0ull, 0ull};  // secondaries stacked, steps ended
+  hipLaunchKernelGGL(k_sh_fill64, dim3(1), dim3(kB), 0, st, reinterpret_cast<int64_t*>(nsec), 2);
 
   for (int64_t s0 = 0; s0 < nevents; s0 += span) {
     const int ns = (int)((nevents - s0) < span ? (nevents - s0) : span);
@@ -965,17 +1072,20 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       // the e-/e+ part of the stack comes first
       const int ge = (ne + kB - 1) / kB, gg = (n - ne + kB - 1) / kB;
       if (ge > 0) {
-        const int gq = ge < residentE ? ge : residentE;
-        hipLaunchKernelGGL(k_sh_fill32, dim3(1), dim3(64), 0, st, equeue, 1, 0u);
         if (fp64) {
-          hipLaunchKernelGGL(k_sh_electron<double>, dim3(gq), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables64(ctx),
-                             seed, ne, equeue);
+          hipLaunchKernelGGL(k_sh_el_begin<double>, dim3(ge), dim3(kB), 0, st, a, g, HepemTables64(ctx), seed, ne);
+          hipLaunchKernelGGL(k_sh_el_substep<double>, dim3(ge), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables64(ctx),
+                             seed, ne);
+          hipLaunchKernelGGL(k_sh_el_finish<double>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables64(ctx), seed, ne);
           hipLaunchKernelGGL(k_sh_electron_final<double>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables64(ctx), seed, ne);
         } else {
-          hipLaunchKernelGGL(k_sh_electron<float>, dim3(gq), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables32(ctx),
-                             seed, ne, equeue);
+          hipLaunchKernelGGL(k_sh_el_begin<float>, dim3(ge), dim3(kB), 0, st, a, g, HepemTables32(ctx), seed, ne);
+          hipLaunchKernelGGL(k_sh_el_substep<float>, dim3(ge), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables32(ctx),
+                             seed, ne);
+          hipLaunchKernelGGL(k_sh_el_finish<float>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables32(ctx), seed, ne);
           hipLaunchKernelGGL(k_sh_electron_final<float>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables32(ctx), seed, ne);
         }
+        r->substeps += ne;
       }
       if (gg > 0) {
         if (fp64 && woodcock)
@@ -997,7 +1107,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       const int k = refillNow ? ((ns - next) < chunk ? (ns - next) : chunk) : 0;
       ++tag;
       hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanB), 0, st, bsum, bsum + nblk, bsec, grid, k, su->primary != 2 ? 1 : 0,
-                         total, nsec, hc.d, tag);
+                         total, nsec, nsec + 1, hc.d, tag);
       hipLaunchKernelGGL(k_sh_scatter, dim3(grid), dim3(kB), 0, st, a, b, g, res, keep, smask, offs, offs + cap, bsum, bsum + nblk,
                          total, cap);
       if (k > 0) {
@@ -1012,7 +1122,6 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       // keeps it behind this round's scatter
       if (!hc.wait(tag, st)) return HEPEM_ERR_HIP;
       const uint32_t m = hc.count();
-      r->steps += n;
       r->iterations += 1;
       if (m > (uint32_t)cap) return HEPEM_ERR_CAPACITY;
       if (growRounds < 0 && (int)m < n) growRounds = sinceInject + 1;  // the stack shrank for the first time
@@ -1038,9 +1147,10 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       kleak.add((double)hleak[e] / kFix);
     }
   }
-  SH_OK(hipMemcpyAsync(&hnsec, nsec, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  SH_OK(hipMemcpyAsync(hnsec, nsec, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   SH_OK(hipStreamSynchronize(st));
-  r->tracks += (int64_t)hnsec;
+  r->tracks += (int64_t)hnsec[0];
+  r->steps = (int64_t)hnsec[1];
   for (int k = 0; k < nbins; ++k) {
     r->edep_sum[k] = ks[k].s;
     r->edep_sum2[k] = ks2[k].s;

@@ -84,6 +84,8 @@ typedef struct hepem_slab_result {
   int64_t  tracks;                      /* primaries + secondaries                         */
   int32_t  iterations;                  /* stepping rounds over the stack                  */
   int32_t  max_in_flight;               /* largest stack occupancy                         */
+  int64_t  substeps;                    /* e-/e+ sub-steps (MSC-limited sub-steps of the
+                                           e-/e+ steps, each counted)                      */
 } hepem_slab_result;
 
 /* Simulates `nevents` events (event e has primary stream id derived from
