@@ -2,14 +2,19 @@
 //
 // The track stack is a pair of structure-of-arrays buffers of `capacity`
 // slots. One stepping round over the n tracks in flight:
-//   k_sh_step      one step of every track: an e-/e+ takes the whole
-//                  G4HepEmTrackingManager::TrackElectron step (MSC sub-steps
-//                  and their displacements, device/dev_tracking.hh) with the
-//                  slab as its geometry; a gamma HowFar, the cut at the next
-//                  plane along x (or, with Woodcock tracking on and above its
-//                  energy limit, a step through the stack to its next
-//                  interaction point or out of it), then the boundary update or
-//                  SelectInteraction + Perform
+//   k_sh_el_substep  the e-/e+ (the first part of the stack): one sub-step of
+//                  G4HepEmTrackingManager::TrackElectron's step each, after
+//                  the step's pre-step part where a track is not within a step
+//                  (device/dev_tracking.hh), with the slab as the geometry and
+//                  the MSC displacement applied; a track within its step
+//                  carries the loop state to the next round
+//   k_sh_el_end    the e-/e+ whose step ended: the loss fluctuation, at-rest
+//                  annihilation or the discrete interaction
+//   k_sh_gamma     the gammas: HowFar, the cut at the next plane along x (or,
+//                  with Woodcock tracking on and above its energy limit, a
+//                  step through the stack to its next interaction point or out
+//                  of it), then the boundary update or SelectInteraction +
+//                  Perform
 //   k_sh_post      slab crossing, leakage, scoring, the number of stack slots
 //                  each track needs next round and their exclusive scan within
 //                  the block
@@ -88,9 +93,8 @@ __device__ inline int AbsorberCouple(const Slabs& g, int a) {
 // one stack buffer
 // What an e-/e+ carries between the sub-steps of one TrackElectron step
 // (G4HepEmTrackingManager.cc:427-606), which run one per stepping round: the
-// step's pre-step part (k_sh_el_begin) sets it, each sub-step (k_sh_el_substep)
-// updates it, the step's end (k_sh_el_finish) reads it. Gamma slots do not
-// carry it.
+// step's pre-step part sets it and each sub-step updates it (k_sh_el_substep),
+// the step's end reads it (k_sh_el_end). Gamma slots do not carry it.
 constexpr uint8_t kInStep = 1;   // between sub-steps of a step (else the next round begins a step)
 constexpr uint8_t kCont = 2;     // continueStepping: MSC may limit the next sub-step too
 constexpr uint8_t kStopped = 4;  // the step's last sub-step stopped the track
@@ -327,54 +331,24 @@ __device__ bool WoodcockStep(const hepem::DevTables<R>& T, const Slabs& g, hepem
 
 // The e-/e+ slots [0, ne) of G4HepEmTrackingManager::TrackElectron's step with
 // the slab as the caller's geometry (device/dev_tracking.hh), one sub-step
-// per stepping round: a slot that is not within a step begins one
-// (k_sh_el_begin), every slot takes one sub-step (k_sh_el_substep), and a slot
-// whose step thereby ended finishes it (k_sh_el_finish, k_sh_electron_final).
-// A slot within a step stays in the stack with its loop state, deposits
-// nothing and creates nothing until its step ends. Every round thus does one
-// uniform unit of work per e-/e+, whatever the length of its sub-step chain.
+// per stepping round: every slot takes one sub-step, after beginning a step
+// if it is not within one (k_sh_el_substep), and a slot whose step thereby
+// ended finishes it (k_sh_el_end). A slot within a step stays in the stack
+// with its loop state, deposits nothing and creates nothing until its step
+// ends. Every round thus does one uniform unit of work per e-/e+, whatever the
+// length of its sub-step chain.
 //
-// k_sh_el_begin: the pre-step safety (kept in t.safety for the step), the
-// interaction lengths and HowFarToDiscreteInteraction.
-template <typename R>
-__global__ void __launch_bounds__(kB) k_sh_el_begin(Stack s, Slabs g, hepem::DevTables<R> T, uint64_t seed, int ne) {
-  using namespace hepem;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ne || (s.lp.flags[i] & kInStep)) return;
-  const int n = s.t.n;
-  hepem_tracks& t = s.t;
-  const int l = s.layer[i];
-  ElState<R> el;
-  Rng rng;
-  InitElectron(t, n, i, seed, el, rng);
-  StepLoop<R> L;
-  SlabPos p{s.x[i], g.plane[l], g.plane[l + 1], l == 0, l == g.nbins - 1, 0};
-  SlabBegin(T, el, L, rng, p);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    t.num_ia_left[k * n + i] = (double)el.t.nial[k];
-    s.lp.mfp[k * n + i] = (double)el.t.mfp[k];
-  }
-  StoreRng(t, i, rng);
-  t.safety[i] = (double)el.t.safety;
-  s.lp.range[i] = (double)el.range;
-  s.lp.stepLeft[i] = (double)L.stepLeft;
-  s.lp.preEkin[i] = L.preEkin;
-  s.lp.meanLoss[i] = 0.0;
-  s.lp.nsub[i] = 0;
-  s.lp.iDProc[i] = (int8_t)L.iDProc;
-  s.lp.flags[i] = (uint8_t)(kInStep | (L.cont ? kCont : 0));
-}
-
-// k_sh_el_substep: one sub-step of every slot (SlabSubstep: HowFarToMSC, the
-// move, the along-step part, the MSC displacement). xend = post-sub-step
+// k_sh_el_substep: for a slot not within a step, the pre-step safety (kept in
+// t.safety for the step), the interaction lengths and
+// HowFarToDiscreteInteraction; then one sub-step (SlabSubstep: HowFarToMSC,
+// the move, the along-step part, the MSC displacement). xend = post-sub-step
 // position; side = +1/-1 when it ended on the upper or lower plane of its
-// slab, 0 inside. A slot still within its step gets proc = -1 and no deposit
-// or secondaries this round.
+// slab, 0 inside. A slot still within its step gets no deposit or
+// secondaries this round.
 template <typename R>
 __global__ void __launch_bounds__(kB) k_sh_el_substep(Stack s, Slabs g, hepem_step_out res, double* __restrict__ xend,
-                                                      int8_t* __restrict__ side, int8_t* __restrict__ proc,
-                                                      hepem::DevTables<R> T, uint64_t seed, int ne) {
+                                                      int8_t* __restrict__ side, hepem::DevTables<R> T, uint64_t seed,
+                                                      int ne) {
   using namespace hepem;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ne) return;
@@ -384,26 +358,36 @@ __global__ void __launch_bounds__(kB) k_sh_el_substep(Stack s, Slabs g, hepem_st
   ElState<R> el;
   Rng rng;
   InitElectron(t, n, i, seed, el, rng);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) el.t.mfp[k] = (R)s.lp.mfp[k * n + i];
-  el.range = (R)s.lp.range[i];
-  el.pstep = (R)s.lp.stepLeft[i];
-  el.t.gstep = el.pstep;  // HowFarToMSC keeps it where MSC is off
-  el.preEkin = el.t.ekin;
-  el.preLEkin = el.t.logE();
   StepLoop<R> L;
-  L.iDProc = s.lp.iDProc[i];
-  el.t.winner = L.iDProc;
-  L.stepLeft = el.pstep;
-  L.truePath = (R)0.;
-  L.meanLoss = s.lp.meanLoss[i];
-  L.preEkin = s.lp.preEkin[i];
-  L.preLEkin = (R)0.;
-  L.cont = (s.lp.flags[i] & kCont) != 0;
-  L.stopped = false;
-  L.left = false;
-  L.nsub = s.lp.nsub[i];
   SlabPos p{s.x[i], g.plane[l], g.plane[l + 1], l == 0, l == g.nbins - 1, 0};
+  const uint8_t f0 = s.lp.flags[i];
+  if (!(f0 & kInStep)) {
+    SlabBegin(T, el, L, rng, p);
+    t.safety[i] = (double)el.t.safety;
+    s.lp.preEkin[i] = L.preEkin;
+    s.lp.iDProc[i] = (int8_t)L.iDProc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s.lp.mfp[k * n + i] = (double)el.t.mfp[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) el.t.mfp[k] = (R)s.lp.mfp[k * n + i];
+    el.range = (R)s.lp.range[i];
+    el.pstep = (R)s.lp.stepLeft[i];
+    el.t.gstep = el.pstep;  // HowFarToMSC keeps it where MSC is off
+    el.preEkin = el.t.ekin;
+    el.preLEkin = el.t.logE();
+    L.iDProc = s.lp.iDProc[i];
+    el.t.winner = L.iDProc;
+    L.stepLeft = el.pstep;
+    L.truePath = (R)0.;
+    L.meanLoss = s.lp.meanLoss[i];
+    L.preEkin = s.lp.preEkin[i];
+    L.preLEkin = (R)0.;
+    L.cont = (f0 & kCont) != 0;
+    L.stopped = false;
+    L.left = false;
+    L.nsub = s.lp.nsub[i];
+  }
   const bool done = SlabSubstep(T, el, L, rng, p);
   StoreElectronState(t, n, i, el, rng);
   t.ekin[i] = el.t.ekin;
@@ -419,20 +403,16 @@ __global__ void __launch_bounds__(kB) k_sh_el_substep(Stack s, Slabs g, hepem_st
   if (!done) {
     res.edep[i] = 0.0;
     res.nsec[i] = 0;
-    proc[i] = -1;
   }
   xend[i] = p.x;
   side[i] = (int8_t)p.side;
 }
 
-// k_sh_el_finish: the step's end for the slots whose step ended this round:
-// the loss fluctuation on the summed mean loss, at-rest annihilation, and the
-// choice of the discrete interaction (LoopFinishSelect); proc = the process
-// whose final state k_sh_electron_final samples (0 ionisation, 1 brem,
-// 2 annihilation in flight) or -1.
+// k_sh_el_end: the step's end for the slots whose step ended this round: the
+// loss fluctuation on the summed mean loss, at-rest annihilation, or the choice
+// of the discrete interaction and its final state (LoopFinish).
 template <typename R>
-__global__ void __launch_bounds__(kB) k_sh_el_finish(Stack s, hepem_step_out res, int8_t* __restrict__ proc,
-                                                     hepem::DevTables<R> T, uint64_t seed, int ne) {
+__global__ void __launch_bounds__(kB) k_sh_el_end(Stack s, hepem_step_out res, hepem::DevTables<R> T, uint64_t seed, int ne) {
   using namespace hepem;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ne) return;
@@ -457,42 +437,15 @@ __global__ void __launch_bounds__(kB) k_sh_el_finish(Stack s, hepem_step_out res
   L.cont = false;
   SecList<R> sl;
   sl.ns = 0;
-  proc[i] = (int8_t)LoopFinishSelect(T, el, L, sl, rng);
-  t.ekin[i] = el.t.ekin;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) t.num_ia_left[k * n + i] = (double)el.t.nial[k];
-  StoreRng(t, i, rng);
+  const int p = LoopFinishSelect(T, el, L, sl, rng);
   res.edep[i] = el.t.edep;
   res.winner[i] = el.t.winner;
-  WriteSecondaries(res, n, i, sl);
-}
-
-// The final states of the discrete interactions k_sh_el_finish selected: the
-// e-/e+ slot's post-step energy and direction, RNG state and secondaries.
-template <typename R>
-__global__ void __launch_bounds__(kB) k_sh_electron_final(Stack s, hepem_step_out res, const int8_t* __restrict__ proc,
-                                                          hepem::DevTables<R> T, uint64_t seed, int ne) {
-  using namespace hepem;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ne) return;
-  const int p = proc[i];
-  if (p < 0) return;
-  const int n = s.t.n;
-  hepem_tracks& t = s.t;
-  Trk<R> e;
-  e.isEl = t.ptype[i] == 0;
-  e.setE(t.ekin[i]);
+  ElectronFinalState(T, el.t, p, sl, rng);
+  t.ekin[i] = el.t.ekin;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) e.dir[c] = (R)t.dir[c * n + i];
-  e.imc = t.imc[i];
-  Rng rng;
-  rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
-  SecList<R> sl;
-  sl.ns = 0;
-  ElectronFinalState(T, e, p, sl, rng);
-  t.ekin[i] = e.ekin;
+  for (int c = 0; c < 3; ++c) t.dir[c * n + i] = (double)el.t.dir[c];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) t.dir[c * n + i] = (double)e.dir[c];
+  for (int k = 0; k < 4; ++k) t.num_ia_left[k * n + i] = (double)el.t.nial[k];
   StoreRng(t, i, rng);
   WriteSecondaries(res, n, i, sl);
 }
@@ -501,7 +454,7 @@ __global__ void __launch_bounds__(kB) k_sh_electron_final(Stack s, hepem_step_ou
 // takes it (G4HepEmTrackingManager.cc:893-1145): HowFar, the cut at the next
 // plane along x (or, with Woodcock tracking on and above its energy limit, a
 // Woodcock step through the stack), then the boundary update or
-// SelectInteraction + Perform. xend and side as k_sh_el_loop; layer = the slab
+// SelectInteraction + Perform. xend and side as k_sh_el_substep; layer = the slab
 // the step's deposit is scored in (changed only by a Woodcock step).
 template <typename R, bool kWoodcock>
 __global__ void __launch_bounds__(kB) k_sh_gamma(Stack s, Slabs g, hepem_step_out res, double* __restrict__ xend,
@@ -962,7 +915,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   res.displacement = nullptr;
   res.stopped = nullptr;
   double *dPlanes, *xend;
-  int8_t *side, *proc;
+  int8_t* side;
   uint8_t *keep, *smask;
   uint32_t *offs, *bsum, *bsec, *total;  // offs: e-/e+ then gamma block offsets (2 x cap); bsum: 2 x nblk
   unsigned long long *dep, *leak, *nsec;
@@ -970,7 +923,7 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
   if (!AllocStack(A, s[0], cap) || !AllocStack(A, s[1], cap) || !A.get(res.true_step, cap) || !A.get(res.edep, cap) ||
       !A.get(res.winner, cap) || !A.get(res.nsec, cap) || !A.get(res.sec_ptype, 2 * (size_t)cap) ||
       !A.get(res.sec_ekin, 2 * (size_t)cap) || !A.get(res.sec_dir, 6 * (size_t)cap) || !A.get(dPlanes, planes.size()) ||
-      !A.get(xend, cap) || !A.get(side, cap) || !A.get(proc, cap) || !A.get(keep, cap) || !A.get(smask, cap) || !A.get(offs, 2 * (size_t)cap) ||
+      !A.get(xend, cap) || !A.get(side, cap) || !A.get(keep, cap) || !A.get(smask, cap) || !A.get(offs, 2 * (size_t)cap) ||
       !A.get(bsum, 2 * (size_t)nblk) || !A.get(bsec, nblk) || !A.get(total, 3) || !A.get(dep, (size_t)span * nbins) ||
       !A.get(leak, (size_t)span) || !A.get(nsec, 2))
     return HEPEM_ERR_ALLOC;
@@ -1073,17 +1026,13 @@ extern "C" int hepem_slab_run(const hepem_ctx* ctx, const hepem_slab_setup* su, 
       const int ge = (ne + kB - 1) / kB, gg = (n - ne + kB - 1) / kB;
       if (ge > 0) {
         if (fp64) {
-          hipLaunchKernelGGL(k_sh_el_begin<double>, dim3(ge), dim3(kB), 0, st, a, g, HepemTables64(ctx), seed, ne);
-          hipLaunchKernelGGL(k_sh_el_substep<double>, dim3(ge), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables64(ctx),
+          hipLaunchKernelGGL(k_sh_el_substep<double>, dim3(ge), dim3(kB), 0, st, a, g, res, xend, side, HepemTables64(ctx),
                              seed, ne);
-          hipLaunchKernelGGL(k_sh_el_finish<double>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables64(ctx), seed, ne);
-          hipLaunchKernelGGL(k_sh_electron_final<double>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables64(ctx), seed, ne);
+          hipLaunchKernelGGL(k_sh_el_end<double>, dim3(ge), dim3(kB), 0, st, a, res, HepemTables64(ctx), seed, ne);
         } else {
-          hipLaunchKernelGGL(k_sh_el_begin<float>, dim3(ge), dim3(kB), 0, st, a, g, HepemTables32(ctx), seed, ne);
-          hipLaunchKernelGGL(k_sh_el_substep<float>, dim3(ge), dim3(kB), 0, st, a, g, res, xend, side, proc, HepemTables32(ctx),
+          hipLaunchKernelGGL(k_sh_el_substep<float>, dim3(ge), dim3(kB), 0, st, a, g, res, xend, side, HepemTables32(ctx),
                              seed, ne);
-          hipLaunchKernelGGL(k_sh_el_finish<float>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables32(ctx), seed, ne);
-          hipLaunchKernelGGL(k_sh_electron_final<float>, dim3(ge), dim3(kB), 0, st, a, res, proc, HepemTables32(ctx), seed, ne);
+          hipLaunchKernelGGL(k_sh_el_end<float>, dim3(ge), dim3(kB), 0, st, a, res, HepemTables32(ctx), seed, ne);
         }
         r->substeps += ne;
       }
