@@ -241,9 +241,11 @@ struct Rng {
 // xsData[start] instead of following the header chain.
 template <typename R>
 struct XsBlock {
-  double minE, maxE;    // first and last grid energy
+  double minE, lastE;   // first and last grid energy
+  R logMin, ild;        // log of the first energy, 1/log-spacing
   int start, n;         // first triple, number of triples
-  R maxV, logMin, ild;  // largest cross section, log of the first energy, 1/log-spacing
+  double maxE;          // energy of the largest cross section
+  R maxV;               // the largest cross section
 };
 
 template <typename R>
@@ -254,6 +256,7 @@ struct DevEl {
   const double* lossData64;
   int N;
   R lossLogMin, lossILD;
+  double lossEmin, lossEmax;  // lossGrid64[0], lossGrid64[N-1]
   int rangeMonotone;    // every couple's range column is non-decreasing (enables the bin hint)
   const int* xsStart;   // [couple] start of the ioni block in xsData
   const XsBlock<R>* xsBlock;  // [2 per couple]: ioni, brem
@@ -265,6 +268,7 @@ struct DevEl {
   const R* nucData;     // [2 nucN per material]
   int nucN;
   R nucLogMin, nucILD;
+  double nucEmin, nucEmax;    // nucGrid64[0], nucGrid64[nucN-1]
   const R* tr1;         // [2N per material]
   const int* selSBStart;
   const R* selSB;

@@ -50,16 +50,20 @@ template <typename R> HD R Linear(double x1, double x2, R y1, R y2, double x) { 
 template <typename R> HD int LogBin(R logx, R logxmin, R invLD, int ndata) {
   return (int)tmax((R)0., tmin((logx - logxmin) * invLD, (R)(ndata - 2)));
 }
-// x grid separate, [y, y''] interleaved (GetSplineLog :84-91)
-template <typename R> HD R SplineLog2(int nd, const double* xd, const R* yd, double x, R logx, R lmin, R ild) {
-  const double xv = tmax(xd[0], tmin(xd[nd - 1], x));
+// x grid separate, [y, y''] interleaved (GetSplineLog :84-91); xlo, xhi are
+// the grid's first and last values (xd[0], xd[nd-1]), held by the caller
+template <typename R>
+HD R SplineLog2(int nd, const double* xd, const R* yd, double x, R logx, R lmin, R ild, double xlo, double xhi) {
+  const double xv = tmax(xlo, tmin(xhi, x));
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i2 = 2 * idx;
   return Spline(xd[idx], xd[idx + 1], yd[i2], yd[i2 + 2], yd[i2 + 1], yd[i2 + 3], xv);
 }
-// [x, y, y''] interleaved (GetSplineLog :95-102); x read from the double copy dx
-template <typename R> HD R SplineLog3(int nd, const double* dx, const R* d, double x, R logx, R lmin, R ild) {
-  const double xv = tmax(dx[0], tmin(dx[3 * (nd - 1)], x));
+// [x, y, y''] interleaved (GetSplineLog :95-102); x read from the double copy
+// dx; xlo, xhi are the first and last abscissas (dx[0], dx[3(nd-1)])
+template <typename R>
+HD R SplineLog3(int nd, const double* dx, const R* d, double x, R logx, R lmin, R ild, double xlo, double xhi) {
+  const double xv = tmax(xlo, tmin(xhi, x));
   const int idx = LogBin(logx, lmin, ild, nd);
   const int i3 = 3 * idx;
   return Spline(dx[i3], dx[i3 + 3], d[i3 + 1], d[i3 + 4], d[i3 + 2], d[i3 + 5], xv);
@@ -157,11 +161,16 @@ template <typename R> HD R PickMFP(const R* mfp, int ip) {
 }
 
 // ---- G4HepEmElectronManager.icc: table accessors ------------------------------------------
+// the [y, y''] column `col` (0 range, 1 dE/dx) of couple imc's loss data at ekin
+template <typename R> HD R LossSpline(const DevEl<R>& e, int imc, int col, double ekin, R lekin) {
+  return SplineLog2(e.N, e.lossGrid64, e.lossData + e.N * (5 * imc + 2 * col), ekin, lekin, e.lossLogMin, e.lossILD,
+                    e.lossEmin, e.lossEmax);
+}
 template <typename R> HD R GetRestRange(const DevEl<R>& e, int imc, double ekin, R lekin) {  // :486-493
-  return tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.lossData + 5 * e.N * imc, ekin, lekin, e.lossLogMin, e.lossILD));
+  return tmax((R)0.0, LossSpline(e, imc, 0, ekin, lekin));
 }
 template <typename R> HD R GetRestDEDX(const DevEl<R>& e, int imc, double ekin, R lekin) {  // :495-502
-  return tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.lossData + e.N * (5 * imc + 2), ekin, lekin, e.lossLogMin, e.lossILD));
+  return tmax((R)0.0, LossSpline(e, imc, 1, ekin, lekin));
 }
 // :504-520; the low-range quadratic branch is an FP64 island in both forms
 // binHint: the loss-grid bin of an energy whose range is near `range` (at or
@@ -178,14 +187,33 @@ template <typename R> HD double GetInvRange(const DevEl<R>& e, int imc, double r
   const R* sd = e.lossData + 5 * N * imc + 4 * N;
   return (double)tmax((R)0.0, Spline(xd[2 * i], xd[2 * (i + 1)], e.lossGrid[i], e.lossGrid[i + 1], sd[i], sd[i + 1], range));
 }
+// GetInvRange of the range of an energy in loss-grid bin `bin`, whose grid
+// energies eLo, eHi the caller holds: on a non-decreasing range column the
+// search ends in that bin whenever range lies in [R_bin, R_bin+1) (or beyond
+// R_bin in the last bin), and the spline then takes eLo, eHi as its ordinates;
+// otherwise the general search.
+template <typename R> HD double GetInvRangeInBin(const DevEl<R>& e, int imc, double range, int bin, double eLo, double eHi) {
+  const int N = e.N;
+  const double* xd = e.lossData64 + 5 * N * imc;
+  if (e.rangeMonotone && range >= xd[0]) {
+    const double r1 = xd[2 * bin], r2 = xd[2 * (bin + 1)];
+    if (r1 <= range && (range < r2 || bin == N - 2)) {
+      const R* sd = e.lossData + 5 * N * imc + 4 * N;
+      return (double)tmax((R)0.0, Spline(r1, r2, (R)eLo, (R)eHi, sd[bin], sd[bin + 1], range));
+    }
+  }
+  return GetInvRange(e, imc, range, bin);
+}
 template <typename R> HD R GetRestMacXSec(const DevEl<R>& e, int imc, double ekin, R lekin, bool isioni) {  // :522-532
   const XsBlock<R> b = e.xsBlock[2 * imc + (isioni ? 0 : 1)];
   if (ekin < b.minE) return (R)0.0;
-  return tmax((R)0.0, SplineLog3(b.n, e.xsData64 + b.start, e.xsData + b.start, ekin, lekin, b.logMin, b.ild));
+  return tmax((R)0.0,
+              SplineLog3(b.n, e.xsData64 + b.start, e.xsData + b.start, ekin, lekin, b.logMin, b.ild, b.minE, b.lastE));
 }
 template <typename R> HD R GetMacXSecNuclear(const DevEl<R>& e, int imat, double ekin, R lekin) {  // :534-541
-  if (ekin < e.nucGrid64[0]) return (R)0.0;
-  return tmax((R)0.0, SplineLog2(e.nucN, e.nucGrid64, e.nucData + imat * 2 * e.nucN, ekin, lekin, e.nucLogMin, e.nucILD));
+  if (ekin < e.nucEmin) return (R)0.0;
+  return tmax((R)0.0, SplineLog2(e.nucN, e.nucGrid64, e.nucData + imat * 2 * e.nucN, ekin, lekin, e.nucLogMin, e.nucILD,
+                                 e.nucEmin, e.nucEmax));
 }
 template <typename R> HD R GetRestMacXSecForStepping(const DevEl<R>& e, int imc, double ekin, R lekin, bool isioni) {  // :544-568
   const R log08 = (R)-0.22314355131420971;
@@ -197,10 +225,12 @@ template <typename R> HD R GetRestMacXSecForStepping(const DevEl<R>& e, int imc,
     lekin += log08;
   }
   if (ekin < b.minE) return (R)0.0;
-  return tmax((R)0.0, SplineLog3(b.n, e.xsData64 + b.start, e.xsData + b.start, ekin, lekin, b.logMin, b.ild));
+  return tmax((R)0.0,
+              SplineLog3(b.n, e.xsData64 + b.start, e.xsData + b.start, ekin, lekin, b.logMin, b.ild, b.minE, b.lastE));
 }
 template <typename R> HD R GetTransportMFP(const DevEl<R>& e, int im, double ekin, R lekin) {  // :576-583
-  const R tr = tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.tr1 + 2 * e.N * im, ekin, lekin, e.lossLogMin, e.lossILD));
+  const R tr = tmax((R)0.0, SplineLog2(e.N, e.lossGrid64, e.tr1 + 2 * e.N * im, ekin, lekin, e.lossLogMin, e.lossILD,
+                                       e.lossEmin, e.lossEmax));
   return tr > (R)0. ? (R)1. / tr : (R)kALargeValue;
 }
 template <typename R> HD R MacXsecAnnihilation(R ekin, R eDens) {  // :585-593

@@ -89,6 +89,8 @@ template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronD
   E.lossLogMin = (R)e->fELossLogMinEkin;
   E.lossILD = (R)e->fELossEILDelta;
   P.addX(E.lossGrid, E.lossGrid64, e->fELossEnergyGrid, N);
+  E.lossEmin = N > 0 ? e->fELossEnergyGrid[0] : 0.0;
+  E.lossEmax = N > 0 ? e->fELossEnergyGrid[N - 1] : 0.0;
   P.addX(E.lossData, E.lossData64, e->fELossData, (size_t)5 * N * e->fNumMatCuts);
   // the inverse-range search may start from a bin hint only where the search
   // result cannot depend on the path: non-decreasing range columns
@@ -105,8 +107,9 @@ template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronD
   P.addX(E.xsData, E.xsData64, e->fResMacXSecData, e->fResMacXSecNumData);
   E.xsN = e->fResMacXSecNumData;
   {
-    // block header: [0] triples, [1] last energy, [2] largest value, [3] log of
-    // the first energy, [4] 1/log-spacing, then the triples from [5]
+    // block header: [0] triples, [1] energy of the largest value, [2] largest
+    // value, [3] log of the first energy, [4] 1/log-spacing, then the
+    // [energy, value, second derivative] triples from [5]
     std::vector<XsBlock<R>> blocks((size_t)2 * e->fNumMatCuts);
     const double* d = e->fResMacXSecData;
     for (int mc = 0; mc < e->fNumMatCuts; ++mc) {
@@ -115,10 +118,11 @@ template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronD
       for (int k = 0; k < 2; ++k) {
         const int s0 = k == 0 ? iIoni : iIoni + 3 * nIoni + 5;
         XsBlock<R>& b = blocks[2 * (size_t)mc + k];
+        b.n = (int)d[s0];
         b.minE = d[s0 + 5];
+        b.lastE = b.n > 0 ? d[s0 + 5 + 3 * (b.n - 1)] : b.minE;
         b.maxE = d[s0 + 1];
         b.start = s0 + 5;
-        b.n = (int)d[s0];
         b.maxV = (R)d[s0 + 2];
         b.logMin = (R)d[s0 + 3];
         b.ild = (R)d[s0 + 4];
@@ -130,6 +134,8 @@ template <typename R> void PackEl(Packer& P, DevEl<R>& E, const G4HepEmElectronD
   E.nucLogMin = (R)e->fENucLogMinEkin;
   E.nucILD = (R)e->fENucEILDelta;
   P.addX(E.nucGrid, E.nucGrid64, e->fENucEnergyGrid, e->fENucEnergyGridSize);
+  E.nucEmin = e->fENucEnergyGridSize > 0 ? e->fENucEnergyGrid[0] : 0.0;
+  E.nucEmax = e->fENucEnergyGridSize > 0 ? e->fENucEnergyGrid[e->fENucEnergyGridSize - 1] : 0.0;
   P.addR(E.nucData, e->fENucMacXsecData, (size_t)2 * e->fENucEnergyGridSize * e->fNumMaterials);
   P.addR(E.tr1, e->fTr1MacXSecData, (size_t)2 * N * e->fNumMaterials);
   P.add(E.selSBStart, e->fElemSelectorBremSBStartIndexPerMatCut, e->fNumMatCuts);

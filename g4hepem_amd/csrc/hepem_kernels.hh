@@ -916,41 +916,28 @@ __global__ void __launch_bounds__(kBlock) k_interact_tiles_batched(DevTables<R> 
                        (int)(tile - d[b].tile0) * kClassifyTile, chunk);
 }
 
-// the five config-1 quantities of query i, whose inputs are (mc, E, lnE)
+// the five config-1 quantities of query i at (mc, E, lnE): restricted dE/dx,
+// range, inverse range of that range, ioni and brem cross sections; NaN for
+// an invalid couple. Those are computed at couple 0 and then replaced, so
+// every lane stores at the same places in the code (no branch around the
+// memory operations: the wait for the next query's inputs then covers no
+// store that a branch left unaccounted).
 template <typename R>
-HD void LookupValues(const DevEl<R>& e, int nmc, int n, int i, int mc, double E, double lnE, double* __restrict__ out) {
+HD void LookupStore(const DevEl<R>& e, int nmc, int n, int i, int mc, double E, double lnE, double* __restrict__ out) {
   const size_t sn = (size_t)n;  // 5 n entries may exceed the int range
-  if (mc < 0 || mc >= nmc) {
-    for (int k = 0; k < 5; ++k) out[k * sn + i] = __longlong_as_double(0x7FF8000000000000LL);
-    return;
-  }
+  const bool ok = mc >= 0 && mc < nmc;
+  const double nan = __longlong_as_double(0x7FF8000000000000LL);
+  mc = ok ? mc : 0;
   const R lE = (R)lnE;
+  // the range's loss-grid bin is where its inverse is searched first (the
+  // spline's grid energies are the inverse's ordinates there)
+  const int bin = LogBin(lE, e.lossLogMin, e.lossILD, e.N);
   const R r = GetRestRange(e, mc, E, lE);
-  out[i] = (double)GetRestDEDX(e, mc, E, lE);
-  out[sn + i] = (double)r;
-  out[2 * sn + i] = GetInvRange(e, mc, (double)r, LogBin(lE, e.lossLogMin, e.lossILD, e.N));
-  out[3 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, true);
-  out[4 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, false);
-}
-
-// the five config-1 quantities of query i
-template <typename R>
-HD void LookupOne(const DevEl<R>& e, int nmc, int n, int i, const int32_t* __restrict__ imc, const double* __restrict__ ekin,
-                  const double* __restrict__ lekin, double* __restrict__ out) {
-  const int mc = imc[i];
-  const size_t sn = (size_t)n;  // 5 n entries may exceed the int range
-  if (mc < 0 || mc >= nmc) {
-    for (int k = 0; k < 5; ++k) out[k * sn + i] = __longlong_as_double(0x7FF8000000000000LL);
-    return;
-  }
-  const double E = ekin[i];
-  const R lE = (R)lekin[i];
-  const R r = GetRestRange(e, mc, E, lE);
-  out[i] = (double)GetRestDEDX(e, mc, E, lE);
-  out[sn + i] = (double)r;
-  out[2 * sn + i] = GetInvRange(e, mc, (double)r, LogBin(lE, e.lossLogMin, e.lossILD, e.N));
-  out[3 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, true);
-  out[4 * sn + i] = (double)GetRestMacXSec(e, mc, E, lE, false);
+  out[i] = ok ? (double)GetRestDEDX(e, mc, E, lE) : nan;
+  out[sn + i] = ok ? (double)r : nan;
+  out[2 * sn + i] = ok ? GetInvRangeInBin(e, mc, (double)r, bin, e.lossGrid64[bin], e.lossGrid64[bin + 1]) : nan;
+  out[3 * sn + i] = ok ? (double)GetRestMacXSec(e, mc, E, lE, true) : nan;
+  out[4 * sn + i] = ok ? (double)GetRestMacXSec(e, mc, E, lE, false) : nan;
 }
 
 template <typename R>
@@ -959,7 +946,7 @@ __global__ void __launch_bounds__(kBlock) k_lookup(DevTables<R> T, int ptype, in
                                                    double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  LookupOne(T.el[ptype], T.nmc, n, i, imc, ekin, lekin, out);
+  LookupStore(T.el[ptype], T.nmc, n, i, imc[i], ekin[i], lekin[i], out);
 }
 
 // Gamma macroscopic cross sections and the interaction they select, one query
@@ -1057,7 +1044,8 @@ __global__ void __launch_bounds__(kBlock) k_lookup_lds(DevTables<R> T, int ptype
   }
   __syncthreads();
   // the next query's inputs are loaded before this one is computed, so their
-  // HBM latency overlaps the table work
+  // HBM latency overlaps the table work; the last round reloads its own
+  // query instead, so the loads are unconditional
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1065,14 +1053,10 @@ __global__ void __launch_bounds__(kBlock) k_lookup_lds(DevTables<R> T, int ptype
   double E = ekin[i], lnE = lekin[i];
   while (true) {
     const int64_t next = i + stride;
-    int mcN = 0;
-    double EN = 0.0, lnEN = 0.0;
-    if (next < n) {
-      mcN = imc[next];
-      EN = ekin[next];
-      lnEN = lekin[next];
-    }
-    LookupValues(e, nmc, n, (int)i, mc, E, lnE, out);
+    const int64_t ld = next < n ? next : i;
+    const int mcN = imc[ld];
+    const double EN = ekin[ld], lnEN = lekin[ld];
+    LookupStore(e, nmc, n, (int)i, mc, E, lnE, out);
     if (next >= n) break;
     i = next;
     mc = mcN;
