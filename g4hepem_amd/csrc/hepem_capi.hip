@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -496,7 +497,9 @@ extern "C" int hepem_lookup_eloss_xsec(const hepem_ctx* c, int ptype, int32_t n,
   const bool fp64 = c->precision == HEPEM_FP64;
   const size_t lds = fp64 ? LookupLdsBytes(c->t64.el[ptype], c->t64.nmc) : LookupLdsBytes(c->t32.el[ptype], c->t32.nmc);
   if (c->lookupStaging && lds <= kLookupLdsMax) {
-    // one resident round of blocks: each block stages the tables once
+    // blocks of kLookupPerThread queries per thread, fewer per thread when
+    // that leaves less than one resident round of blocks; each block stages
+    // the tables once
     int& resident = const_cast<hepem_ctx*>(c)->lookupGrid[fp64 ? 0 : 1][ptype];
     if (resident == 0) {
       int cus = 0, perCu = 0;
@@ -507,7 +510,8 @@ extern "C" int hepem_lookup_eloss_xsec(const hepem_ctx* c, int ptype, int32_t n,
         HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_lookup_lds<float>, kBlock, lds));
       resident = cus * perCu > 0 ? cus * perCu : kLookupBlocks;
     }
-    const int sgrid = grid < resident ? grid : resident;
+    const int64_t perThread = std::max<int64_t>(1, std::min<int64_t>(kLookupPerThread, (int64_t)grid / resident));
+    const int sgrid = (int)(((int64_t)n + kBlock * perThread - 1) / (kBlock * perThread));
     if (fp64)
       hipLaunchKernelGGL(k_lookup_lds<double>, dim3(sgrid), dim3(kBlock), lds, s, c->t64, ptype, n, imc, ekin, lekin, out);
     else

@@ -921,7 +921,8 @@ __global__ void __launch_bounds__(kBlock) k_interact_tiles_batched(DevTables<R> 
 // an invalid couple. Those are computed at couple 0 and then replaced, so
 // every lane stores at the same places in the code (no branch around the
 // memory operations: the wait for the next query's inputs then covers no
-// store that a branch left unaccounted).
+// store that a branch left unaccounted). The results are streamed out
+// (non-temporal stores): nothing in the kernel reads them back.
 template <typename R>
 HD void LookupStore(const DevEl<R>& e, int nmc, int n, int i, int mc, double E, double lnE, double* __restrict__ out) {
   const size_t sn = (size_t)n;  // 5 n entries may exceed the int range
@@ -933,11 +934,11 @@ HD void LookupStore(const DevEl<R>& e, int nmc, int n, int i, int mc, double E, 
   // spline's grid energies are the inverse's ordinates there)
   const int bin = LogBin(lE, e.lossLogMin, e.lossILD, e.N);
   const R r = GetRestRange(e, mc, E, lE);
-  out[i] = ok ? (double)GetRestDEDX(e, mc, E, lE) : nan;
-  out[sn + i] = ok ? (double)r : nan;
-  out[2 * sn + i] = ok ? GetInvRangeInBin(e, mc, (double)r, bin, e.lossGrid64[bin], e.lossGrid64[bin + 1]) : nan;
-  out[3 * sn + i] = ok ? (double)GetRestMacXSec(e, mc, E, lE, true) : nan;
-  out[4 * sn + i] = ok ? (double)GetRestMacXSec(e, mc, E, lE, false) : nan;
+  __builtin_nontemporal_store(ok ? (double)GetRestDEDX(e, mc, E, lE) : nan, out + i);
+  __builtin_nontemporal_store(ok ? (double)r : nan, out + sn + i);
+  __builtin_nontemporal_store(ok ? GetInvRangeInBin(e, mc, (double)r, bin, e.lossGrid64[bin], e.lossGrid64[bin + 1]) : nan, out + 2 * sn + i);
+  __builtin_nontemporal_store(ok ? (double)GetRestMacXSec(e, mc, E, lE, true) : nan, out + 3 * sn + i);
+  __builtin_nontemporal_store(ok ? (double)GetRestMacXSec(e, mc, E, lE, false) : nan, out + 4 * sn + i);
 }
 
 template <typename R>
@@ -986,10 +987,11 @@ __global__ void __launch_bounds__(kBlock) k_gamma_lookup(DevTables<R> T, int n, 
 // LDS staging of the config-1 lookups. The loss grid and data and the
 // cross-section blocks of every couple (the doubles, and in the FP32 form
 // the floats too) are a few kB per couple; each block copies them to LDS
-// once and then serves a grid-stride share of the queries from there, so the
+// once and then serves a contiguous chunk of the queries from there, so the
 // table gathers leave the texture path to the query stream.
 constexpr size_t kLookupLdsMax = 64 * 1024;  // larger images use k_lookup
-constexpr int kLookupBlocks = 2048;          // grid if the occupancy query gives nothing
+constexpr int kLookupBlocks = 2048;          // resident blocks if the occupancy query gives nothing
+constexpr int kLookupPerThread = 16;         // queries per thread of a chunk, at most
 
 __host__ __device__ inline size_t Al16(size_t b) { return (b + 15) & ~size_t(15); }
 
@@ -1043,21 +1045,26 @@ __global__ void __launch_bounds__(kBlock) k_lookup_lds(DevTables<R> T, int ptype
     e.xsData = static_cast<const R*>(StageArray(lds, off, e.xsData, X * sizeof(R)));
   }
   __syncthreads();
-  // the next query's inputs are loaded before this one is computed, so their
-  // HBM latency overlaps the table work; the last round reloads its own
-  // query instead, so the loads are unconditional
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  // each block serves one contiguous chunk of `per` queries per thread; the
+  // next query's inputs are loaded before this one is computed, so their HBM
+  // latency overlaps the table work (the chunk's last round reloads its own
+  // query instead, so the loads are unconditional)
+  const int64_t threads = (int64_t)gridDim.x * blockDim.x;
+  const int64_t per = ((int64_t)n + threads - 1) / threads;
+  const int64_t base = (int64_t)blockIdx.x * blockDim.x * per;
+  const int64_t end = min((int64_t)n, base + per * (int64_t)blockDim.x);
+  const int64_t stride = blockDim.x;
+  int64_t i = base + threadIdx.x;
+  if (i >= end) return;
   int mc = imc[i];
   double E = ekin[i], lnE = lekin[i];
   while (true) {
     const int64_t next = i + stride;
-    const int64_t ld = next < n ? next : i;
+    const int64_t ld = next < end ? next : i;
     const int mcN = imc[ld];
     const double EN = ekin[ld], lnEN = lekin[ld];
     LookupStore(e, nmc, n, (int)i, mc, E, lnE, out);
-    if (next >= n) break;
+    if (next >= end) break;
     i = next;
     mc = mcN;
     E = EN;
