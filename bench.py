@@ -153,15 +153,42 @@ def verify_batches(ctx, batches, seed) -> dict:
             "mismatches": {str(k): v for k, v in list(bad.items())[:4]}}
 
 
-def time_batched(ctx, batches, seed, reps):
+def _shape_name(block, tile, persistent):
+    return f"b{block}-{'sort' + str(tile) if tile else 'fused'}-{'persistent' if persistent else 'dispatched'}"
+
+
+def sweep_batched(plan, seed, reps=4) -> dict:
+    """ms per direct run of `plan` (HIP events, default stream, after one
+    untimed run) for every launch shape the tuner chooses from."""
+    import torch
+    from g4hepem_amd import BatchPlan
+    out = {}
+    for block, tile, pers in BatchPlan.shapes():
+        plan.set_shape(block, tile, pers)
+        plan.run(seed)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            plan.run(seed)
+        b.record()
+        torch.cuda.synchronize()
+        out[_shape_name(block, tile, pers)] = a.elapsed_time(b) / reps
+    return out
+
+
+def time_batched(ctx, batches, seed, reps, sweep=False):
     """A batch plan over `batches` (device SoA triples): tune the launch shape,
     capture one run into a hipGraph and time `reps` replays with HIP events on
     the replay stream; then check the last replay's outputs against hepem_step
-    on each batch. Returns (ms per run, (block, grid), check)."""
+    on each batch. With `sweep`, first time every shape the tuner chooses from.
+    Returns (ms per run, chosen shape, check, sweep or None)."""
     import torch
     from g4hepem_amd import BatchPlan
     plan = BatchPlan(ctx, batches)
-    shape = plan.tune(seed)
+    swept = sweep_batched(plan, seed) if sweep else None
+    plan.tune(seed)
+    shape = plan.get_shape()
+    shape["name"] = _shape_name(shape["block"], shape["sort_tile"], shape["persistent"])
     s = torch.cuda.Stream()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.stream(s):
@@ -180,7 +207,32 @@ def time_batched(ctx, batches, seed, reps):
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
     plan.close()
-    return ms, shape, verify_batches(ctx, batches, seed)
+    return ms, shape, verify_batches(ctx, batches, seed), swept
+
+
+def run_config3(ctx64, ctx32, seed: int, reps: int) -> dict:
+    """The 64-batch workload in both forms: tuned plan, graph replays, the
+    per-shape sweep and the bit-identity check against unbatched steps."""
+    import torch
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA
+    dev = torch.device("cuda:0")
+    host_batches = config3_batches(seed)
+    ntot = sum(len(b["ekin"]) for b in host_batches)
+    dev_batches = [(TrackSoA.from_numpy(b, "torch", dev), TrackSoA(len(b["ekin"]), "torch", dev),
+                    StepOutSoA(len(b["ekin"]), "torch", dev)) for b in host_batches]
+    del host_batches
+    ms_b64, shape64, chk64, sw64 = time_batched(ctx64, dev_batches, seed, reps, sweep=True)
+    ms_b32, shape32, chk32, sw32 = time_batched(ctx32, dev_batches, seed, reps, sweep=True)
+    del dev_batches
+    torch.cuda.empty_cache()
+    return {
+        "batches": 64, "tracks": ntot, "launch": "hipGraph replay of a tuned batch plan",
+        "fp64": {"value": ntot / (ms_b64 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b64,
+                 "shape": shape64, **chk64, "shape_sweep_ms": sw64},
+        "fp32_table_form": {"value": ntot / (ms_b32 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b32,
+                            "shape": shape32, **chk32, "shape_sweep_ms": sw32},
+        "speedup_fp32_vs_fp64": ms_b64 / ms_b32,
+    }
 
 
 def _chi2_means(a, b):
@@ -430,6 +482,7 @@ def main():
                     help="config-5 runs: the FP64 form (always), the FP32-table form, FP64 with Woodcock tracking")
     ap.add_argument("--lookups-only", action="store_true",
                     help="config-1 lookups only, staged and from the table image (for profiling); prints their JSON object")
+    ap.add_argument("--batched-only", action="store_true", help="config-3 batch plans only (tuner sweep included)")
     ap.add_argument("--shower-only", action="store_true",
                     help="config-5 slab showers only, without the CPU harness (for profiling); prints their JSON object")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -457,6 +510,11 @@ def main():
                                               "queries_per_s_100M": q100m, "ms_100M": ms100m,
                                               "tables_in_image": {"queries_per_s_10M": g10m, "ms_10M": gms10m}}}),
               flush=True)
+        return
+    if args.batched_only:
+        tables2 = Tables(SET2, 0.7)
+        print(json.dumps({"config3_batched": run_config3(Context(tables2, HEPEM_FP64), Context(tables2, HEPEM_FP32T),
+                                                         args.seed, max(3, args.steps // 4))}), flush=True)
         return
     if args.shower_only:
         tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr
@@ -544,22 +602,7 @@ def main():
         del tin, tout, res
         torch.cuda.empty_cache()
         log("config-3 batches")
-        host_batches = config3_batches(args.seed)
-        ntot = sum(len(b["ekin"]) for b in host_batches)
-        dev_batches = [(TrackSoA.from_numpy(b, "torch", dev), TrackSoA(len(b["ekin"]), "torch", dev),
-                        StepOutSoA(len(b["ekin"]), "torch", dev)) for b in host_batches]
-        del host_batches
-        ms_b64, shape64, chk64 = time_batched(ctx64, dev_batches, args.seed, max(3, args.steps // 4))
-        ms_b32, shape32, chk32 = time_batched(ctx32, dev_batches, args.seed, max(3, args.steps // 4))
-        line["config3_batched"] = {
-            "batches": len(dev_batches), "tracks": ntot, "launch": "hipGraph replay of a tuned batch plan",
-            "fp64": {"value": ntot / (ms_b64 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b64,
-                     "block": shape64[0], **chk64},
-            "fp32_table_form": {"value": ntot / (ms_b32 * 1e-3), "unit": "track-steps/s", "ms_per_run": ms_b32,
-                                "block": shape32[0], **chk32},
-            "speedup_fp32_vs_fp64": ms_b64 / ms_b32,
-        }
-        del dev_batches
+        line["config3_batched"] = run_config3(ctx64, ctx32, args.seed, max(3, args.steps // 4))
     if not args.no_extras and args.shower_events > 0:
         log("config-5 slab showers")
         tables_slab = Tables(SET1, 0.7)   # couples 0: G4_Pb, 1: G4_lAr

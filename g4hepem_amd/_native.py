@@ -87,6 +87,11 @@ class BatchDesc(C.Structure):
     _fields_ = [("inp", Tracks), ("out", Tracks), ("res", StepOut)]
 
 
+class BatchShape(C.Structure):
+    """hepem_batch_shape: launch shape of a batch plan's step."""
+    _fields_ = [("block", C.c_int32), ("sort_tile", C.c_int32), ("persistent", C.c_int32), ("grid", C.c_int32)]
+
+
 # field name -> (dtype name, components)
 TRACK_FIELDS = {
     "ptype": ("int8", 1), "id": ("int32", 1), "imc": ("int32", 1), "ekin": ("float64", 1),
@@ -155,6 +160,8 @@ def lib() -> C.CDLL:
     L.hepem_batch_plan_run.argtypes = [vp, u64, vp]
     L.hepem_batch_plan_tune.argtypes = [vp, u64, vp]
     L.hepem_batch_plan_shape.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+    L.hepem_batch_plan_get_shape.argtypes = [vp, C.POINTER(BatchShape)]
+    L.hepem_batch_plan_set_shape.argtypes = [vp, C.POINTER(BatchShape)]
     L.hepem_batch_plan_rejected.argtypes = [vp, i32, C.POINTER(C.c_uint32), vp]
     L.hepem_tracks_reset_state.argtypes = [C.POINTER(Tracks), vp]
     for name in ("hepem_electron_step_begin", "hepem_electron_substep_howfar", "hepem_electron_substep_along"):
@@ -175,7 +182,8 @@ def lib() -> C.CDLL:
                  "hepem_step_batched", "hepem_tracks_reset_state", "hepem_howfar", "hepem_perform",
                  "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform",
                  "hepem_batch_plan_create", "hepem_batch_plan_destroy", "hepem_batch_plan_run", "hepem_batch_plan_tune",
-                 "hepem_batch_plan_shape", "hepem_batch_plan_rejected", "hepem_slab_run", "hepem_electron_step_begin",
+                 "hepem_batch_plan_shape", "hepem_batch_plan_get_shape", "hepem_batch_plan_set_shape",
+                 "hepem_batch_plan_rejected", "hepem_slab_run", "hepem_electron_step_begin",
                  "hepem_electron_substep_howfar", "hepem_electron_substep_along", "hepem_electron_substep_active",
                  "hepem_electron_step_end"):
         getattr(L, name).restype = C.c_int

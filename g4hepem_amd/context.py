@@ -216,6 +216,31 @@ class BatchPlan:
         N.check("hepem_batch_plan_shape", N.lib().hepem_batch_plan_shape(self.h, C.byref(b), C.byref(g)))
         return b.value, g.value
 
+    def get_shape(self) -> dict:
+        """The full launch shape: block, sort_tile (0: no process sort),
+        persistent, grid."""
+        x = N.BatchShape()
+        N.check("hepem_batch_plan_get_shape", N.lib().hepem_batch_plan_get_shape(self.h, C.byref(x)))
+        return {"block": x.block, "sort_tile": x.sort_tile, "persistent": x.persistent, "grid": x.grid}
+
+    def set_shape(self, block: int, sort_tile: int, persistent: int) -> dict:
+        x = N.BatchShape(block, sort_tile, persistent, 0)
+        N.check("hepem_batch_plan_set_shape", N.lib().hepem_batch_plan_set_shape(self.h, C.byref(x)))
+        return self.get_shape()
+
+    @staticmethod
+    def shapes(sort_modes=(0, 1)):
+        """Every launch shape hepem_batch_plan_tune may choose from, as
+        (block, sort_tile, persistent); sort_modes: 0 without, 1 with the
+        process sort."""
+        out = []
+        for srt in sort_modes:
+            for block in (64, 128, 256):
+                for tile in ((1024, 2048, 4096, 8192) if srt else (0,)):
+                    for pers in (0, 1):
+                        out.append((block, tile, pers))
+        return out
+
     def rejected(self, b: int, stream=None) -> int:
         v = C.c_uint32()
         N.check("hepem_batch_plan_rejected", N.lib().hepem_batch_plan_rejected(self.h, b, C.byref(v), _stream_handle(stream)))

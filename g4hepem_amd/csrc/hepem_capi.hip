@@ -874,35 +874,74 @@ struct hepem_batch_plan {
   const hepem_ctx* ctx = nullptr;
   int nbatch = 0;
   uint64_t total = 0;             // tracks over all batches
-  uint32_t ntiles = 0;            // classify tiles over all batches
+  uint32_t ntiles = 0;            // particle-classify tiles over all batches (kClassifyTile slots)
+  uint32_t nptiles[kNumTiles] = {0, 0, 0, 0};  // process-sort tiles over all batches, per tile size
   uint32_t nchunks[kNumShapes] = {0, 0, 0};
   BatchDev* dDescs = nullptr;
   uint32_t* dQueue = nullptr;
-  uint32_t* dCounters = nullptr;  // 4 per batch
+  uint32_t* dCounters = nullptr;  // 4 per batch, then kQueueWords work counters
   // process sort of the pending interactions over all batches (as hepem_step)
+  int sortMode = HEPEM_SORT_AUTO; // the context's mode at plan creation: what the tuner may choose from
   bool processSort = true;
   uint8_t* dCode = nullptr;       // process code per global slot
   Staged* dStaged = nullptr;      // staged record per global slot
   uint16_t* dTList = nullptr;     // tile-local process layout, batch b's slots from qoff_b
-  uint32_t* dTInfo = nullptr;     // kTileInfo words per classification tile (tiles numbered across batches)
+  uint32_t* dTInfo = nullptr;     // kTileInfo words per process-sort tile (tiles numbered across batches)
   int shape = kNumShapes - 1;     // block size ShapeBlock(shape)
-  int grid = 0;                   // step-kernel blocks (nchunks[shape])
+  int tile = 2;                   // process-sort tile size TileSlots(tile)
+  bool persistent = false;        // step kernels as one resident round of blocks taking work from counters
+  int grid = 0;                   // step-kernel blocks (nchunks[shape], or the resident round)
+  int igrid = 0;                  // blocks of the persistent tile kernel (its resident round)
 };
 
 namespace {
-template <typename R, int S>
+// one resident round of blocks of `kernel` at `block` threads, a multiple of
+// kXcds (the persistent tile kernel gives every XCD's counter its blocks)
+template <typename K> int ResidentGrid(const hepem_ctx* c, K kernel, int block) {
+  int cus = 0, perCu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, block, 0) != hipSuccess || cus * perCu <= 0)
+    return 2048;
+  return (cus * perCu + kXcds - 1) / kXcds * kXcds;
+}
+
+template <typename R, int S, int TI>
 void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
+  uint32_t* next = p->dCounters + 4 * p->nbatch;
   if (!p->processSort) {
-    hipLaunchKernelGGL((k_step_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
-                       p->nchunks[S], p->dQueue, p->dCounters);
+    if (p->persistent)
+      hipLaunchKernelGGL((k_step_batched_q<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+                         p->nchunks[S], p->dQueue, p->dCounters, next);
+    else
+      hipLaunchKernelGGL((k_step_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+                         p->nchunks[S], p->dQueue, p->dCounters);
     return;
   }
-  hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
-                     p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
-  hipLaunchKernelGGL(k_classify_tiles_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dCode,
+  if (p->persistent)
+    hipLaunchKernelGGL((k_step_select_batched_q<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs,
+                       p->nbatch, p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged, next);
+  else
+    hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+                       p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
+  const uint32_t nt = p->nptiles[TI];
+  hipLaunchKernelGGL(k_classify_tiles_batched<TI>, dim3(nt), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dCode,
                      p->dTList, p->dTInfo);
-  hipLaunchKernelGGL(k_interact_tiles_batched<R>, dim3(InteractGridTiles((int)p->ntiles)), dim3(kBlock), 0, s, T, seed, p->dDescs,
-                     p->nbatch, p->ntiles, p->dCode, p->dStaged, p->dTList, p->dTInfo);
+  if (p->persistent) {
+    hipLaunchKernelGGL((k_interact_tiles_batched_q<R, TI>), dim3(p->igrid), dim3(kBlock), 0, s, T, seed, p->dDescs, p->nbatch,
+                       nt, p->dCode, p->dStaged, p->dTList, p->dTInfo, next + 1);
+  } else {
+    hipLaunchKernelGGL((k_interact_tiles_batched<R, TI>), dim3(InteractGridTiles((int)nt, InteractParts(TileItems(TI)))),
+                       dim3(kBlock), 0, s, T, seed, p->dDescs, p->nbatch, nt, p->dCode, p->dStaged, p->dTList, p->dTInfo);
+  }
+}
+template <typename R, int S>
+void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
+  switch (p->tile) {
+    case 0: LaunchStepBatched<R, S, 0>(T, p, seed, s); break;
+    case 1: LaunchStepBatched<R, S, 1>(T, p, seed, s); break;
+    case 3: LaunchStepBatched<R, S, 3>(T, p, seed, s); break;
+    default: LaunchStepBatched<R, S, 2>(T, p, seed, s); break;
+  }
 }
 template <typename R> void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_t seed, hipStream_t s) {
   switch (p->shape) {
@@ -910,6 +949,34 @@ template <typename R> void LaunchStepBatched(const DevTables<R>& T, const hepem_
     case 1: LaunchStepBatched<R, 1>(T, p, seed, s); break;
     default: LaunchStepBatched<R, 2>(T, p, seed, s); break;
   }
+}
+
+// blocks of the step kernel for the plan's shape: every chunk, or one resident round
+template <typename R, int S> int StepGrid(const hepem_batch_plan* p) {
+  if (!p->persistent) return (int)(p->nchunks[S] > 0 ? p->nchunks[S] : 1);
+  return p->processSort ? ResidentGrid(p->ctx, k_step_select_batched_q<R, S>, ShapeBlock(S))
+                        : ResidentGrid(p->ctx, k_step_batched_q<R, S>, ShapeBlock(S));
+}
+int StepGrid(const hepem_batch_plan* p) {
+  const bool fp64 = p->ctx->precision == HEPEM_FP64;
+  switch (p->shape) {
+    case 0: return fp64 ? StepGrid<double, 0>(p) : StepGrid<float, 0>(p);
+    case 1: return fp64 ? StepGrid<double, 1>(p) : StepGrid<float, 1>(p);
+    default: return fp64 ? StepGrid<double, 2>(p) : StepGrid<float, 2>(p);
+  }
+}
+template <typename R> int TileGrid(const hepem_batch_plan* p) {
+  switch (p->tile) {
+    case 0: return ResidentGrid(p->ctx, k_interact_tiles_batched_q<R, 0>, kBlock);
+    case 1: return ResidentGrid(p->ctx, k_interact_tiles_batched_q<R, 1>, kBlock);
+    case 3: return ResidentGrid(p->ctx, k_interact_tiles_batched_q<R, 3>, kBlock);
+    default: return ResidentGrid(p->ctx, k_interact_tiles_batched_q<R, 2>, kBlock);
+  }
+}
+// the grids of the plan's current shape
+void SetGrids(hepem_batch_plan* p) {
+  p->grid = StepGrid(p);
+  p->igrid = !p->persistent ? 0 : p->ctx->precision == HEPEM_FP64 ? TileGrid<double>(p) : TileGrid<float>(p);
 }
 
 void FreePlan(hepem_batch_plan* p) {
@@ -930,7 +997,7 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
   *out = nullptr;
   if (!c || nbatch < 1 || !descs) return HEPEM_ERR_INVALID;
   std::vector<BatchDev> h((size_t)nbatch + 1);
-  uint64_t q = 0, tiles = 0, chunks[kNumShapes] = {0, 0, 0};
+  uint64_t q = 0, tiles = 0, chunks[kNumShapes] = {0, 0, 0}, ptiles[kNumTiles] = {0, 0, 0, 0};
   for (int b = 0; b < nbatch; ++b) {
     const hepem_batch_desc& d = descs[b];
     if (!TracksOk(&d.in) || !TracksOk(&d.out) || !ResOk(&d.res, d.in.n) || d.out.n != d.in.n) return HEPEM_ERR_INVALID;
@@ -940,10 +1007,12 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
     x.res = d.res;
     x.qoff = (uint32_t)q;
     x.tile0 = (uint32_t)tiles;
+    for (int t = 0; t < kNumTiles; ++t) x.ptile0[t] = (uint32_t)ptiles[t];
     for (int s = 0; s < kNumShapes; ++s) x.chunk0[s] = (uint32_t)chunks[s];
     const uint64_t n = (uint64_t)d.in.n;
     q += n;
     tiles += (n + kClassifyTile - 1) / kClassifyTile;
+    for (int t = 0; t < kNumTiles; ++t) ptiles[t] += (n + TileSlots(t) - 1) / TileSlots(t);
     for (int s = 0; s < kNumShapes; ++s) chunks[s] += (n + ShapeBlock(s) - 1) / ShapeBlock(s);
   }
   if (q >= (1ull << 31)) return HEPEM_ERR_CAPACITY;
@@ -951,23 +1020,27 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
   std::memset(&last, 0, sizeof(last));
   last.qoff = (uint32_t)q;
   last.tile0 = (uint32_t)tiles;
+  for (int t = 0; t < kNumTiles; ++t) last.ptile0[t] = (uint32_t)ptiles[t];
   for (int s = 0; s < kNumShapes; ++s) last.chunk0[s] = (uint32_t)chunks[s];
 
   hepem_batch_plan* p = new (std::nothrow) hepem_batch_plan();
   if (!p) return HEPEM_ERR_ALLOC;
   p->ctx = c;
+  p->sortMode = c->processSort;
   p->processSort = SortsProcesses(c, q);
   p->nbatch = nbatch;
   p->total = q;
   p->ntiles = (uint32_t)tiles;
+  for (int t = 0; t < kNumTiles; ++t) p->nptiles[t] = (uint32_t)ptiles[t];
   for (int s = 0; s < kNumShapes; ++s) p->nchunks[s] = (uint32_t)chunks[s];
-  p->grid = (int)(p->nchunks[p->shape] > 0 ? p->nchunks[p->shape] : 1);
+  SetGrids(p);
+  // the smallest tile size has the most tiles: its info words fit every size
   if (hipMalloc(&p->dDescs, h.size() * sizeof(BatchDev)) != hipSuccess ||
       hipMalloc(&p->dQueue, (q > 0 ? q : 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&p->dCounters, (size_t)4 * nbatch * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&p->dCounters, ((size_t)4 * nbatch + kQueueWords) * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&p->dCode, (q > 0 ? q : 1)) != hipSuccess || hipMalloc(&p->dStaged, (q > 0 ? q : 1) * sizeof(Staged)) != hipSuccess ||
       hipMalloc(&p->dTList, (q > 0 ? q : 1) * sizeof(uint16_t)) != hipSuccess ||
-      hipMalloc(&p->dTInfo, (size_t)kTileInfo * (tiles > 0 ? tiles : 1) * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&p->dTInfo, (size_t)kTileInfo * (ptiles[0] > 0 ? ptiles[0] : 1) * sizeof(uint32_t)) != hipSuccess) {
     FreePlan(p);
     return HEPEM_ERR_ALLOC;
   }
@@ -988,7 +1061,7 @@ extern "C" int hepem_batch_plan_destroy(hepem_batch_plan* p) {
 extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, void* stream) {
   if (!p || !p->ctx) return HEPEM_ERR_INVALID;
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  const int nc = 4 * p->nbatch;
+  const int nc = 4 * p->nbatch + kQueueWords;
   hipLaunchKernelGGL(k_fill_u32, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dCounters, nc, 0u);
   if (p->total > 0) {
     hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue, p->dCounters);
@@ -999,41 +1072,61 @@ extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, vo
   return HEPEM_OK;
 }
 
+namespace {
+struct PlanShape {
+  int shape, tile;
+  bool sort, persistent;
+};
+void ApplyShape(hepem_batch_plan* p, const PlanShape& x) {
+  p->shape = x.shape;
+  p->tile = x.tile;
+  p->processSort = x.sort;
+  p->persistent = x.persistent;
+  SetGrids(p);
+}
+}  // namespace
+
 extern "C" int hepem_batch_plan_tune(hepem_batch_plan* p, uint64_t seed, void* stream) {
   if (!p || !p->ctx) return HEPEM_ERR_INVALID;
   if (p->total == 0) return HEPEM_OK;
   const hipStream_t s = static_cast<hipStream_t>(stream);
+  // the shapes the plan may take: the sort modes its context allows, every
+  // block size, every tile size when sorting, dispatched or persistent
+  std::vector<PlanShape> cand;
+  for (int so = 0; so < 2; ++so) {
+    const bool sort = so == 1;
+    if ((p->sortMode == HEPEM_SORT_OFF && sort) || (p->sortMode == HEPEM_SORT_ON && !sort)) continue;
+    for (int sh = 0; sh < kNumShapes; ++sh)
+      for (int t = 0; t < (sort ? kNumTiles : 1); ++t)
+        for (int pe = 0; pe < 2; ++pe) cand.push_back({sh, sort ? t : 2, sort, pe == 1});
+  }
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return HEPEM_ERR_HIP; }
-  float best = 1e30f;
-  int bestShape = p->shape, bestGrid = p->grid;
+  const PlanShape start{p->shape, p->tile, p->processSort, p->persistent};
+  PlanShape best = start;
+  float bestMs = 1e30f;
   int rc = HEPEM_OK;
-  for (int sh = 0; sh < kNumShapes && rc == HEPEM_OK; ++sh) {
-    {
-      p->shape = sh;
-      p->grid = (int)p->nchunks[sh];
-      rc = hepem_batch_plan_run(p, seed, stream);  // warm-up
-      if (rc != HEPEM_OK) break;
-      if (hipEventRecord(e0, s) != hipSuccess) { rc = HEPEM_ERR_HIP; break; }
-      for (int r = 0; r < 3 && rc == HEPEM_OK; ++r) rc = hepem_batch_plan_run(p, seed, stream);
-      if (rc != HEPEM_OK || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-        rc = rc != HEPEM_OK ? rc : HEPEM_ERR_HIP;
-        break;
-      }
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = HEPEM_ERR_HIP; break; }
-      if (ms < best) {
-        best = ms;
-        bestShape = sh;
-        bestGrid = p->grid;
-      }
+  for (const PlanShape& x : cand) {
+    ApplyShape(p, x);
+    rc = hepem_batch_plan_run(p, seed, stream);  // warm-up
+    if (rc != HEPEM_OK) break;
+    if (hipEventRecord(e0, s) != hipSuccess) { rc = HEPEM_ERR_HIP; break; }
+    for (int r = 0; r < 3 && rc == HEPEM_OK; ++r) rc = hepem_batch_plan_run(p, seed, stream);
+    if (rc != HEPEM_OK || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = rc != HEPEM_OK ? rc : HEPEM_ERR_HIP;
+      break;
+    }
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = HEPEM_ERR_HIP; break; }
+    if (ms < bestMs) {
+      bestMs = ms;
+      best = x;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  p->shape = bestShape;
-  p->grid = bestGrid;
+  ApplyShape(p, rc == HEPEM_OK ? best : start);
   return rc;
 }
 
@@ -1041,6 +1134,31 @@ extern "C" int hepem_batch_plan_shape(const hepem_batch_plan* p, int32_t* block,
   if (!p || !block || !grid) return HEPEM_ERR_INVALID;
   *block = ShapeBlock(p->shape);
   *grid = p->grid;
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_batch_plan_get_shape(const hepem_batch_plan* p, hepem_batch_shape* out) {
+  if (!p || !out) return HEPEM_ERR_INVALID;
+  out->block = ShapeBlock(p->shape);
+  out->sort_tile = p->processSort ? TileSlots(p->tile) : 0;
+  out->persistent = p->persistent ? 1 : 0;
+  out->grid = p->grid;
+  return HEPEM_OK;
+}
+
+extern "C" int hepem_batch_plan_set_shape(hepem_batch_plan* p, const hepem_batch_shape* in) {
+  if (!p || !p->ctx || !in || (in->persistent != 0 && in->persistent != 1)) return HEPEM_ERR_INVALID;
+  PlanShape x{-1, 2, in->sort_tile != 0, in->persistent == 1};
+  for (int sh = 0; sh < kNumShapes; ++sh)
+    if (in->block == ShapeBlock(sh)) x.shape = sh;
+  if (x.shape < 0) return HEPEM_ERR_INVALID;
+  if (x.sort) {
+    x.tile = -1;
+    for (int t = 0; t < kNumTiles; ++t)
+      if (in->sort_tile == TileSlots(t)) x.tile = t;
+    if (x.tile < 0) return HEPEM_ERR_INVALID;
+  }
+  ApplyShape(p, x);
   return HEPEM_OK;
 }
 

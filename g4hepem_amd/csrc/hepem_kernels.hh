@@ -680,25 +680,30 @@ __global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint6
 constexpr int kXcds = 8;
 constexpr int kTileInfo = 2 * (kNumProc + 1);   // per tile: process offsets, first chunks
 constexpr int kInteractWaves = kBlock / 64;
-// chunks of a tile: at most one per 64 slots plus one partial chunk per process
-constexpr int kInteractParts = (kClassifyTile / 64 + kNumProc + kInteractWaves - 1) / kInteractWaves;
+// blocks of a tile of `items` * kClassifyBlock slots: at most one chunk per 64
+// slots plus one partial chunk per process, kInteractWaves chunks per block
+constexpr int InteractParts(int items) { return (items * kClassifyBlock / 64 + kNumProc + kInteractWaves - 1) / kInteractWaves; }
+constexpr int kInteractParts = InteractParts(kClassifyItems);
 
-inline int InteractGridTiles(int tiles) { return (tiles + kXcds - 1) / kXcds * kXcds * kInteractParts; }
+inline int InteractGridTiles(int tiles, int parts = kInteractParts) { return (tiles + kXcds - 1) / kXcds * kXcds * parts; }
 inline int InteractGrid(int n) { return InteractGridTiles((n + kClassifyTile - 1) / kClassifyTile); }
 
 // the process layout of tile `tile` of a batch of n slots (codes, tile list
-// and the tile's info words of that batch)
+// and the tile's info words of that batch); a tile is ITEMS * kClassifyBlock
+// slots
+template <int ITEMS = kClassifyItems>
 __device__ void ClassifyTileLayout(int n, const uint8_t* __restrict__ code, uint16_t* __restrict__ tlist,
                                    uint32_t* __restrict__ info, int tile) {
+  static_assert(ITEMS * kClassifyBlock <= 65536, "tile-local offsets are 16 bits");
   __shared__ uint32_t sW[kNumProc][kClassifyWaves];
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int tile0 = tile * kClassifyTile;
+  const int tile0 = tile * ITEMS * kClassifyBlock;
   uint32_t cnt[kNumProc];
 #pragma unroll
   for (int q = 0; q < kNumProc; ++q) cnt[q] = 0u;
-  for (int it = 0; it < kClassifyItems; ++it) {
+  for (int it = 0; it < ITEMS; ++it) {
     const int i = tile0 + it * kClassifyBlock + threadIdx.x;
     const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
 #pragma unroll
@@ -729,7 +734,7 @@ __device__ void ClassifyTileLayout(int n, const uint8_t* __restrict__ code, uint
   uint32_t pos[kNumProc];
 #pragma unroll
   for (int q = 0; q < kNumProc; ++q) pos[q] = sW[q][wave];
-  for (int it = 0; it < kClassifyItems; ++it) {
+  for (int it = 0; it < ITEMS; ++it) {
     const int local = it * kClassifyBlock + threadIdx.x;
     const int i = tile0 + local;
     const int c = i < n ? (int)(code[i] & kCodeProc) : 0;
@@ -750,11 +755,11 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles(int n, const 
 // block -> (tile, chunk of this wave): XCD x takes tiles x, x + kXcds, ...,
 // each as kInteractParts consecutive blocks of its share; false past the last
 // tile
-__device__ inline bool TileChunk(uint32_t ntiles, uint32_t& tile, uint32_t& chunk) {
+template <int PARTS = kInteractParts> __device__ inline bool TileChunk(uint32_t ntiles, uint32_t& tile, uint32_t& chunk) {
   const uint32_t x = blockIdx.x % kXcds;
   const uint32_t inXcd = blockIdx.x / kXcds;
-  tile = (inXcd / kInteractParts) * kXcds + x;
-  chunk = (inXcd % kInteractParts) * kInteractWaves + (threadIdx.x >> 6);
+  tile = (inXcd / PARTS) * kXcds + x;
+  chunk = (inXcd % PARTS) * kInteractWaves + (threadIdx.x >> 6);
   return tile < ntiles;
 }
 
@@ -793,6 +798,16 @@ __global__ void __launch_bounds__(kBlock) k_interact_tiles(DevTables<R> T, uint6
 // Block sizes of the batched step kernel, selected at run time.
 constexpr int kNumShapes = 3;
 constexpr int ShapeBlock(int s) { return 64 << s; }
+// Process-sort tile sizes of the batched step, selected at run time:
+// TileSlots(t) = 1024 << t slots (1024 ... 8192), TileItems(t) per thread of
+// the classification block.
+constexpr int kNumTiles = 4;
+constexpr int TileItems(int t) { return 1 << t; }
+constexpr int TileSlots(int t) { return kClassifyBlock << t; }
+// Words of a plan's work counters after its per-batch counters: the next
+// chunk of the persistent step kernel, then the next work item of each XCD's
+// share of the persistent tile kernel.
+constexpr int kQueueWords = 1 + kXcds;
 
 // Device descriptor of one batch of a plan. A plan holds nbatch + 1 of them;
 // the last carries only the totals of the prefix fields.
@@ -800,7 +815,8 @@ struct BatchDev {
   hepem_tracks in, out;
   hepem_step_out res;
   uint32_t qoff;                 // first entry of the batch's queue
-  uint32_t tile0;                // first classify tile of the batch
+  uint32_t tile0;                // first particle-classify tile of the batch (kClassifyTile slots)
+  uint32_t ptile0[kNumTiles];    // first process-sort tile of the batch for tile size TileSlots(t)
   uint32_t chunk0[kNumShapes];   // first step chunk for block size ShapeBlock(s)
 };
 
@@ -824,50 +840,38 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_batched(const Batch
 }
 
 // One thread per queue entry of every batch: block c takes chunk c of the
-// chunk sequence (all batches' chunks of BS entries, in batch order). A
-// grid-stride form keeps the descriptor and table state live across
-// iterations and costs a third of the occupancy, so the hardware dispatcher
-// does the load balancing instead.
+// chunk sequence (all batches' chunks of BS entries, in batch order). The
+// batch index is uniform over the block: it (and the descriptor loads) stay
+// scalar.
 template <typename R, int S>
-__global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
-                                                                int nbatch, uint32_t nchunks, const uint32_t* __restrict__ queue,
-                                                                uint32_t* __restrict__ bcounters) {
+__device__ void StepChunk(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d, int nbatch, uint32_t c,
+                          const uint32_t* __restrict__ queue, uint32_t* __restrict__ bcounters) {
   constexpr int BS = ShapeBlock(S);
-  {
-    const uint32_t c = blockIdx.x;
-    if (c >= nchunks) return;
-    // the batch index is uniform over the block: keep it (and the descriptor
-    // loads) scalar
-    const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, c, [](const BatchDev& x) { return x.chunk0[S]; }));
-    const hepem_tracks in = d[b].in;
-    hepem_tracks out = d[b].out;
-    const hepem_step_out res = d[b].res;
-    const int n = in.n;
-    const int j = (int)(c - d[b].chunk0[S]) * BS + (int)threadIdx.x;
-    if (j >= n) return;
-    uint32_t* cnt = bcounters + 4 * b;
-    const int i = (int)queue[d[b].qoff + j];
-    if (RejectTrack(T, in, out, res, n, i, cnt)) return;
-    CopyIdentity(in, out, i);
-    if ((uint32_t)j < cnt[0]) StepElectron(T, seed, in, out, res, n, i);
-    else StepGamma(T, seed, in, out, res, n, i);
-  }
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, c, [](const BatchDev& x) { return x.chunk0[S]; }));
+  const hepem_tracks in = d[b].in;
+  hepem_tracks out = d[b].out;
+  const hepem_step_out res = d[b].res;
+  const int n = in.n;
+  const int j = (int)(c - d[b].chunk0[S]) * BS + (int)threadIdx.x;
+  if (j >= n) return;
+  uint32_t* cnt = bcounters + 4 * b;
+  const int i = (int)queue[d[b].qoff + j];
+  if (RejectTrack(T, in, out, res, n, i, cnt)) return;
+  CopyIdentity(in, out, i);
+  if ((uint32_t)j < cnt[0]) StepElectron(T, seed, in, out, res, n, i);
+  else StepGamma(T, seed, in, out, res, n, i);
 }
 
-// Two-stage form of k_step_batched: stage 1 over every batch (block c takes
-// chunk c, as k_step_batched), recording the process code and staged record of
-// slot i of batch b at global index qoff_b + i; k_classify_tiles_batched and
-// k_interact_tiles_batched then sort and sample them tile by tile, as the
-// single-batch step does (tiles numbered across the batches).
+// Stage 1 of the two-stage form over every batch: records the process code
+// and staged record of slot i of batch b at global index qoff_b + i;
+// k_classify_tiles_batched and k_interact_tiles_batched then sort and sample
+// them tile by tile, as the single-batch step does (tiles numbered across
+// the batches).
 template <typename R, int S>
-__global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables<R> T, uint64_t seed,
-                                                                       const BatchDev* __restrict__ d, int nbatch,
-                                                                       uint32_t nchunks, const uint32_t* __restrict__ queue,
-                                                                       uint32_t* __restrict__ bcounters,
-                                                                       uint8_t* __restrict__ code, Staged* __restrict__ st) {
+__device__ void StepSelectChunk(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d, int nbatch, uint32_t c,
+                                const uint32_t* __restrict__ queue, uint32_t* __restrict__ bcounters,
+                                uint8_t* __restrict__ code, Staged* __restrict__ st) {
   constexpr int BS = ShapeBlock(S);
-  const uint32_t c = blockIdx.x;
-  if (c >= nchunks) return;
   const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, c, [](const BatchDev& x) { return x.chunk0[S]; }));
   const hepem_tracks in = d[b].in;
   hepem_tracks out = d[b].out;
@@ -888,32 +892,116 @@ __global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables
                                : StepGammaSelect(T, seed, in, out, res, st + qoff, n, i);
 }
 
-// The tile-local process sort over all batches of a plan: tiles are numbered
-// across the batches (BatchDev::tile0), each batch's tile list and codes sit
-// at its queue offset.
+// Next work item of a persistent kernel: thread 0 takes it from *next and the
+// block reads it from LDS (the second barrier keeps thread 0 from overwriting
+// it before every thread has read it).
+__device__ inline uint32_t NextItem(uint32_t* next) {
+  __shared__ uint32_t sItem;
+  if (threadIdx.x == 0) sItem = atomicAdd(next, 1u);
+  __syncthreads();
+  const uint32_t c = sItem;
+  __syncthreads();
+  return c;
+}
+
+// Dispatched form: block c takes chunk c.
+template <typename R, int S>
+__global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
+                                                                int nbatch, uint32_t nchunks, const uint32_t* __restrict__ queue,
+                                                                uint32_t* __restrict__ bcounters) {
+  if (blockIdx.x < nchunks) StepChunk<R, S>(T, seed, d, nbatch, blockIdx.x, queue, bcounters);
+}
+
+// Persistent form: one resident round of blocks, each taking chunks from the
+// counter *next (zero at launch) until they run out. Nothing but the chunk
+// index is carried from one chunk to the next.
+template <typename R, int S>
+__global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched_q(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
+                                                                  int nbatch, uint32_t nchunks,
+                                                                  const uint32_t* __restrict__ queue,
+                                                                  uint32_t* __restrict__ bcounters, uint32_t* __restrict__ next) {
+  for (uint32_t c = NextItem(next); c < nchunks; c = NextItem(next)) StepChunk<R, S>(T, seed, d, nbatch, c, queue, bcounters);
+}
+
+template <typename R, int S>
+__global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables<R> T, uint64_t seed,
+                                                                       const BatchDev* __restrict__ d, int nbatch,
+                                                                       uint32_t nchunks, const uint32_t* __restrict__ queue,
+                                                                       uint32_t* __restrict__ bcounters,
+                                                                       uint8_t* __restrict__ code, Staged* __restrict__ st) {
+  if (blockIdx.x < nchunks) StepSelectChunk<R, S>(T, seed, d, nbatch, blockIdx.x, queue, bcounters, code, st);
+}
+
+template <typename R, int S>
+__global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched_q(DevTables<R> T, uint64_t seed,
+                                                                         const BatchDev* __restrict__ d, int nbatch,
+                                                                         uint32_t nchunks, const uint32_t* __restrict__ queue,
+                                                                         uint32_t* __restrict__ bcounters,
+                                                                         uint8_t* __restrict__ code, Staged* __restrict__ st,
+                                                                         uint32_t* __restrict__ next) {
+  for (uint32_t c = NextItem(next); c < nchunks; c = NextItem(next))
+    StepSelectChunk<R, S>(T, seed, d, nbatch, c, queue, bcounters, code, st);
+}
+
+// The tile-local process sort over all batches of a plan, in tiles of
+// TileSlots(TI) slots: tiles are numbered across the batches
+// (BatchDev::ptile0[TI]), each batch's tile list and codes sit at its queue
+// offset.
+template <int TI>
 __global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles_batched(const BatchDev* __restrict__ d, int nbatch,
                                                                            const uint8_t* __restrict__ code,
                                                                            uint16_t* __restrict__ tlist,
                                                                            uint32_t* __restrict__ tinfo) {
   const uint32_t t = blockIdx.x;
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.ptile0[TI]; }));
   const uint32_t qoff = d[b].qoff;
-  ClassifyTileLayout(d[b].in.n, code + qoff, tlist + qoff, tinfo + (size_t)t * kTileInfo, (int)(t - d[b].tile0));
+  ClassifyTileLayout<TileItems(TI)>(d[b].in.n, code + qoff, tlist + qoff, tinfo + (size_t)t * kTileInfo,
+                                    (int)(t - d[b].ptile0[TI]));
 }
 
-template <typename R>
+template <typename R, int TI>
+__device__ void InteractBatchedTile(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d, int nbatch,
+                                    uint32_t tile, uint32_t chunk, const uint8_t* __restrict__ code,
+                                    const Staged* __restrict__ st, const uint16_t* __restrict__ tlist,
+                                    const uint32_t* __restrict__ tinfo) {
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, tile, [](const BatchDev& x) { return x.ptile0[TI]; }));
+  hepem_tracks out = d[b].out;
+  const uint32_t qoff = d[b].qoff;
+  InteractTileChunk<R>(T, seed, out, d[b].res, out.n, code + qoff, st + qoff, tlist + qoff, tinfo + (size_t)tile * kTileInfo,
+                       (int)(tile - d[b].ptile0[TI]) * TileSlots(TI), chunk);
+}
+
+template <typename R, int TI>
 __global__ void __launch_bounds__(kBlock) k_interact_tiles_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
                                                                    int nbatch, uint32_t ntiles, const uint8_t* __restrict__ code,
                                                                    const Staged* __restrict__ st,
                                                                    const uint16_t* __restrict__ tlist,
                                                                    const uint32_t* __restrict__ tinfo) {
   uint32_t tile, chunk;
-  if (!TileChunk(ntiles, tile, chunk)) return;
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, tile, [](const BatchDev& x) { return x.tile0; }));
-  hepem_tracks out = d[b].out;
-  const uint32_t qoff = d[b].qoff;
-  InteractTileChunk<R>(T, seed, out, d[b].res, out.n, code + qoff, st + qoff, tlist + qoff, tinfo + (size_t)tile * kTileInfo,
-                       (int)(tile - d[b].tile0) * kClassifyTile, chunk);
+  if (!TileChunk<InteractParts(TileItems(TI))>(ntiles, tile, chunk)) return;
+  InteractBatchedTile<R, TI>(T, seed, d, nbatch, tile, chunk, code, st, tlist, tinfo);
+}
+
+// Persistent form of k_interact_tiles_batched: one resident round of blocks.
+// Block b works for XCD b % kXcds (blocks are dealt to the XCDs round-robin)
+// and takes that XCD's work items, tiles x, x + kXcds, ... in
+// InteractParts blocks' worth each, from its counter next[x] (zero at launch).
+template <typename R, int TI>
+__global__ void __launch_bounds__(kBlock) k_interact_tiles_batched_q(DevTables<R> T, uint64_t seed,
+                                                                     const BatchDev* __restrict__ d, int nbatch, uint32_t ntiles,
+                                                                     const uint8_t* __restrict__ code,
+                                                                     const Staged* __restrict__ st,
+                                                                     const uint16_t* __restrict__ tlist,
+                                                                     const uint32_t* __restrict__ tinfo,
+                                                                     uint32_t* __restrict__ next) {
+  constexpr uint32_t P = InteractParts(TileItems(TI));
+  const uint32_t x = blockIdx.x % kXcds;
+  for (uint32_t k = NextItem(next + x);; k = NextItem(next + x)) {
+    const uint32_t tile = (k / P) * kXcds + x;
+    if (tile >= ntiles) return;
+    const uint32_t chunk = (k % P) * kInteractWaves + (threadIdx.x >> 6);
+    InteractBatchedTile<R, TI>(T, seed, d, nbatch, tile, chunk, code, st, tlist, tinfo);
+  }
 }
 
 // the five config-1 quantities of query i at (mc, E, lnE): restricted dE/dx,

@@ -367,13 +367,34 @@ int  hepem_batch_plan_create(const hepem_ctx* ctx, const hepem_batch_desc* descs
                              hepem_batch_plan** out_plan);
 int  hepem_batch_plan_destroy(hepem_batch_plan* plan);
 int  hepem_batch_plan_run(const hepem_batch_plan* plan, uint64_t seed, void* hip_stream);
-/* Times each launch shape of the batched step (block size 64, 128 or 256
- * threads) on the plan's own batches and keeps the fastest. This runs the
- * step several times: batches stepped in place advance. Synchronises the
+/* Launch shape of a plan's step. Every shape gives the same results.
+ *   block       threads per block of the step kernel: 64, 128 or 256;
+ *   sort_tile   slots per tile of the process sort of the pending
+ *               interactions (1024, 2048, 4096 or 8192), or 0: no process
+ *               sort (one fused kernel);
+ *   persistent  0: one block per chunk of work, dispatched by the hardware;
+ *               1: one resident round of blocks taking chunks from a work
+ *               counter (reset by every run) until none are left;
+ *   grid        (read only) blocks of the step kernel. */
+typedef struct hepem_batch_shape {
+  int32_t block;
+  int32_t sort_tile;
+  int32_t persistent;
+  int32_t grid;
+} hepem_batch_shape;
+/* Times every launch shape the plan may take on the plan's own batches and
+ * keeps the fastest: each block size; with and without the process sort as
+ * the context's mode at plan creation allows (HEPEM_SORT_AUTO: both); each
+ * sort tile size; dispatched and persistent. This runs the step several
+ * times per shape: batches stepped in place advance. Synchronises the
  * stream. */
 int  hepem_batch_plan_tune(hepem_batch_plan* plan, uint64_t seed, void* hip_stream);
 /* Launch shape in use: block size and grid blocks of the step kernel. */
 int  hepem_batch_plan_shape(const hepem_batch_plan* plan, int32_t* block, int32_t* grid);
+/* The full launch shape in use, and setting one (grid is ignored on input;
+ * HEPEM_ERR_INVALID for a block, tile or persistent value not listed above). */
+int  hepem_batch_plan_get_shape(const hepem_batch_plan* plan, hepem_batch_shape* shape);
+int  hepem_batch_plan_set_shape(hepem_batch_plan* plan, const hepem_batch_shape* shape);
 /* Tracks of batch `b` the last run rejected (see hepem_step_rejected).
  * Synchronises the stream. */
 int  hepem_batch_plan_rejected(const hepem_batch_plan* plan, int32_t b, uint32_t* nrejected, void* hip_stream);

@@ -60,6 +60,10 @@ def test_null_arguments_are_rejected():
     assert L.hepem_batch_plan_run(None, 1, None) == -1
     assert L.hepem_batch_plan_tune(None, 1, None) == -1
     assert L.hepem_batch_plan_destroy(None) == -1
+    from g4hepem_amd._native import BatchShape
+    bs = BatchShape(256, 4096, 0, 0)
+    assert L.hepem_batch_plan_get_shape(None, C.byref(bs)) == -1
+    assert L.hepem_batch_plan_set_shape(None, C.byref(bs)) == -1
     assert L.hepem_ctx_set_lookup_staging(None, 1) == -1
     assert L.hepem_ctx_set_process_sort(None, 1) == -1
     assert L.hepem_status_string(-3) == b"capacity exceeded"

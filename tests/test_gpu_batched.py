@@ -80,13 +80,54 @@ def test_batched_tune_then_run(ctx64, dev):
     host, batches = make_batches(sizes, dev, seed0=950)
     plan = BatchPlan(ctx64, batches)
     block, grid = plan.tune(SEED)
-    _print("tuned shape", block, grid)
-    assert block in (64, 128, 256) and grid == sum((n + block - 1) // block for n in sizes)
+    shape = plan.get_shape()
+    _print("tuned shape", shape)
+    assert block == shape["block"] and grid == shape["grid"]
+    assert (block, shape["sort_tile"], shape["persistent"]) in BatchPlan.shapes()
+    if not shape["persistent"]:
+        assert grid == sum((n + block - 1) // block for n in sizes)
     for _, tout, res in batches:  # results must come from the run below
         res["edep"].fill_(-1.0)
     plan.run(SEED)
     torch.cuda.synchronize()
     check_against_unbatched(ctx64, host, batches, dev, "tuned")
+
+
+def test_batched_every_shape_bit_identical(ctx64, dev):
+    """Every launch shape the tuner may choose (block size, process sort on
+    and off, each sort tile size, dispatched and persistent) gives the
+    unbatched results bit for bit, on batches whose sizes straddle the tile
+    sizes."""
+    from g4hepem_amd import BatchPlan
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_outputs_differ
+    sizes = [1, 1023, 1024, 1025, 2049, 8191, 8193, 20000, 0, 64, 4096, 300]
+    host, batches = make_batches(sizes, dev, seed0=1200)
+    work = ctx64.work_buffer(max(sizes), dev)
+    refs = []
+    for tin, _, _ in batches:
+        t_ref, r_ref = TrackSoA(tin.n, "torch", dev), StepOutSoA(tin.n, "torch", dev)
+        ctx64.step(SEED, tin, t_ref, r_ref, work)
+        refs.append((t_ref, r_ref))
+    plan = BatchPlan(ctx64, batches)
+    shapes = BatchPlan.shapes()
+    assert len(shapes) == 30
+    for block, tile, pers in shapes:
+        got = plan.set_shape(block, tile, pers)
+        assert (got["block"], got["sort_tile"], got["persistent"]) == (block, tile, pers)
+        for _, tout, res in batches:   # the checked results must come from this run
+            res["winner"].fill_(-7)
+            tout["ekin"].fill_(-1.0)
+        plan.run(SEED)
+        torch.cuda.synchronize()
+        for k, ((tin, tout, res), (t_ref, r_ref)) in enumerate(zip(batches, refs)):
+            diff = step_outputs_differ(tin["ptype"], tout, res, t_ref, r_ref)
+            assert not diff, (block, tile, pers, k, diff)
+        assert plan.rejected(7) == 1
+    _print(f"{len(shapes)} shapes bit-identical")
+    with pytest.raises(RuntimeError):
+        plan.set_shape(96, 4096, 0)
+    with pytest.raises(RuntimeError):
+        plan.set_shape(256, 3000, 0)
 
 
 def test_batched_graph_replay(ctx64, dev):
