@@ -483,6 +483,7 @@ def main():
     ap.add_argument("--lookups-only", action="store_true",
                     help="config-1 lookups only, staged and from the table image (for profiling); prints their JSON object")
     ap.add_argument("--batched-only", action="store_true", help="config-3 batch plans only (tuner sweep included)")
+    ap.add_argument("--fp32-only", action="store_true", help="the config-2 step in the FP32-table form only (for profiling)")
     ap.add_argument("--shower-only", action="store_true",
                     help="config-5 slab showers only, without the CPU harness (for profiling); prints their JSON object")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -538,6 +539,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.fp32_only:
+        ms32, run32, _ = time_steps(Context(tables2, HEPEM_FP32T), tin, tout, res, work, args.steps, args.warmup, args.seed)
+        print(json.dumps({"fp32_table_form_step": {"tracks": n, "ms_per_step": ms32, "kernel_ms": run32,
+                                                   "value": n / (ms32 * 1e-3), "unit": "track-steps/s"}}), flush=True)
+        return
     log("headline FP64 step")
     ctx64 = Context(tables2, HEPEM_FP64)
     ms64, run64, wall64 = time_steps(ctx64, tin, tout, res, work, args.steps, args.warmup, args.seed)
