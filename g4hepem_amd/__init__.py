@@ -8,6 +8,6 @@ stepping physics behind a C-ABI (include/g4hepem_amd.h).
 """
 from ._native import (ELECTRON, GAMMA, HEPEM_FP32T, HEPEM_FP64, POSITRON, HepEmError, Tables,  # noqa: F401
                       lib)
-from .context import BatchPlan, Context  # noqa: F401
+from .context import BatchPlan, Context, TrackStack  # noqa: F401
 
-__all__ = ["Tables", "Context", "BatchPlan", "HepEmError", "lib", "HEPEM_FP64", "HEPEM_FP32T", "ELECTRON", "POSITRON", "GAMMA"]
+__all__ = ["Tables", "Context", "BatchPlan", "TrackStack", "HepEmError", "lib", "HEPEM_FP64", "HEPEM_FP32T", "ELECTRON", "POSITRON", "GAMMA"]

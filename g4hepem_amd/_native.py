@@ -161,6 +161,12 @@ def lib() -> C.CDLL:
     L.hepem_batch_plan_tune.argtypes = [vp, u64, vp]
     L.hepem_batch_plan_shape.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     L.hepem_batch_plan_get_shape.argtypes = [vp, C.POINTER(BatchShape)]
+    L.hepem_stack_create.argtypes = [vp, i32, C.POINTER(vp)]
+    L.hepem_stack_destroy.argtypes = [vp]
+    L.hepem_stack_tracks.argtypes = [vp, C.POINTER(Tracks)]
+    L.hepem_stack_push.argtypes = [vp, C.POINTER(Tracks), vp]
+    L.hepem_stack_copy_out.argtypes = [vp, C.POINTER(Tracks), vp]
+    L.hepem_stack_advance.argtypes = [vp, C.POINTER(Tracks), C.POINTER(StepOut), vp, vp, vp, vp, C.POINTER(i32), vp]
     L.hepem_batch_plan_set_shape.argtypes = [vp, C.POINTER(BatchShape)]
     L.hepem_batch_plan_rejected.argtypes = [vp, i32, C.POINTER(C.c_uint32), vp]
     L.hepem_tracks_reset_state.argtypes = [C.POINTER(Tracks), vp]
@@ -183,6 +189,8 @@ def lib() -> C.CDLL:
                  "hepem_electron_howfar", "hepem_electron_perform", "hepem_gamma_howfar", "hepem_gamma_perform",
                  "hepem_batch_plan_create", "hepem_batch_plan_destroy", "hepem_batch_plan_run", "hepem_batch_plan_tune",
                  "hepem_batch_plan_shape", "hepem_batch_plan_get_shape", "hepem_batch_plan_set_shape",
+                 "hepem_stack_create", "hepem_stack_destroy", "hepem_stack_tracks", "hepem_stack_push",
+                 "hepem_stack_copy_out", "hepem_stack_advance",
                  "hepem_batch_plan_rejected", "hepem_slab_run", "hepem_electron_step_begin",
                  "hepem_electron_substep_howfar", "hepem_electron_substep_along", "hepem_electron_substep_active",
                  "hepem_electron_step_end"):

@@ -50,7 +50,7 @@ def _run(cmd, cwd=None):
 
 
 def build_native() -> Path:
-    sources = [CSRC / "hepem_capi.hip", CSRC / "hepem_shower.hip", CSRC / "tables" / "hepem_tables.cpp",
+    sources = [CSRC / "hepem_capi.hip", CSRC / "hepem_shower.hip", CSRC / "hepem_stack.hip", CSRC / "tables" / "hepem_tables.cpp",
                CSRC / "tables" / "hepem_json.cpp", CSRC / "tables" / "hepem_validate.cpp"]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [_hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}", "-o", tmp, *sources]

@@ -256,3 +256,67 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+class _StackView:
+    """The current batch of a TrackStack, in the stack's device memory: n and
+    the hepem_tracks view (valid until the next push or advance)."""
+
+    def __init__(self, t):
+        self._t = t
+        self.n = t.n
+
+    def struct(self):
+        return self._t
+
+
+class TrackStack:
+    """hepem_stack: a device track stack for callers that step batches with
+    their own geometry (StackSecondaries for whole batches)."""
+
+    def __init__(self, ctx: Context, capacity: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        N.check("hepem_stack_create", N.lib().hepem_stack_create(ctx.h, int(capacity), C.byref(h)))
+        self.h = h
+
+    def push(self, tracks: TrackSoA, stream=None) -> None:
+        t = tracks.struct()
+        N.check("hepem_stack_push", N.lib().hepem_stack_push(self.h, C.byref(t), _stream_handle(stream)))
+
+    def view(self) -> _StackView:
+        t = N.Tracks()
+        N.check("hepem_stack_tracks", N.lib().hepem_stack_tracks(self.h, C.byref(t)))
+        return _StackView(t)
+
+    def __len__(self) -> int:
+        return self.view().n
+
+    def copy_out(self, dst: TrackSoA, stream=None) -> TrackSoA:
+        t = dst.struct()
+        N.check("hepem_stack_copy_out", N.lib().hepem_stack_copy_out(self.h, C.byref(t), _stream_handle(stream)))
+        return dst
+
+    def advance(self, out, res: StepOutSoA, keep=None, cut_edep=None, parent=None, origin=None, stream=None) -> int:
+        """The next batch from a step of the current one (`out`: its
+        post-step state, the view itself when stepped in place); keep,
+        cut_edep, parent, origin: optional device arrays (uint8, float64,
+        int32, int8). Returns the next batch's count."""
+        t, r = out.struct(), res.struct()
+        ptr = lambda a: a.data_ptr() if a is not None else None
+        m = C.c_int32()
+        N.check("hepem_stack_advance", N.lib().hepem_stack_advance(self.h, C.byref(t), C.byref(r), ptr(keep), ptr(cut_edep),
+                                                                    ptr(parent), ptr(origin), C.byref(m),
+                                                                    _stream_handle(stream)))
+        return m.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().hepem_stack_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

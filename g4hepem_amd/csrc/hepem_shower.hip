@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "device/dev_physics.hh"
+#include "device/dev_scan.hh"
 #include "device/dev_soa.hh"
 #include "device/dev_tracking.hh"
 #include "g4hepem_amd_shower.h"
@@ -166,47 +167,6 @@ __global__ void k_sh_inject(Stack s, Slabs g, int ptype, double ekin, int e0, in
 // isotropic safety of a point in slab l: distance to the nearer plane
 __device__ inline double Safety(const Slabs& g, int l, double x) {
   return fmax(0.0, fmin(x - g.plane[l], g.plane[l + 1] - x));
-}
-
-// Exclusive scans of a and b over the block (blockDim.x == NT) into xa, xb and
-// the block sums of a, b and w. Every thread of the block must call it.
-template <int NT>
-__device__ inline void BlockScan2(uint32_t a, uint32_t b, uint32_t w, uint32_t& xa, uint32_t& xb, uint32_t& sa, uint32_t& sb,
-                                  uint32_t& sw) {
-  constexpr int kW = NT / 64;
-  __shared__ uint32_t sh[3][kW];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  uint32_t x = a, y = b, z = w;
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t ux = __shfl_up(x, o, 64), uy = __shfl_up(y, o, 64);
-    if (lane >= o) {
-      x += ux;
-      y += uy;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o, 64);
-  if (lane == 63) {
-    sh[0][wv] = x;
-    sh[1][wv] = y;
-  }
-  if (lane == 0) sh[2][wv] = z;
-  __syncthreads();
-  uint32_t ba = 0, bb = 0, ta = 0, tb = 0, tw = 0;
-  for (int k = 0; k < kW; ++k) {
-    const uint32_t u = sh[0][k], v = sh[1][k];
-    if (k < wv) {
-      ba += u;
-      bb += v;
-    }
-    ta += u;
-    tb += v;
-    tw += sh[2][k];
-  }
-  xa = ba + x - a;
-  xb = bb + y - b;
-  sa = ta;
-  sb = tb;
-  sw = tw;
 }
 
 // Cuts the proposed geometrical step at the next plane along the pre-step
@@ -624,7 +584,7 @@ __global__ void __launch_bounds__(kB) k_sh_post(Stack s, Slabs g, hepem_step_out
     if (k != kNone) atomicAdd(&dep[k], hval[e]);
   }
   uint32_t xE, xG, tE, tG, tns;
-  BlockScan2<kB>(cE, cG, ns | (ended << 16), xE, xG, tE, tG, tns);
+  hepem::BlockScan2<kB>(cE, cG, ns | (ended << 16), xE, xG, tE, tG, tns);
   if (valid) {
     offE[i] = xE;
     offG[i] = xG;
@@ -664,7 +624,7 @@ __global__ void __launch_bounds__(kScanB) k_scan_sums(uint32_t* bsumE, uint32_t*
     const uint32_t w = packed & 0xFFFFu;
     myEnded += packed >> 16;
     uint32_t xa, xb, ta, tb, tw;
-    BlockScan2<kScanB>(a, b, w, xa, xb, ta, tb, tw);
+    hepem::BlockScan2<kScanB>(a, b, w, xa, xb, ta, tb, tw);
     if (i < nb) {
       bsumE[i] = carryE + xa;
       bsumG[i] = carryG + xb;

@@ -404,6 +404,49 @@ int  hepem_batch_plan_rejected(const hepem_batch_plan* plan, int32_t b, uint32_t
 int  hepem_step_batched(const hepem_ctx* ctx, uint64_t seed, const hepem_batch_desc* descs,
                         int32_t nbatch, void* hip_stream);
 
+/* ---- a device track stack -------------------------------------------------
+ * For callers that step batches with their own geometry. The current batch
+ * lives in the stack's device memory: hepem_stack_tracks gives the view to
+ * step, in place or into another SoA. hepem_stack_advance then builds the
+ * next batch from that step's output, as G4HepEmTrackingManager::
+ * StackSecondaries does for one track
+ * (G4HepEm/G4HepEm/src/G4HepEmTrackingManager.cc:1254-1330):
+ *   - a track stays if keep[i] != 0 (keep NULL: every track; the caller's
+ *     geometry clears it for a track that left the world) and its kinetic
+ *     energy is above zero;
+ *   - each secondary of its step (res: nsec, sec_*) is stacked, unless the
+ *     region of the parent's couple (out->imc[i]) applies production cuts and
+ *     the secondary is an e- below the e- threshold, an e+ below the e+
+ *     threshold while the gamma threshold exceeds the electron mass, or a
+ *     gamma below the gamma threshold; such a secondary is deposited instead
+ *     (an e+ with its annihilation energy 2 m_e c^2), into cut_edep[i]
+ *     (optional, out->n entries);
+ *   - the next batch holds the e-/e+ first, then the gammas, each in parent
+ *     order: a track before its stacked secondaries, those in slot order.
+ *     parent[j] and origin[j] (optional, capacity entries each) give the slot
+ *     of the step's output that slot j came from and whether it is that track
+ *     (0) or its secondary slot 0 or 1 (1, 2), so the caller can carry its own
+ *     per-track data (position, touchable) along;
+ *   - a stacked secondary starts at its parent's post-step point (the
+ *     parent's couple, safety and boundary flag) with the start-of-track
+ *     state of hepem_tracks_reset_state and a Philox stream of its own,
+ *     derived from the parent's stream, its post-step draw counter and the
+ *     slot.
+ * `out` may be the view of the current batch but not the stack's other
+ * buffer. Advancing synchronises the stream to read the next count
+ * (*n_next); HEPEM_ERR_CAPACITY (next count above the capacity) leaves the
+ * current batch as it was. hepem_stack_push appends a caller's tracks to the
+ * current batch (device arrays, copied; rng_stream may be NULL). */
+typedef struct hepem_stack hepem_stack;
+int  hepem_stack_create(const hepem_ctx* ctx, int32_t capacity, hepem_stack** out_stack);
+int  hepem_stack_destroy(hepem_stack* stack);
+int  hepem_stack_tracks(const hepem_stack* stack, hepem_tracks* view);
+int  hepem_stack_push(hepem_stack* stack, const hepem_tracks* tracks, void* hip_stream);
+/* Copies the current batch into the caller's SoA (dst->n = the batch's count). */
+int  hepem_stack_copy_out(const hepem_stack* stack, hepem_tracks* dst, void* hip_stream);
+int  hepem_stack_advance(hepem_stack* stack, const hepem_tracks* out, const hepem_step_out* res, const uint8_t* keep,
+                         double* cut_edep, int32_t* parent, int8_t* origin, int32_t* n_next, void* hip_stream);
+
 /* ---- track-state helpers -------------------------------------------------
  * Reset the persistent physics state of n tracks to the reference's start-of-
  * track values (G4HepEmTrack::ReSet, G4HepEmMSCTrackData::ReSet,

@@ -64,6 +64,14 @@ def test_null_arguments_are_rejected():
     bs = BatchShape(256, 4096, 0, 0)
     assert L.hepem_batch_plan_get_shape(None, C.byref(bs)) == -1
     assert L.hepem_batch_plan_set_shape(None, C.byref(bs)) == -1
+    assert L.hepem_stack_create(None, 10, C.byref(h)) == -1
+    assert L.hepem_stack_create(None, 10, None) == -1
+    assert L.hepem_stack_destroy(None) == -1
+    assert L.hepem_stack_tracks(None, C.byref(t)) == -1
+    assert L.hepem_stack_push(None, C.byref(t), None) == -1
+    assert L.hepem_stack_copy_out(None, C.byref(t), None) == -1
+    m = C.c_int32()
+    assert L.hepem_stack_advance(None, C.byref(t), C.byref(r), None, None, None, None, C.byref(m), None) == -1
     assert L.hepem_ctx_set_lookup_staging(None, 1) == -1
     assert L.hepem_ctx_set_process_sort(None, 1) == -1
     assert L.hepem_status_string(-3) == b"capacity exceeded"
