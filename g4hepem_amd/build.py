@@ -59,19 +59,23 @@ def build_native() -> Path:
     return LIB
 
 
-CPP_TEST_SRC = ROOT / "tests" / "cpp" / "test_managers.cpp"
-CPP_TEST_BIN = ROOT / "tests" / "cpp" / "test_managers"
+CPP_TEST_DIR = ROOT / "tests" / "cpp"
+# the C++ host-layer test programs: g4hepem_amd.hpp (batches) and
+# g4hepem_amd_onetrack.hpp (the one-track drop-in)
+CPP_TESTS = ("test_managers", "test_onetrack")
+CPP_TEST_BIN = CPP_TEST_DIR / "test_managers"
 
 
 def build_cpp_tests() -> Path:
-    """The C++ host-layer test program (include/g4hepem_amd.hpp), linked
-    against the in-tree library."""
-    tmp = CPP_TEST_BIN.with_suffix(".tmp")
-    cmd = [_hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-           "-packed-fp32-ops", f"-I{ROOT / 'include'}", "-o", tmp, CPP_TEST_SRC, f"-L{PKG}", "-lg4hepem_amd",
-           "-Wl,-rpath,$ORIGIN/../../g4hepem_amd"]
-    _run(cmd)
-    os.replace(tmp, CPP_TEST_BIN)
+    """The C++ host-layer test programs, linked against the in-tree library."""
+    for name in CPP_TESTS:
+        out = CPP_TEST_DIR / name
+        tmp = out.with_suffix(".tmp")
+        cmd = [_hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", f"-I{ROOT / 'include'}", "-o", tmp, CPP_TEST_DIR / f"{name}.cpp", f"-L{PKG}",
+               "-lg4hepem_amd", "-Wl,-rpath,$ORIGIN/../../g4hepem_amd"]
+        _run(cmd)
+        os.replace(tmp, out)
     return CPP_TEST_BIN
 
 
