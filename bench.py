@@ -571,7 +571,7 @@ def main():
             "tracks": n, "wall_s": round(wall64, 4),
         },
         "roofline": {
-            "bound": "hbm", "kernel": "hepem_step_run = k_step_select<double> + k_classify_tiles + k_interact_tiles<double>",
+            "bound": "hbm", "kernel": "hepem_step_run = k_step_select_local<double> + k_classify_tiles + k_interact_tiles<double>",
             "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel_ms": run64, "algorithmic_bytes_per_launch": abytes,

@@ -606,7 +606,12 @@ bool SortsProcesses(const hepem_ctx* c, int64_t n) {
 }
 }  // namespace
 
-extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
+namespace {
+// The particle-type queue of a batch in `work`: over the whole batch (e-/e+
+// from the front, gammas from the back; counters[0..1]), or, for the two-stage
+// step, tile by tile (k_classify_local: each tile's e-/e+ first, then its
+// gammas; the tile's e-/e+ count in its first tinfo word).
+int PrepareQueue(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream, bool tileLocal) {
   if (!c || !TracksOk(in) || !work) return HEPEM_ERR_INVALID;
   const int n = in->n;
   const hipStream_t s = static_cast<hipStream_t>(stream);
@@ -614,10 +619,19 @@ extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, vo
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, w.counters, 64, 0u);
   if (n > 0) {
     const int grid = (n + kClassifyTile - 1) / kClassifyTile;
-    hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kClassifyBlock), 0, s, n, in->ptype, w.queue, w.counters);
+    if (tileLocal)
+      hipLaunchKernelGGL(k_classify_local, dim3(grid), dim3(kClassifyBlock), 0, s, n, in->ptype, w.queue, w.tinfo);
+    else
+      hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kClassifyBlock), 0, s, n, in->ptype, w.queue, w.counters);
   }
   HIP_OK(hipGetLastError());
   return HEPEM_OK;
+}
+}  // namespace
+
+extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
+  if (!c || !TracksOk(in)) return HEPEM_ERR_INVALID;
+  return PrepareQueue(c, in, work, stream, SortsProcesses(c, in->n));
 }
 
 extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
@@ -638,14 +652,14 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
     const int igrid = InteractGrid(n);
     if (fp64) {
-      hipLaunchKernelGGL(k_step_select<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue,
-                         w.counters, w.code, w.st);
+      hipLaunchKernelGGL(k_step_select_local<double>, dim3(LocalSelectGrid(n)), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue,
+                         w.counters, w.tinfo, w.code, w.st);
       hipLaunchKernelGGL(k_classify_tiles, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.tlist, w.tinfo);
       hipLaunchKernelGGL(k_interact_tiles<double>, dim3(igrid), dim3(kBlock), 0, s, c->t64, seed, *out, *res, w.code, w.st,
                          w.tlist, w.tinfo);
     } else {
-      hipLaunchKernelGGL(k_step_select<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue,
-                         w.counters, w.code, w.st);
+      hipLaunchKernelGGL(k_step_select_local<float>, dim3(LocalSelectGrid(n)), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue,
+                         w.counters, w.tinfo, w.code, w.st);
       hipLaunchKernelGGL(k_classify_tiles, dim3(tiles), dim3(kClassifyBlock), 0, s, n, w.code, w.tlist, w.tinfo);
       hipLaunchKernelGGL(k_interact_tiles<float>, dim3(igrid), dim3(kBlock), 0, s, c->t32, seed, *out, *res, w.code, w.st,
                          w.tlist, w.tinfo);
@@ -721,7 +735,7 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
   const uint32_t* queue = KIND == kMixed ? wl.queue : nullptr;
   if (!perform) {
     if (KIND == kMixed) {
-      rc = hepem_step_prepare(c, t, work, stream);
+      rc = PrepareQueue(c, t, work, stream, false);
       if (rc != HEPEM_OK) return rc;
     } else {
       hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters, 64, 0u);

@@ -688,6 +688,88 @@ constexpr int kInteractParts = InteractParts(kClassifyItems);
 inline int InteractGridTiles(int tiles, int parts = kInteractParts) { return (tiles + kXcds - 1) / kXcds * kXcds * parts; }
 inline int InteractGrid(int n) { return InteractGridTiles((n + kClassifyTile - 1) / kClassifyTile); }
 
+// Tile-local particle partition: each tile of kClassifyTile slots lists its
+// e-/e+ slots first, then its gammas, each in slot order, at the tile's own
+// range of the queue; tinfo[tile * kTileInfo] = the tile's e-/e+ count.
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const int8_t* __restrict__ ptype,
+                                                                   uint32_t* __restrict__ queue, uint32_t* __restrict__ tinfo) {
+  __shared__ uint32_t sE[kClassifyWaves], sG[kClassifyWaves], sTot;
+  const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int tile0 = blockIdx.x * kClassifyTile;
+  uint32_t cE = 0, cG = 0;
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
+    const bool valid = i < n;
+    const int8_t pt = valid ? ptype[i] : (int8_t)-1;
+    const bool isE = valid && (pt == 0 || pt == 1);
+    cE += (uint32_t)__popcll(__ballot(isE));
+    cG += (uint32_t)__popcll(__ballot(valid && !isE));
+  }
+  if (lane == 0) {
+    sE[wave] = cE;
+    sG[wave] = cG;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tE = 0, tG = 0;
+    for (int w = 0; w < kClassifyWaves; ++w) {
+      const uint32_t e = sE[w], g = sG[w];
+      sE[w] = tE;
+      sG[w] = tG;
+      tE += e;
+      tG += g;
+    }
+    sTot = tE;
+    tinfo[(size_t)blockIdx.x * kTileInfo] = tE;
+  }
+  __syncthreads();
+  uint32_t pE = sE[wave], pG = sTot + sG[wave];
+  for (int it = 0; it < kClassifyItems; ++it) {
+    const int i = tile0 + it * kClassifyBlock + threadIdx.x;
+    const bool valid = i < n;
+    const int8_t pt = valid ? ptype[i] : (int8_t)-1;
+    const bool isE = valid && (pt == 0 || pt == 1);
+    const bool isG = valid && !isE;
+    const uint64_t mE = __ballot(isE), mG = __ballot(isG);
+    if (isE) queue[tile0 + pE + __popcll(mE & below)] = (uint32_t)i;
+    if (isG) queue[tile0 + pG + __popcll(mG & below)] = (uint32_t)i;
+    pE += (uint32_t)__popcll(mE);
+    pG += (uint32_t)__popcll(mG);
+  }
+}
+
+constexpr int kTileChunks = kClassifyTile / kBlock;
+inline int LocalSelectGrid(int n) { return ((n + kClassifyTile - 1) / kClassifyTile + kXcds - 1) / kXcds * kXcds * kTileChunks; }
+
+// k_step_select over the tile-local partition: XCD x takes tiles x, x + kXcds,
+// ..., each as kTileChunks consecutive blocks of its share, so a tile's e-/e+
+// and gamma blocks run on one XCD close together and its lines are read and
+// written through one L2.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_step_select_local(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
+                                                        hepem_step_out res, const uint32_t* __restrict__ queue,
+                                                        uint32_t* __restrict__ counters, const uint32_t* __restrict__ tinfo,
+                                                        uint8_t* __restrict__ code, Staged* __restrict__ st) {
+  const int n = in.n;
+  const int x = blockIdx.x % kXcds, inXcd = blockIdx.x / kXcds;
+  const int tile = (inXcd / kTileChunks) * kXcds + x;
+  const int tile0 = tile * kClassifyTile;
+  if (tile0 >= n) return;
+  const int m = min(kClassifyTile, n - tile0);
+  const int jl = (inXcd % kTileChunks) * kBlock + (int)threadIdx.x;
+  if (jl >= m) return;
+  const int nEt = (int)tinfo[(size_t)tile * kTileInfo];
+  const int i = (int)queue[tile0 + jl];
+  if (RejectTrack(T, in, out, res, n, i, counters)) {
+    code[i] = kPNone;
+    return;
+  }
+  CopyIdentity(in, out, i);
+  code[i] = jl < nEt ? StepElectronSelect(T, seed, in, out, res, st, n, i) : StepGammaSelect(T, seed, in, out, res, st, n, i);
+}
+
 // the process layout of tile `tile` of a batch of n slots (codes, tile list
 // and the tile's info words of that batch); a tile is ITEMS * kClassifyBlock
 // slots
