@@ -741,24 +741,44 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const 
 }
 
 constexpr int kTileChunks = kClassifyTile / kBlock;
+// tiles at the end of each XCD's share whose second halves (mostly gammas,
+// short) run after all their first halves (mostly e-/e+, long)
+constexpr int kTailTiles = 8;
+static_assert(kTileChunks % 2 == 0, "a tile splits into two halves of whole blocks");
 inline int LocalSelectGrid(int n) { return ((n + kClassifyTile - 1) / kClassifyTile + kXcds - 1) / kXcds * kXcds * kTileChunks; }
 
 // k_step_select over the tile-local partition: XCD x takes tiles x, x + kXcds,
 // ..., each as kTileChunks consecutive blocks of its share, so a tile's e-/e+
 // and gamma blocks run on one XCD close together and its lines are read and
-// written through one L2.
+// written through one L2. In the last kTailTiles tiles of a share the blocks
+// of the first halves come first, then those of the second halves, so the
+// launch ends on short gamma blocks rather than on long e-/e+ ones.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_step_select_local(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
                                                         hepem_step_out res, const uint32_t* __restrict__ queue,
                                                         uint32_t* __restrict__ counters, const uint32_t* __restrict__ tinfo,
                                                         uint8_t* __restrict__ code, Staged* __restrict__ st) {
   const int n = in.n;
-  const int x = blockIdx.x % kXcds, inXcd = blockIdx.x / kXcds;
-  const int tile = (inXcd / kTileChunks) * kXcds + x;
+  const int x = blockIdx.x % kXcds, q = blockIdx.x / kXcds;
+  const int ntiles = (n + kClassifyTile - 1) / kClassifyTile;
+  const int share = ntiles > x ? (ntiles - x + kXcds - 1) / kXcds : 0;
+  const int tail = min(kTailTiles, share), body = (share - tail) * kTileChunks;
+  int k, chunk;
+  if (q < body) {
+    k = q / kTileChunks;
+    chunk = q % kTileChunks;
+  } else {
+    const int r = q - body, h = kTileChunks / 2;
+    if (r >= tail * kTileChunks) return;
+    const bool first = r < tail * h;
+    const int r2 = first ? r : r - tail * h;
+    k = share - tail + r2 / h;
+    chunk = (first ? 0 : h) + r2 % h;
+  }
+  const int tile = k * kXcds + x;
   const int tile0 = tile * kClassifyTile;
-  if (tile0 >= n) return;
   const int m = min(kClassifyTile, n - tile0);
-  const int jl = (inXcd % kTileChunks) * kBlock + (int)threadIdx.x;
+  const int jl = chunk * kBlock + (int)threadIdx.x;
   if (jl >= m) return;
   const int nEt = (int)tinfo[(size_t)tile * kTileInfo];
   const int i = (int)queue[tile0 + jl];
