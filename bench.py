@@ -497,7 +497,7 @@ def main():
 
     import torch
     from g4hepem_amd import HEPEM_FP32T, HEPEM_FP64, Context, Tables
-    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_tracks
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_tracks, type_ordered
 
     assert torch.cuda.is_available(), "no GPU visible"
     dev = torch.device("cuda:0")
@@ -587,6 +587,14 @@ def main():
                                    "errors_vs_fp64": fp32_errors(r64, t64, res.to_numpy(), tout.to_numpy(),
                                                                  tracks["ptype"])}
         del r64, t64
+        log("config-2 tracks in type order")
+        tin_o = TrackSoA.from_numpy(type_ordered(tracks), "torch", dev)
+        ordered = {"layout": "the same tracks, e-/e+ first then gammas (as hepem_stack_advance lays out a batch)"}
+        for form, ctx in (("fp64", ctx64), ("fp32_table_form", ctx32)):
+            mso, runo, _ = time_steps(ctx, tin_o, tout, res, work, args.steps, args.warmup, args.seed)
+            ordered[form] = {"value": n / (mso * 1e-3), "unit": "track-steps/s", "ms_per_step": mso, "kernel_ms": runo}
+        line["config2_type_ordered"] = ordered
+        del tin_o
         log("config-1 lookups")
         tables1 = Tables(SET1, 0.7)
         ctx1 = Context(tables1, HEPEM_FP64)

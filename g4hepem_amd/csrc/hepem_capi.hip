@@ -599,10 +599,13 @@ extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
 }
 
 namespace {
-// whether a step or Perform over n slots sorts its pending interactions by process
-bool SortsProcesses(const hepem_ctx* c, int64_t n) {
+// whether a step, batch plan or split Perform (`perform`) over n slots sorts
+// its pending interactions by process; HEPEM_SORT_AUTO: the faster choice as
+// measured for each (the FP32 split Perform, over the global particle queue,
+// runs fused from HEPEM_SORT_AUTO_FP32_MAX_TRACKS slots)
+bool SortsProcesses(const hepem_ctx* c, int64_t n, bool perform = false) {
   if (c->processSort != HEPEM_SORT_AUTO) return c->processSort == HEPEM_SORT_ON;
-  return c->precision == HEPEM_FP64 || n < HEPEM_SORT_AUTO_FP32_MAX_TRACKS;
+  return c->precision == HEPEM_FP64 || !perform || n < HEPEM_SORT_AUTO_FP32_MAX_TRACKS;
 }
 }  // namespace
 
@@ -747,7 +750,7 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
     hipLaunchKernelGGL((k_howfar<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, queue, counters);
   } else if (!perform) {
     hipLaunchKernelGGL((k_howfar<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, queue, counters);
-  } else if (SortsProcesses(c, n)) {
+  } else if (SortsProcesses(c, n, true)) {
     // choice of the interaction, then the tile-local process sort and final
     // states (as hepem_step_run)
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;

@@ -185,3 +185,12 @@ def step_tracks(n: int, seed: int, ncouples: int, lnemin=math.log(0.1), lnemax=m
     t["on_boundary"][:] = 0
     reset_state_numpy(t, n)
     return t
+
+
+def type_ordered(t: dict) -> dict:
+    """The same tracks with the e-/e+ first and the gammas after them, each in
+    their original order: the layout a stack advance (hepem_stack_advance)
+    hands to the next step."""
+    n = len(t["ekin"])
+    perm = np.argsort(t["ptype"] == GAMMA, kind="stable")
+    return {k: np.ascontiguousarray(t[k].reshape(c, n)[:, perm]).reshape(-1) for k, (_, c) in TRACK_FIELDS.items()}

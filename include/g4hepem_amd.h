@@ -240,8 +240,9 @@ int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
 /* hepem_step_run, batch plans and the split Perform calls (hepem_perform,
  * hepem_*_perform) in one fused kernel (HEPEM_SORT_OFF), with the process sort
  * (HEPEM_SORT_ON), or with the faster of the two for the call as measured on
- * MI355X (HEPEM_SORT_AUTO, the default): the HEPEM_FP64 form sorts at every
- * size, the HEPEM_FP32T form sorts calls of fewer than
+ * MI355X (HEPEM_SORT_AUTO, the default): steps and batch plans sort at every
+ * size in both forms; the split Perform calls sort at every size in the
+ * HEPEM_FP64 form and, in the HEPEM_FP32T form, below
  * HEPEM_SORT_AUTO_FP32_MAX_TRACKS slots. Results are identical; only the
  * speed differs. */
 #define HEPEM_SORT_OFF  0

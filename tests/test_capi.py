@@ -102,3 +102,21 @@ def test_oracle_library_is_separate():
     from g4hepem_amd._native import LIB_PATH
     data = LIB_PATH.read_bytes()
     assert b"oracle_step" not in data and b"ref_step" not in data
+
+
+def test_type_ordered_batch():
+    """type_ordered keeps every track whole and puts the e-/e+ before the
+    gammas, each kind in its original order (the layout of a stack advance)."""
+    import numpy as np
+    from g4hepem_amd.tracks import TRACK_FIELDS, step_tracks, type_ordered
+    n = 1003
+    t = step_tracks(n, 5, 4)
+    o = type_ordered(t)
+    ng = int((t["ptype"] == 2).sum())
+    assert (o["ptype"][: n - ng] != 2).all() and (o["ptype"][n - ng:] == 2).all()
+    ids = o["id"]
+    assert (np.diff(ids[: n - ng]) > 0).all() and (np.diff(ids[n - ng:]) > 0).all()
+    pos = {int(v): i for i, v in enumerate(t["id"])}
+    src = np.array([pos[int(v)] for v in ids])
+    for k, (_, c) in TRACK_FIELDS.items():
+        assert np.array_equal(o[k].reshape(c, n), t[k].reshape(c, n)[:, src]), k

@@ -168,8 +168,7 @@ def test_batched_graph_replay(ctx64, dev):
                          ids=["two-stage", "fused"])
 def test_batched_equals_unbatched_fp32(tables2, dev, sizes):
     """The FP32-table form of a batch plan (the automatic sort choice takes the
-    two-stage kernels below 65536 tracks in all and the fused kernel above) is
-    bit-identical to hepem_step on each batch."""
+    two-stage kernels) is bit-identical to hepem_step on each batch."""
     from g4hepem_amd import HEPEM_FP32T, BatchPlan, Context
     ctx32 = Context(tables2, HEPEM_FP32T)
     host, batches = make_batches(sizes, dev, seed0=990)
