@@ -619,12 +619,13 @@ int PrepareQueue(const hepem_ctx* c, const hepem_tracks* in, void* work, void* s
   const int n = in->n;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const WorkLayout w(work, n);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, w.counters, 64, 0u);
-  if (n > 0) {
-    const int grid = (n + kClassifyTile - 1) / kClassifyTile;
-    if (tileLocal)
-      hipLaunchKernelGGL(k_classify_local, dim3(grid), dim3(kClassifyBlock), 0, s, n, in->ptype, w.queue, w.tinfo);
-    else
+  const int grid = (n + kClassifyTile - 1) / kClassifyTile;
+  if (tileLocal && n > 0) {
+    // clears the counters itself
+    hipLaunchKernelGGL(k_classify_local, dim3(grid), dim3(kClassifyBlock), 0, s, n, in->ptype, w.queue, w.tinfo, w.counters);
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kCounterWords), 0, s, w.counters, kCounterWords, 0u);
+    if (n > 0)
       hipLaunchKernelGGL(k_classify, dim3(grid), dim3(kClassifyBlock), 0, s, n, in->ptype, w.queue, w.counters);
   }
   HIP_OK(hipGetLastError());
@@ -708,8 +709,9 @@ extern "C" int hepem_step_noalloc(const hepem_ctx* c, uint64_t seed, const hepem
   if (in->n > HEPEM_NOALLOC_MAX_TRACKS) return HEPEM_ERR_CAPACITY;
   if (in->n == 0) return HEPEM_OK;
   if (c->scratchStatus != HEPEM_OK) return c->scratchStatus;
-  // the same kernels as hepem_step; prepare clears the counters with a fill
-  // kernel, so the scratch's previous contents do not matter
+  // the same kernels as hepem_step; prepare clears the counters (in the
+  // particle classification, or with a fill kernel), so the scratch's previous
+  // contents do not matter
   const int rc = hepem_step_prepare(c, in, c->scratch, stream);
   if (rc != HEPEM_OK) return rc;
   return hepem_step_run(c, seed, in, out, res, c->scratch, stream);

@@ -26,6 +26,7 @@ constexpr int kWaves = kBlock / 64;
 
 // counters[0]: e-/e+ tracks, counters[1]: gammas, counters[2]: rejected tracks
 // (counters[3..63] reserved)
+constexpr int kCounterWords = 64;
 __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -690,10 +691,14 @@ inline int InteractGrid(int n) { return InteractGridTiles((n + kClassifyTile - 1
 
 // Tile-local particle partition: each tile of kClassifyTile slots lists its
 // e-/e+ slots first, then its gammas, each in slot order, at the tile's own
-// range of the queue; tinfo[tile * kTileInfo] = the tile's e-/e+ count.
+// range of the queue; tinfo[tile * kTileInfo] = the tile's e-/e+ count. The
+// first block also clears the step's counters (no counter is read before the
+// next kernel).
 __global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const int8_t* __restrict__ ptype,
-                                                                   uint32_t* __restrict__ queue, uint32_t* __restrict__ tinfo) {
+                                                                   uint32_t* __restrict__ queue, uint32_t* __restrict__ tinfo,
+                                                                   uint32_t* __restrict__ counters) {
   __shared__ uint32_t sE[kClassifyWaves], sG[kClassifyWaves], sTot;
+  if (blockIdx.x == 0 && threadIdx.x < kCounterWords) counters[threadIdx.x] = 0u;
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));

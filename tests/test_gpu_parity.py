@@ -267,6 +267,39 @@ def test_process_sort_equals_fused(tables2, dev, form):
         assert same_bits(a_t[k][m], b_t[k][m]), k
 
 
+@pytest.mark.parametrize("mix", ["4:1:5", "shuffled", "e-/e+ only", "gamma only"])
+def test_tile_local_select_covers_every_slot(tables2, dev, mix):
+    """The tile-local select stage maps blocks to (tile, chunk) per XCD, with
+    the last 8 tiles of each XCD's share split into e-/e+ and gamma halves:
+    batches smaller than one tile, ragged last tiles, XCDs without tiles,
+    shares shorter and longer than the tail, and tiles whose e-/e+ count is
+    not a multiple of 64 (a shuffled layout) all step bit for bit like the
+    fused kernel."""
+    from g4hepem_amd import HEPEM_FP64, Context
+    from g4hepem_amd.tracks import step_tracks
+    ctx = Context(tables2, HEPEM_FP64)
+    for n in (1, 63, 4095, 4097, 8 * 4096 + 1, 40000, 200000, 300001):
+        _print(f"tile-local select, mix {mix}, n {n}")
+        tr = step_tracks(n, 500 + n % 97, 4, lnemin=math.log(1e-2), lnemax=math.log(1e5))
+        rng = np.random.default_rng(n)
+        if mix == "shuffled":
+            tr["ptype"][:] = rng.permutation(tr["ptype"])
+        elif mix == "e-/e+ only":
+            tr["ptype"][:] = np.where(rng.random(n) < 0.8, 0, 1).astype(np.int8)
+        elif mix == "gamma only":
+            tr["ptype"][:] = 2
+        ctx.set_process_sort(True)
+        a_t, a_r, _ = gpu_step(ctx, 43, tr, dev)
+        ctx.set_process_sort(False)
+        b_t, b_r, _ = gpu_step(ctx, 43, tr, dev)
+        for k in OUT_FIELDS:
+            assert same_bits(a_r[k], b_r[k]), (n, k)
+        el = tr["ptype"] != 2
+        for k, (dt, c) in TRACK_FIELDS.items():
+            m = np.tile(el, c) if k.startswith("msc") else np.ones(c * n, bool)
+            assert same_bits(a_t[k][m], b_t[k][m]), (n, k)
+
+
 def test_step_with_tables_from_json(tables2, ctx2, dev):
     """Tables written as a G4HepEmState JSON document and read back drive the
     step bit for bit like the originals (the device image is rebuilt from
