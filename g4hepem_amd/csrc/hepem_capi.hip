@@ -344,6 +344,7 @@ bool DataComplete(const G4HepEmData* d, const G4HepEmParameters* p) {
 // kClassifyTile slots)
 namespace {
 constexpr uint64_t kCountersBytes = 256;
+static_assert(kCountersBytes == 4 * kCounterWords, "the counter words fill the counters' part of a work buffer");
 constexpr uint64_t Al(uint64_t b) { return (b + 255) & ~255ull; }
 // bytes of the work buffer of a step over m slots
 constexpr uint64_t WorkBytes(uint64_t m) {

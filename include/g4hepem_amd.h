@@ -227,8 +227,10 @@ int  hepem_step_noalloc(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks*
                         hepem_step_out* res, void* hip_stream);
 
 /* The two phases of hepem_step, for callers that time them separately:
- * hepem_step_prepare clears the step counters and compacts the track slots
- * by particle type into the queue held in `work` (e-/e+ first); hepem_step_run
+ * hepem_step_prepare clears the step counters and lays the track slots out
+ * by particle type in the queue held in `work`: with the process sort, within
+ * each tile of 4096 slots (its e-/e+ first, then its gammas); without it,
+ * over the whole batch (e-/e+ first); hepem_step_run
  * performs the physics step of every queued slot: HowFar, the continuous step
  * and the choice of the discrete interaction, then (see
  * hepem_ctx_set_process_sort) a second sort of the pending interactions by
