@@ -10,7 +10,7 @@ the step, stacked secondaries with their parent's couple, safety and boundary
 flag, start-of-track state and derived streams), the e-/e+-then-gamma order,
 parent and origin, and the energy the production cuts deposit; with cuts
 applied (the e+ rule exercised by a couple whose gamma threshold is above the
-electron mass) and without; over two rounds; and the capacity error leaving
+electron mass) and without; over two passes; and the capacity error leaving
 the batch unchanged.
 """
 import json
@@ -145,7 +145,7 @@ def test_stack_advance_follows_stack_secondaries(dev, apply):
     stack = TrackStack(ctx, 4 * n)
     stack.push(TrackSoA.from_numpy(tr, "torch", dev))
     assert len(stack) == n
-    _print(f"stack, apply cuts {apply}: round 1 of {n} tracks")
+    _print(f"stack, apply cuts {apply}: first pass over {n} tracks")
     post, r, keep, m, cut, parent, origin = step_and_advance(ctx, stack, 7, 0.9, dev)
     nxt, eparent, eorigin, ecut, ne = expected_next(post, r, keep, cuts)
     assert m == len(eparent)
@@ -174,7 +174,7 @@ def test_stack_advance_follows_stack_secondaries(dev, apply):
     # secondaries' streams are distinct from each other and from the tracks'
     sid = (got["rng_stream"].astype(np.uint64) << np.uint64(32)) | got["id"].view(np.uint32).astype(np.uint64)
     assert len(np.unique(sid)) == m
-    # round 2 on the next batch, without the optional outputs
+    # a second pass over the next batch, without the optional outputs
     post2, r2, keep2, m2, _, _, _ = step_and_advance(ctx, stack, 8, 0.8, dev, with_outputs=False)
     _, ep2, _, _, _ = expected_next(post2, r2, keep2, cuts)
     assert m2 == len(ep2) == len(stack)
