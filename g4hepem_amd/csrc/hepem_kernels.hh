@@ -367,8 +367,8 @@ __global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, 
 }
 
 // ---- two-stage step: particle sort, then process sort ----------------------------------------
-// k_step_select does HowFar, the continuous step and the choice of the discrete
-// interaction (one thread per particle-queue entry), records the chosen process
+// k_step_select_local does HowFar, the continuous step and the choice of the
+// discrete interaction (one thread per particle-queue entry), records the chosen process
 // per slot and, for a pending interaction, the 64-byte record of what the
 // final-state sampling needs (at the slot: one line per track instead of ten
 // scattered fields); k_classify_tiles orders each tile's pending slots by
@@ -622,25 +622,6 @@ HD void GammaInteract(const DevTables<R>& T, uint64_t seed, hepem_tracks& out, c
   WriteSecondaries(res, n, i, sl);
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_step_select(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
-                                                        hepem_step_out res, const uint32_t* __restrict__ queue,
-                                                        uint32_t* __restrict__ counters, uint8_t* __restrict__ code,
-                                                        Staged* __restrict__ st) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = in.n;
-  if (j >= n) return;
-  const uint32_t nE = counters[0];
-  const int i = (int)queue[j];
-  if (RejectTrack(T, in, out, res, n, i, counters)) {
-    code[i] = kPNone;
-    return;
-  }
-  CopyIdentity(in, out, i);
-  code[i] = (uint32_t)j < nE ? StepElectronSelect(T, seed, in, out, res, st, n, i)
-                             : StepGammaSelect(T, seed, in, out, res, st, n, i);
-}
-
 // Stage 1 of the split Perform (as k_perform up to the choice of the
 // interaction).
 template <typename R, int KIND>
@@ -752,7 +733,7 @@ constexpr int kTailTiles = 8;
 static_assert(kTileChunks % 2 == 0, "a tile splits into two halves of whole blocks");
 inline int LocalSelectGrid(int n) { return ((n + kClassifyTile - 1) / kClassifyTile + kXcds - 1) / kXcds * kXcds * kTileChunks; }
 
-// k_step_select over the tile-local partition: XCD x takes tiles x, x + kXcds,
+// Stage 1 over the tile-local particle partition: XCD x takes tiles x, x + kXcds,
 // ..., each as kTileChunks consecutive blocks of its share, so a tile's e-/e+
 // and gamma blocks run on one XCD close together and its lines are read and
 // written through one L2. In the last kTailTiles tiles of a share the blocks
