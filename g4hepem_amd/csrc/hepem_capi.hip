@@ -907,6 +907,7 @@ struct hepem_batch_plan {
   Staged* dStaged = nullptr;      // staged record per global slot
   uint16_t* dTList = nullptr;     // tile-local process layout, batch b's slots from qoff_b
   uint32_t* dTInfo = nullptr;     // kTileInfo words per process-sort tile (tiles numbered across batches)
+  uint32_t* dLocal = nullptr;     // e-/e+ count per particle-classify tile of the tile-local partition
   int shape = kNumShapes - 1;     // block size ShapeBlock(shape)
   int tile = 2;                   // process-sort tile size TileSlots(tile)
   bool persistent = false;        // step kernels as one resident round of blocks taking work from counters
@@ -915,6 +916,10 @@ struct hepem_batch_plan {
 };
 
 namespace {
+// whether the plan's shape steps over the tile-local particle partition
+// (sorted, dispatched, kBlock threads) or over each batch's particle queue
+bool TileLocal(const hepem_batch_plan* p) { return p->processSort && !p->persistent && ShapeBlock(p->shape) == kBlock; }
+
 // one resident round of blocks of `kernel` at `block` threads, a multiple of
 // kXcds (the persistent tile kernel gives every XCD's counter its blocks)
 template <typename K> int ResidentGrid(const hepem_ctx* c, K kernel, int block) {
@@ -940,6 +945,9 @@ void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_
   if (p->persistent)
     hipLaunchKernelGGL((k_step_select_batched_q<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs,
                        p->nbatch, p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged, next);
+  else if (TileLocal(p))
+    hipLaunchKernelGGL((k_step_select_local_batched<R>), dim3(LocalSelectGridTiles((int)p->ntiles)), dim3(kBlock), 0, s, T,
+                       seed, p->dDescs, p->nbatch, (int)p->ntiles, p->dQueue, p->dCounters, p->dLocal, p->dCode, p->dStaged);
   else
     hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
                        p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
@@ -1007,6 +1015,7 @@ void FreePlan(hepem_batch_plan* p) {
   if (p->dStaged) (void)hipFree(p->dStaged);
   if (p->dTList) (void)hipFree(p->dTList);
   if (p->dTInfo) (void)hipFree(p->dTInfo);
+  if (p->dLocal) (void)hipFree(p->dLocal);
   delete p;
 }
 }  // namespace
@@ -1060,7 +1069,8 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
       hipMalloc(&p->dCounters, ((size_t)4 * nbatch + kQueueWords) * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&p->dCode, (q > 0 ? q : 1)) != hipSuccess || hipMalloc(&p->dStaged, (q > 0 ? q : 1) * sizeof(Staged)) != hipSuccess ||
       hipMalloc(&p->dTList, (q > 0 ? q : 1) * sizeof(uint16_t)) != hipSuccess ||
-      hipMalloc(&p->dTInfo, (size_t)kTileInfo * (ptiles[0] > 0 ? ptiles[0] : 1) * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&p->dTInfo, (size_t)kTileInfo * (ptiles[0] > 0 ? ptiles[0] : 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&p->dLocal, (tiles > 0 ? tiles : 1) * sizeof(uint32_t)) != hipSuccess) {
     FreePlan(p);
     return HEPEM_ERR_ALLOC;
   }
@@ -1084,7 +1094,12 @@ extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, vo
   const int nc = 4 * p->nbatch + kQueueWords;
   hipLaunchKernelGGL(k_fill_u32, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dCounters, nc, 0u);
   if (p->total > 0) {
-    hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue, p->dCounters);
+    if (TileLocal(p))
+      hipLaunchKernelGGL(k_classify_local_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue,
+                         p->dLocal);
+    else
+      hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue,
+                         p->dCounters);
     if (p->ctx->precision == HEPEM_FP64) LaunchStepBatched(p->ctx->t64, p, seed, s);
     else LaunchStepBatched(p->ctx->t32, p, seed, s);
   }

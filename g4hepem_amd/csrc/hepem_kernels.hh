@@ -670,20 +670,17 @@ constexpr int kInteractParts = InteractParts(kClassifyItems);
 inline int InteractGridTiles(int tiles, int parts = kInteractParts) { return (tiles + kXcds - 1) / kXcds * kXcds * parts; }
 inline int InteractGrid(int n) { return InteractGridTiles((n + kClassifyTile - 1) / kClassifyTile); }
 
-// Tile-local particle partition: each tile of kClassifyTile slots lists its
-// e-/e+ slots first, then its gammas, each in slot order, at the tile's own
-// range of the queue; tinfo[tile * kTileInfo] = the tile's e-/e+ count. The
-// first block also clears the step's counters (no counter is read before the
-// next kernel).
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const int8_t* __restrict__ ptype,
-                                                                   uint32_t* __restrict__ queue, uint32_t* __restrict__ tinfo,
-                                                                   uint32_t* __restrict__ counters) {
+// Tile-local particle partition of tile `tile` of a batch of n slots: the
+// tile's e-/e+ slots first, then its gammas, each in slot order, at the
+// tile's own range of the batch's queue; *ecount = the tile's e-/e+ count.
+// One kClassifyBlock-thread block per tile.
+__device__ void ClassifyLocalTile(int n, const int8_t* __restrict__ ptype, uint32_t* __restrict__ queue, int tile,
+                                  uint32_t* __restrict__ ecount) {
   __shared__ uint32_t sE[kClassifyWaves], sG[kClassifyWaves], sTot;
-  if (blockIdx.x == 0 && threadIdx.x < kCounterWords) counters[threadIdx.x] = 0u;
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int tile0 = blockIdx.x * kClassifyTile;
+  const int tile0 = tile * kClassifyTile;
   uint32_t cE = 0, cG = 0;
   for (int it = 0; it < kClassifyItems; ++it) {
     const int i = tile0 + it * kClassifyBlock + threadIdx.x;
@@ -708,7 +705,7 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const 
       tG += g;
     }
     sTot = tE;
-    tinfo[(size_t)blockIdx.x * kTileInfo] = tE;
+    *ecount = tE;
   }
   __syncthreads();
   uint32_t pE = sE[wave], pG = sTot + sG[wave];
@@ -726,42 +723,61 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const 
   }
 }
 
+// The tile-local particle partition of one batch, one block per tile, with
+// tinfo[tile * kTileInfo] = the tile's e-/e+ count. The first block also
+// clears the step's counters (no counter is read before the next kernel).
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const int8_t* __restrict__ ptype,
+                                                                   uint32_t* __restrict__ queue, uint32_t* __restrict__ tinfo,
+                                                                   uint32_t* __restrict__ counters) {
+  if (blockIdx.x == 0 && threadIdx.x < kCounterWords) counters[threadIdx.x] = 0u;
+  ClassifyLocalTile(n, ptype, queue, (int)blockIdx.x, tinfo + (size_t)blockIdx.x * kTileInfo);
+}
+
 constexpr int kTileChunks = kClassifyTile / kBlock;
 // tiles at the end of each XCD's share whose second halves (mostly gammas,
 // short) run after all their first halves (mostly e-/e+, long)
 constexpr int kTailTiles = 8;
 static_assert(kTileChunks % 2 == 0, "a tile splits into two halves of whole blocks");
-inline int LocalSelectGrid(int n) { return ((n + kClassifyTile - 1) / kClassifyTile + kXcds - 1) / kXcds * kXcds * kTileChunks; }
+inline int LocalSelectGridTiles(int ntiles) { return (ntiles + kXcds - 1) / kXcds * kXcds * kTileChunks; }
+inline int LocalSelectGrid(int n) { return LocalSelectGridTiles((n + kClassifyTile - 1) / kClassifyTile); }
 
-// Stage 1 over the tile-local particle partition: XCD x takes tiles x, x + kXcds,
-// ..., each as kTileChunks consecutive blocks of its share, so a tile's e-/e+
-// and gamma blocks run on one XCD close together and its lines are read and
+// Block -> (tile, chunk of kBlock slots) of stage 1 over the tile-local
+// particle partition of ntiles tiles: XCD x takes tiles x, x + kXcds, ...,
+// each as kTileChunks consecutive blocks of its share, so a tile's e-/e+ and
+// gamma blocks run on one XCD close together and its lines are read and
 // written through one L2. In the last kTailTiles tiles of a share the blocks
 // of the first halves come first, then those of the second halves, so the
-// launch ends on short gamma blocks rather than on long e-/e+ ones.
+// launch ends on short gamma blocks rather than on long e-/e+ ones. False for
+// the blocks of a grid of LocalSelectGridTiles(ntiles) that have no chunk.
+__device__ inline bool LocalSelectChunk(int ntiles, int& tile, int& chunk) {
+  const int x = blockIdx.x % kXcds, q = blockIdx.x / kXcds;
+  const int share = ntiles > x ? (ntiles - x + kXcds - 1) / kXcds : 0;
+  const int tail = min(kTailTiles, share), body = (share - tail) * kTileChunks;
+  int k;
+  if (q < body) {
+    k = q / kTileChunks;
+    chunk = q % kTileChunks;
+  } else {
+    const int r = q - body, h = kTileChunks / 2;
+    if (r >= tail * kTileChunks) return false;
+    const bool first = r < tail * h;
+    const int r2 = first ? r : r - tail * h;
+    k = share - tail + r2 / h;
+    chunk = (first ? 0 : h) + r2 % h;
+  }
+  tile = k * kXcds + x;
+  return true;
+}
+
+// Stage 1 of the step over the tile-local particle partition of one batch.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_step_select_local(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
                                                         hepem_step_out res, const uint32_t* __restrict__ queue,
                                                         uint32_t* __restrict__ counters, const uint32_t* __restrict__ tinfo,
                                                         uint8_t* __restrict__ code, Staged* __restrict__ st) {
   const int n = in.n;
-  const int x = blockIdx.x % kXcds, q = blockIdx.x / kXcds;
-  const int ntiles = (n + kClassifyTile - 1) / kClassifyTile;
-  const int share = ntiles > x ? (ntiles - x + kXcds - 1) / kXcds : 0;
-  const int tail = min(kTailTiles, share), body = (share - tail) * kTileChunks;
-  int k, chunk;
-  if (q < body) {
-    k = q / kTileChunks;
-    chunk = q % kTileChunks;
-  } else {
-    const int r = q - body, h = kTileChunks / 2;
-    if (r >= tail * kTileChunks) return;
-    const bool first = r < tail * h;
-    const int r2 = first ? r : r - tail * h;
-    k = share - tail + r2 / h;
-    chunk = (first ? 0 : h) + r2 % h;
-  }
-  const int tile = k * kXcds + x;
+  int tile, chunk;
+  if (!LocalSelectChunk((n + kClassifyTile - 1) / kClassifyTile, tile, chunk)) return;
   const int tile0 = tile * kClassifyTile;
   const int m = min(kClassifyTile, n - tile0);
   const int jl = chunk * kBlock + (int)threadIdx.x;
@@ -978,6 +994,51 @@ __device__ void StepSelectChunk(const DevTables<R>& T, uint64_t seed, const Batc
   CopyIdentity(in, out, i);
   cb[i] = (uint32_t)j < cnt[0] ? StepElectronSelect(T, seed, in, out, res, st + qoff, n, i)
                                : StepGammaSelect(T, seed, in, out, res, st + qoff, n, i);
+}
+
+// The tile-local particle partition over every batch: block t lays out
+// particle-classify tile t of the plan (tiles numbered across the batches, as
+// BatchDev::tile0), within its batch's queue; ecount[t] = its e-/e+ count.
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_local_batched(const BatchDev* __restrict__ d, int nbatch,
+                                                                           uint32_t* __restrict__ queue,
+                                                                           uint32_t* __restrict__ ecount) {
+  const uint32_t t = blockIdx.x;
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
+  ClassifyLocalTile(d[b].in.n, d[b].in.ptype, queue + d[b].qoff, (int)(t - d[b].tile0), ecount + t);
+}
+
+// Stage 1 of the two-stage form over the tile-local partition of every
+// batch (kBlock threads, a grid of LocalSelectGridTiles(ntiles)): the blocks
+// of the plan's tiles are placed as k_step_select_local places a single
+// batch's, and each records what StepSelectChunk records.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_step_select_local_batched(DevTables<R> T, uint64_t seed,
+                                                                      const BatchDev* __restrict__ d, int nbatch, int ntiles,
+                                                                      const uint32_t* __restrict__ queue,
+                                                                      uint32_t* __restrict__ bcounters,
+                                                                      const uint32_t* __restrict__ ecount,
+                                                                      uint8_t* __restrict__ code, Staged* __restrict__ st) {
+  int tile, chunk;
+  if (!LocalSelectChunk(ntiles, tile, chunk)) return;
+  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, (uint32_t)tile, [](const BatchDev& x) { return x.tile0; }));
+  const hepem_tracks in = d[b].in;
+  hepem_tracks out = d[b].out;
+  const hepem_step_out res = d[b].res;
+  const int n = in.n;
+  const int tile0 = (tile - (int)d[b].tile0) * kClassifyTile;
+  const int m = min(kClassifyTile, n - tile0);
+  const int jl = chunk * kBlock + (int)threadIdx.x;
+  if (jl >= m) return;
+  const uint32_t qoff = d[b].qoff;
+  const int i = (int)queue[qoff + tile0 + jl];
+  uint8_t* cb = code + qoff;
+  if (RejectTrack(T, in, out, res, n, i, bcounters + 4 * b)) {
+    cb[i] = kPNone;
+    return;
+  }
+  CopyIdentity(in, out, i);
+  cb[i] = jl < (int)ecount[tile] ? StepElectronSelect(T, seed, in, out, res, st + qoff, n, i)
+                                 : StepGammaSelect(T, seed, in, out, res, st + qoff, n, i);
 }
 
 // Next work item of a persistent kernel: thread 0 takes it from *next and the
