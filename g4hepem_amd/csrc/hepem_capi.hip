@@ -946,7 +946,7 @@ void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_
     hipLaunchKernelGGL((k_step_select_batched_q<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs,
                        p->nbatch, p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged, next);
   else if (TileLocal(p))
-    hipLaunchKernelGGL((k_step_select_local_batched<R>), dim3(LocalSelectGridTiles((int)p->ntiles)), dim3(kBlock), 0, s, T,
+    hipLaunchKernelGGL((k_step_select_local_batched<R>), dim3(p->grid), dim3(kBlock), 0, s, T,
                        seed, p->dDescs, p->nbatch, (int)p->ntiles, p->dQueue, p->dCounters, p->dLocal, p->dCode, p->dStaged);
   else
     hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
@@ -979,8 +979,11 @@ template <typename R> void LaunchStepBatched(const DevTables<R>& T, const hepem_
   }
 }
 
-// blocks of the step kernel for the plan's shape: every chunk, or one resident round
+// blocks of the step kernel (its first stage with the process sort) for the
+// plan's shape: every chunk, the tile-local placement of every tile's chunks,
+// or one resident round
 template <typename R, int S> int StepGrid(const hepem_batch_plan* p) {
+  if (!p->persistent && TileLocal(p)) return std::max(1, LocalSelectGridTiles((int)p->ntiles));
   if (!p->persistent) return (int)(p->nchunks[S] > 0 ? p->nchunks[S] : 1);
   return p->processSort ? ResidentGrid(p->ctx, k_step_select_batched_q<R, S>, ShapeBlock(S))
                         : ResidentGrid(p->ctx, k_step_batched_q<R, S>, ShapeBlock(S));

@@ -378,7 +378,11 @@ int  hepem_batch_plan_run(const hepem_batch_plan* plan, uint64_t seed, void* hip
  *   persistent  0: one block per chunk of work, dispatched by the hardware;
  *               1: one resident round of blocks taking chunks from a work
  *               counter (reset by every run) until none are left;
- *   grid        (read only) blocks of the step kernel. */
+ *   grid        (read only) blocks of the step kernel, or of its first stage
+ *               with the process sort: one per chunk of `block` slots, or,
+ *               dispatched at 256 threads, 16 per 4096-slot tile of the
+ *               batches (tiles rounded up to a multiple of 8), or the
+ *               persistent form's resident round. */
 typedef struct hepem_batch_shape {
   int32_t block;
   int32_t sort_tile;
