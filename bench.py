@@ -119,6 +119,43 @@ def time_steps(ctx, tin, tout, res, work, steps, warmup, seed):
     return total_ms / steps, float(np.mean(run_ms)), wall
 
 
+def time_split(ctx, tin, res, work, steps, warmup, seed):
+    """Times hepem_howfar + hepem_perform on a copy of `tin` (the split calls
+    step in place; the caller's geometry would run between them: here it
+    changes nothing). The copy back to the original tracks runs before each
+    timed pair, outside its events. Returns ms per HowFar + Perform."""
+    import ctypes as C
+
+    import torch
+    from g4hepem_amd import lib
+    from g4hepem_amd.tracks import HowFarSoA, TrackSoA
+    L = lib()
+    dev = tin["ekin"].device
+    t = TrackSoA(tin.n, "torch", dev)
+    hs = HowFarSoA(tin.n, "torch", dev)
+    stream = torch.cuda.current_stream()
+    sh = C.c_void_p(int(stream.cuda_stream))
+    st, shs, rr = t.struct(), hs.struct(), res.struct()
+    pt, phs, prr = C.byref(st), C.byref(shs), C.byref(rr)
+    wp = C.c_void_p(work.data_ptr())
+    sd = C.c_uint64(seed)
+    pairs = []
+    for k in range(warmup + steps):
+        for name in t.a:
+            t.a[name].copy_(tin.a[name])
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        rc = L.hepem_howfar(ctx.h, sd, pt, phs, wp, sh)
+        assert rc == 0, rc
+        rc = L.hepem_perform(ctx.h, sd, pt, phs, prr, wp, sh)
+        assert rc == 0, rc
+        b.record(stream)
+        if k >= warmup:
+            pairs.append((a, b))
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+
+
 def config3_batches(seed: int, nbatch: int = 64):
     """64 batches, sizes log-uniform in [1e3, 1e6], per-track distributions
     as the headline workload; track ids run on across batches."""
@@ -595,6 +632,12 @@ def main():
             ordered[form] = {"value": n / (mso * 1e-3), "unit": "track-steps/s", "ms_per_step": mso, "kernel_ms": runo}
         line["config2_type_ordered"] = ordered
         del tin_o
+        log("config-2 tracks through the split HowFar and Perform calls")
+        split = {"calls": "hepem_howfar, then hepem_perform (in place, no geometry between them)"}
+        for form, ctx in (("fp64", ctx64), ("fp32_table_form", ctx32)):
+            mss = time_split(ctx, tin, res, work, args.steps, args.warmup, args.seed)
+            split[form] = {"value": n / (mss * 1e-3), "unit": "track-steps/s", "ms_per_step": mss}
+        line["config2_split_calls"] = split
         log("config-1 lookups")
         tables1 = Tables(SET1, 0.7)
         ctx1 = Context(tables1, HEPEM_FP64)
