@@ -135,6 +135,51 @@ def test_batched_every_shape_bit_identical(ctx64, dev):
         plan.set_shape(256, 3000, 0)
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_batched_tile_local_layouts(tables2, dev, form):
+    """The sorted, dispatched 256-thread shape steps over the tile-local
+    particle partition of the plan's tiles (numbered across the batches): on
+    batches with shuffled, e-/e+-only and gamma-only layouts, sizes below,
+    at and across the 4096-slot tile, and more tiles than one XCD share's
+    tail, every batch equals hepem_step bit for bit."""
+    from g4hepem_amd import HEPEM_FP32T, HEPEM_FP64, BatchPlan, Context
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_outputs_differ
+    ctx = Context(tables2, HEPEM_FP64 if form == "fp64" else HEPEM_FP32T)
+    sizes = [4096, 1, 70000, 4097, 0, 300000, 63, 12288, 9000]
+    host, batches = make_batches(sizes, dev, seed0=1400)
+    rng = np.random.default_rng(14)
+    for k, (tr, (tin, _, _)) in enumerate(zip(host, batches)):
+        if k % 3 == 0:
+            tr["ptype"][:] = rng.permutation(tr["ptype"])
+        elif k % 3 == 1:
+            tr["ptype"][:] = np.where(rng.random(len(tr["ptype"])) < 0.7, 0, 1).astype(np.int8)
+        else:
+            tr["ptype"][:] = 2
+        tin["ptype"].copy_(torch.from_numpy(tr["ptype"]))
+    work = ctx.work_buffer(max(sizes), dev)
+    refs = []
+    for tin, _, _ in batches:
+        t_ref, r_ref = TrackSoA(tin.n, "torch", dev), StepOutSoA(tin.n, "torch", dev)
+        ctx.step(SEED, tin, t_ref, r_ref, work)
+        refs.append((t_ref, r_ref))
+    plan = BatchPlan(ctx, batches)
+    for tile in (1024, 4096, 8192):
+        _print(f"tile-local batched, {form}, sort tile {tile}")
+        got = plan.set_shape(256, tile, 0)
+        ntiles = sum((n + 4095) // 4096 for n in sizes)
+        assert got["grid"] == (ntiles + 7) // 8 * 8 * 16
+        for _, tout, res in batches:
+            res["winner"].fill_(-7)
+            tout["ekin"].fill_(-1.0)
+        plan.run(SEED)
+        torch.cuda.synchronize()
+        for k, ((tin, tout, res), (t_ref, r_ref)) in enumerate(zip(batches, refs)):
+            diff = step_outputs_differ(tin["ptype"], tout, res, t_ref, r_ref)
+            assert not diff, (form, tile, k, diff)
+    plan.close()
+    ctx.close()
+
+
 def test_batched_graph_replay(ctx64, dev):
     """The plan run captured into a graph; every replay, with eager work and
     synchronisation in between, reproduces the eager result."""
