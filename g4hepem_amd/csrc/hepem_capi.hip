@@ -908,6 +908,8 @@ struct hepem_batch_plan {
   uint16_t* dTList = nullptr;     // tile-local process layout, batch b's slots from qoff_b
   uint32_t* dTInfo = nullptr;     // kTileInfo words per process-sort tile (tiles numbered across batches)
   uint32_t* dLocal = nullptr;     // e-/e+ count per particle-classify tile of the tile-local partition
+  uint32_t* dTileBatch = nullptr; // batch of every tile, one array per tile size TileSlots(t), from tbOff[t]
+  uint64_t tbOff[kNumTiles] = {0, 0, 0, 0};
   int shape = kNumShapes - 1;     // block size ShapeBlock(shape)
   int tile = 2;                   // process-sort tile size TileSlots(tile)
   bool persistent = false;        // step kernels as one resident round of blocks taking work from counters
@@ -947,19 +949,21 @@ void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_
                        p->nbatch, p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged, next);
   else if (TileLocal(p))
     hipLaunchKernelGGL((k_step_select_local_batched<R>), dim3(p->grid), dim3(kBlock), 0, s, T,
-                       seed, p->dDescs, p->nbatch, (int)p->ntiles, p->dQueue, p->dCounters, p->dLocal, p->dCode, p->dStaged);
+                       seed, p->dDescs, p->dTileBatch + p->tbOff[kParticleTiles], (int)p->ntiles, p->dQueue, p->dCounters,
+                       p->dLocal, p->dCode, p->dStaged);
   else
     hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
                        p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
   const uint32_t nt = p->nptiles[TI];
-  hipLaunchKernelGGL(k_classify_tiles_batched<TI>, dim3(nt), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dCode,
+  hipLaunchKernelGGL(k_classify_tiles_batched<TI>, dim3(nt), dim3(kClassifyBlock), 0, s, p->dDescs, p->dTileBatch + p->tbOff[TI], p->dCode,
                      p->dTList, p->dTInfo);
   if (p->persistent) {
-    hipLaunchKernelGGL((k_interact_tiles_batched_q<R, TI>), dim3(p->igrid), dim3(kBlock), 0, s, T, seed, p->dDescs, p->nbatch,
-                       nt, p->dCode, p->dStaged, p->dTList, p->dTInfo, next + 1);
+    hipLaunchKernelGGL((k_interact_tiles_batched_q<R, TI>), dim3(p->igrid), dim3(kBlock), 0, s, T, seed, p->dDescs,
+                       p->dTileBatch + p->tbOff[TI], nt, p->dCode, p->dStaged, p->dTList, p->dTInfo, next + 1);
   } else {
     hipLaunchKernelGGL((k_interact_tiles_batched<R, TI>), dim3(InteractGridTiles((int)nt, InteractParts(TileItems(TI)))),
-                       dim3(kBlock), 0, s, T, seed, p->dDescs, p->nbatch, nt, p->dCode, p->dStaged, p->dTList, p->dTInfo);
+                       dim3(kBlock), 0, s, T, seed, p->dDescs, p->dTileBatch + p->tbOff[TI], nt, p->dCode, p->dStaged, p->dTList,
+                       p->dTInfo);
   }
 }
 template <typename R, int S>
@@ -1019,6 +1023,7 @@ void FreePlan(hepem_batch_plan* p) {
   if (p->dTList) (void)hipFree(p->dTList);
   if (p->dTInfo) (void)hipFree(p->dTInfo);
   if (p->dLocal) (void)hipFree(p->dLocal);
+  if (p->dTileBatch) (void)hipFree(p->dTileBatch);
   delete p;
 }
 }  // namespace
@@ -1048,6 +1053,17 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
     for (int s = 0; s < kNumShapes; ++s) chunks[s] += (n + ShapeBlock(s) - 1) / ShapeBlock(s);
   }
   if (q >= (1ull << 31)) return HEPEM_ERR_CAPACITY;
+  // the batch of every tile, per tile size: the tile kernels look it up
+  // instead of searching the descriptors' prefix sums
+  std::vector<uint32_t> tileBatch;
+  uint64_t tbOff[kNumTiles];
+  for (int t = 0; t < kNumTiles; ++t) {
+    tbOff[t] = tileBatch.size();
+    for (int b = 0; b < nbatch; ++b) {
+      const uint64_t k = (b + 1 < nbatch ? (uint64_t)h[b + 1].ptile0[t] : ptiles[t]) - h[b].ptile0[t];
+      tileBatch.insert(tileBatch.end(), (size_t)k, (uint32_t)b);
+    }
+  }
   BatchDev& last = h[nbatch];
   std::memset(&last, 0, sizeof(last));
   last.qoff = (uint32_t)q;
@@ -1073,11 +1089,15 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
       hipMalloc(&p->dCode, (q > 0 ? q : 1)) != hipSuccess || hipMalloc(&p->dStaged, (q > 0 ? q : 1) * sizeof(Staged)) != hipSuccess ||
       hipMalloc(&p->dTList, (q > 0 ? q : 1) * sizeof(uint16_t)) != hipSuccess ||
       hipMalloc(&p->dTInfo, (size_t)kTileInfo * (ptiles[0] > 0 ? ptiles[0] : 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&p->dLocal, (tiles > 0 ? tiles : 1) * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&p->dLocal, (tiles > 0 ? tiles : 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&p->dTileBatch, std::max<size_t>(1, tileBatch.size()) * sizeof(uint32_t)) != hipSuccess) {
     FreePlan(p);
     return HEPEM_ERR_ALLOC;
   }
-  if (hipMemcpy(p->dDescs, h.data(), h.size() * sizeof(BatchDev), hipMemcpyHostToDevice) != hipSuccess) {
+  for (int t = 0; t < kNumTiles; ++t) p->tbOff[t] = tbOff[t];
+  if (hipMemcpy(p->dDescs, h.data(), h.size() * sizeof(BatchDev), hipMemcpyHostToDevice) != hipSuccess ||
+      (!tileBatch.empty() && hipMemcpy(p->dTileBatch, tileBatch.data(), tileBatch.size() * sizeof(uint32_t),
+                                       hipMemcpyHostToDevice) != hipSuccess)) {
     FreePlan(p);
     return HEPEM_ERR_HIP;
   }
@@ -1098,11 +1118,11 @@ extern "C" int hepem_batch_plan_run(const hepem_batch_plan* p, uint64_t seed, vo
   hipLaunchKernelGGL(k_fill_u32, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p->dCounters, nc, 0u);
   if (p->total > 0) {
     if (TileLocal(p))
-      hipLaunchKernelGGL(k_classify_local_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue,
-                         p->dLocal);
+      hipLaunchKernelGGL(k_classify_local_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs,
+                         p->dTileBatch + p->tbOff[kParticleTiles], p->dQueue, p->dLocal);
     else
-      hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs, p->nbatch, p->dQueue,
-                         p->dCounters);
+      hipLaunchKernelGGL(k_classify_batched, dim3(p->ntiles), dim3(kClassifyBlock), 0, s, p->dDescs,
+                         p->dTileBatch + p->tbOff[kParticleTiles], p->dQueue, p->dCounters);
     if (p->ctx->precision == HEPEM_FP64) LaunchStepBatched(p->ctx->t64, p, seed, s);
     else LaunchStepBatched(p->ctx->t32, p, seed, s);
   }

@@ -908,6 +908,9 @@ constexpr int ShapeBlock(int s) { return 64 << s; }
 constexpr int kNumTiles = 4;
 constexpr int TileItems(int t) { return 1 << t; }
 constexpr int TileSlots(int t) { return kClassifyBlock << t; }
+// the tile size of the particle classification among them
+constexpr int kParticleTiles = 2;
+static_assert(TileSlots(kParticleTiles) == kClassifyTile, "particle tiles are one of the sort tile sizes");
 // Words of a plan's work counters after its per-batch counters: the next
 // chunk of the persistent step kernel, then the next work item of each XCD's
 // share of the persistent tile kernel.
@@ -936,10 +939,11 @@ template <typename Key> HD int FindBatch(const BatchDev* __restrict__ d, int nba
 }
 
 // bcounters[4b + 0..2]: e-/e+, gammas, rejected tracks of batch b
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_batched(const BatchDev* __restrict__ d, int nbatch,
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_batched(const BatchDev* __restrict__ d,
+                                                             const uint32_t* __restrict__ tbatch,
                                                              uint32_t* __restrict__ queue, uint32_t* __restrict__ bcounters) {
   const uint32_t t = blockIdx.x;
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
+  const int b = __builtin_amdgcn_readfirstlane((int)tbatch[t]);
   ClassifyTile(d[b].in.n, d[b].in.ptype, queue + d[b].qoff, bcounters + 4 * b, (int)(t - d[b].tile0));
 }
 
@@ -999,11 +1003,12 @@ __device__ void StepSelectChunk(const DevTables<R>& T, uint64_t seed, const Batc
 // The tile-local particle partition over every batch: block t lays out
 // particle-classify tile t of the plan (tiles numbered across the batches, as
 // BatchDev::tile0), within its batch's queue; ecount[t] = its e-/e+ count.
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_local_batched(const BatchDev* __restrict__ d, int nbatch,
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_local_batched(const BatchDev* __restrict__ d,
+                                                                           const uint32_t* __restrict__ tbatch,
                                                                            uint32_t* __restrict__ queue,
                                                                            uint32_t* __restrict__ ecount) {
   const uint32_t t = blockIdx.x;
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.tile0; }));
+  const int b = __builtin_amdgcn_readfirstlane((int)tbatch[t]);
   ClassifyLocalTile(d[b].in.n, d[b].in.ptype, queue + d[b].qoff, (int)(t - d[b].tile0), ecount + t);
 }
 
@@ -1013,14 +1018,15 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_local_batched(const
 // batch's, and each records what StepSelectChunk records.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_step_select_local_batched(DevTables<R> T, uint64_t seed,
-                                                                      const BatchDev* __restrict__ d, int nbatch, int ntiles,
+                                                                      const BatchDev* __restrict__ d,
+                                                                      const uint32_t* __restrict__ tbatch, int ntiles,
                                                                       const uint32_t* __restrict__ queue,
                                                                       uint32_t* __restrict__ bcounters,
                                                                       const uint32_t* __restrict__ ecount,
                                                                       uint8_t* __restrict__ code, Staged* __restrict__ st) {
   int tile, chunk;
   if (!LocalSelectChunk(ntiles, tile, chunk)) return;
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, (uint32_t)tile, [](const BatchDev& x) { return x.tile0; }));
+  const int b = __builtin_amdgcn_readfirstlane((int)tbatch[tile]);
   const hepem_tracks in = d[b].in;
   hepem_tracks out = d[b].out;
   const hepem_step_out res = d[b].res;
@@ -1097,23 +1103,24 @@ __global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched_q(DevTabl
 // (BatchDev::ptile0[TI]), each batch's tile list and codes sit at its queue
 // offset.
 template <int TI>
-__global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles_batched(const BatchDev* __restrict__ d, int nbatch,
+__global__ void __launch_bounds__(kClassifyBlock) k_classify_tiles_batched(const BatchDev* __restrict__ d,
+                                                                           const uint32_t* __restrict__ tbatch,
                                                                            const uint8_t* __restrict__ code,
                                                                            uint16_t* __restrict__ tlist,
                                                                            uint32_t* __restrict__ tinfo) {
   const uint32_t t = blockIdx.x;
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, t, [](const BatchDev& x) { return x.ptile0[TI]; }));
+  const int b = __builtin_amdgcn_readfirstlane((int)tbatch[t]);
   const uint32_t qoff = d[b].qoff;
   ClassifyTileLayout<TileItems(TI)>(d[b].in.n, code + qoff, tlist + qoff, tinfo + (size_t)t * kTileInfo,
                                     (int)(t - d[b].ptile0[TI]));
 }
 
 template <typename R, int TI>
-__device__ void InteractBatchedTile(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d, int nbatch,
-                                    uint32_t tile, uint32_t chunk, const uint8_t* __restrict__ code,
-                                    const Staged* __restrict__ st, const uint16_t* __restrict__ tlist,
-                                    const uint32_t* __restrict__ tinfo) {
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, tile, [](const BatchDev& x) { return x.ptile0[TI]; }));
+__device__ void InteractBatchedTile(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d,
+                                    const uint32_t* __restrict__ tbatch, uint32_t tile, uint32_t chunk,
+                                    const uint8_t* __restrict__ code, const Staged* __restrict__ st,
+                                    const uint16_t* __restrict__ tlist, const uint32_t* __restrict__ tinfo) {
+  const int b = __builtin_amdgcn_readfirstlane((int)tbatch[tile]);
   hepem_tracks out = d[b].out;
   const uint32_t qoff = d[b].qoff;
   InteractTileChunk<R>(T, seed, out, d[b].res, out.n, code + qoff, st + qoff, tlist + qoff, tinfo + (size_t)tile * kTileInfo,
@@ -1122,13 +1129,14 @@ __device__ void InteractBatchedTile(const DevTables<R>& T, uint64_t seed, const 
 
 template <typename R, int TI>
 __global__ void __launch_bounds__(kBlock) k_interact_tiles_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
-                                                                   int nbatch, uint32_t ntiles, const uint8_t* __restrict__ code,
+                                                                   const uint32_t* __restrict__ tbatch, uint32_t ntiles,
+                                                                   const uint8_t* __restrict__ code,
                                                                    const Staged* __restrict__ st,
                                                                    const uint16_t* __restrict__ tlist,
                                                                    const uint32_t* __restrict__ tinfo) {
   uint32_t tile, chunk;
   if (!TileChunk<InteractParts(TileItems(TI))>(ntiles, tile, chunk)) return;
-  InteractBatchedTile<R, TI>(T, seed, d, nbatch, tile, chunk, code, st, tlist, tinfo);
+  InteractBatchedTile<R, TI>(T, seed, d, tbatch, tile, chunk, code, st, tlist, tinfo);
 }
 
 // Persistent form of k_interact_tiles_batched: one resident round of blocks.
@@ -1137,7 +1145,8 @@ __global__ void __launch_bounds__(kBlock) k_interact_tiles_batched(DevTables<R> 
 // InteractParts blocks' worth each, from its counter next[x] (zero at launch).
 template <typename R, int TI>
 __global__ void __launch_bounds__(kBlock) k_interact_tiles_batched_q(DevTables<R> T, uint64_t seed,
-                                                                     const BatchDev* __restrict__ d, int nbatch, uint32_t ntiles,
+                                                                     const BatchDev* __restrict__ d,
+                                                                     const uint32_t* __restrict__ tbatch, uint32_t ntiles,
                                                                      const uint8_t* __restrict__ code,
                                                                      const Staged* __restrict__ st,
                                                                      const uint16_t* __restrict__ tlist,
@@ -1149,7 +1158,7 @@ __global__ void __launch_bounds__(kBlock) k_interact_tiles_batched_q(DevTables<R
     const uint32_t tile = (k / P) * kXcds + x;
     if (tile >= ntiles) return;
     const uint32_t chunk = (k % P) * kInteractWaves + (threadIdx.x >> 6);
-    InteractBatchedTile<R, TI>(T, seed, d, nbatch, tile, chunk, code, st, tlist, tinfo);
+    InteractBatchedTile<R, TI>(T, seed, d, tbatch, tile, chunk, code, st, tlist, tinfo);
   }
 }
 
