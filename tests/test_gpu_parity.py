@@ -267,19 +267,20 @@ def test_process_sort_equals_fused(tables2, dev, form):
         assert same_bits(a_t[k][m], b_t[k][m]), k
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
 @pytest.mark.parametrize("mix", ["4:1:5", "shuffled", "e-/e+ only", "gamma only"])
-def test_tile_local_select_covers_every_slot(tables2, dev, mix):
+def test_tile_local_select_covers_every_slot(tables2, dev, mix, form):
     """The tile-local select stage maps blocks to (tile, chunk) per XCD, with
     the last 8 tiles of each XCD's share split into e-/e+ and gamma halves:
     batches smaller than one tile, ragged last tiles, XCDs without tiles,
     shares shorter and longer than the tail, and tiles whose e-/e+ count is
     not a multiple of 64 (a shuffled layout) all step bit for bit like the
-    fused kernel."""
-    from g4hepem_amd import HEPEM_FP64, Context
+    fused kernel, in both forms."""
+    from g4hepem_amd import HEPEM_FP32T, HEPEM_FP64, Context
     from g4hepem_amd.tracks import step_tracks
-    ctx = Context(tables2, HEPEM_FP64)
+    ctx = Context(tables2, HEPEM_FP64 if form == "fp64" else HEPEM_FP32T)
     for n in (1, 63, 4095, 4097, 8 * 4096 + 1, 40000, 200000, 300001):
-        _print(f"tile-local select, mix {mix}, n {n}")
+        _print(f"tile-local select, {form}, mix {mix}, n {n}")
         tr = step_tracks(n, 500 + n % 97, 4, lnemin=math.log(1e-2), lnemax=math.log(1e5))
         rng = np.random.default_rng(n)
         if mix == "shuffled":
