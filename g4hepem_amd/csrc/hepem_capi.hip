@@ -650,9 +650,11 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
   const bool fp64 = c->precision == HEPEM_FP64;
   if (!SortsProcesses(c, n)) {
     if (fp64)
-      hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue, w.counters);
+      hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue, w.counters,
+                         w.tinfo);
     else
-      hipLaunchKernelGGL(k_step<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue, w.counters);
+      hipLaunchKernelGGL(k_step<float>, dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *in, *out, *res, w.queue, w.counters,
+                         w.tinfo);
   } else {
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
     const int igrid = InteractGrid(n);

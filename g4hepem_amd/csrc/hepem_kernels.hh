@@ -24,9 +24,11 @@ namespace hepem {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
-// counters[0]: e-/e+ tracks, counters[1]: gammas, counters[2]: rejected tracks
-// (counters[3..63] reserved)
+// counters[0]: e-/e+ tracks, counters[1]: gammas, counters[2]: rejected tracks,
+// counters[3]: 1 when the queue holds the tile-local partition (counters[4..63]
+// reserved)
 constexpr int kCounterWords = 64;
+constexpr int kCounterLayout = 3;
 __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -349,21 +351,6 @@ __global__ void __launch_bounds__(kBlock) k_el_end(DevTables<R> T, uint64_t seed
   res.winner[i] = el.t.winner;
   res.stopped[i] = el.t.ekin <= 0.0 ? 1 : 0;
   WriteSecondaries(res, n, i, sl);
-}
-
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
-                                                 hepem_step_out res, const uint32_t* __restrict__ queue,
-                                                 uint32_t* __restrict__ counters) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = in.n;
-  if (j >= n) return;
-  const uint32_t nE = counters[0];
-  const int i = (int)queue[j];
-  if (RejectTrack(T, in, out, res, n, i, counters)) return;
-  CopyIdentity(in, out, i);
-  if ((uint32_t)j < nE) StepElectron(T, seed, in, out, res, n, i);
-  else StepGamma(T, seed, in, out, res, n, i);
 }
 
 // ---- two-stage step: particle sort, then process sort ----------------------------------------
@@ -725,11 +712,12 @@ __device__ void ClassifyLocalTile(int n, const int8_t* __restrict__ ptype, uint3
 
 // The tile-local particle partition of one batch, one block per tile, with
 // tinfo[tile * kTileInfo] = the tile's e-/e+ count. The first block also
-// clears the step's counters (no counter is read before the next kernel).
+// clears the step's counters and marks the layout in counters[kCounterLayout]
+// (no counter is read before the next kernel).
 __global__ void __launch_bounds__(kClassifyBlock) k_classify_local(int n, const int8_t* __restrict__ ptype,
                                                                    uint32_t* __restrict__ queue, uint32_t* __restrict__ tinfo,
                                                                    uint32_t* __restrict__ counters) {
-  if (blockIdx.x == 0 && threadIdx.x < kCounterWords) counters[threadIdx.x] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < kCounterWords) counters[threadIdx.x] = threadIdx.x == kCounterLayout ? 1u : 0u;
   ClassifyLocalTile(n, ptype, queue, (int)blockIdx.x, tinfo + (size_t)blockIdx.x * kTileInfo);
 }
 
@@ -782,14 +770,35 @@ __global__ void __launch_bounds__(kBlock) k_step_select_local(DevTables<R> T, ui
   const int m = min(kClassifyTile, n - tile0);
   const int jl = chunk * kBlock + (int)threadIdx.x;
   if (jl >= m) return;
-  const int nEt = (int)tinfo[(size_t)tile * kTileInfo];
+  // the layout hepem_step_prepare chose: tile-local, or e-/e+ first over the batch
+  const bool isE = counters[kCounterLayout] ? jl < (int)tinfo[(size_t)tile * kTileInfo] : (uint32_t)(tile0 + jl) < counters[0];
   const int i = (int)queue[tile0 + jl];
   if (RejectTrack(T, in, out, res, n, i, counters)) {
     code[i] = kPNone;
     return;
   }
   CopyIdentity(in, out, i);
-  code[i] = jl < nEt ? StepElectronSelect(T, seed, in, out, res, st, n, i) : StepGammaSelect(T, seed, in, out, res, st, n, i);
+  code[i] = isE ? StepElectronSelect(T, seed, in, out, res, st, n, i) : StepGammaSelect(T, seed, in, out, res, st, n, i);
+}
+
+// The fused step over either particle layout of the queue (the one
+// hepem_step_prepare chose, as counters[kCounterLayout] records): e-/e+ are
+// the first counters[0] entries, or the first tinfo[tile * kTileInfo] entries
+// of each tile.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_step(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
+                                                 hepem_step_out res, const uint32_t* __restrict__ queue,
+                                                 uint32_t* __restrict__ counters, const uint32_t* __restrict__ tinfo) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = in.n;
+  if (j >= n) return;
+  const bool isE = counters[kCounterLayout] ? j % kClassifyTile < (int)tinfo[(size_t)(j / kClassifyTile) * kTileInfo]
+                                            : (uint32_t)j < counters[0];
+  const int i = (int)queue[j];
+  if (RejectTrack(T, in, out, res, n, i, counters)) return;
+  CopyIdentity(in, out, i);
+  if (isE) StepElectron(T, seed, in, out, res, n, i);
+  else StepGamma(T, seed, in, out, res, n, i);
 }
 
 // the process layout of tile `tile` of a batch of n slots (codes, tile list

@@ -230,7 +230,9 @@ int  hepem_step_noalloc(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks*
  * hepem_step_prepare clears the step counters and lays the track slots out
  * by particle type in the queue held in `work`: with the process sort, within
  * each tile of 4096 slots (its e-/e+ first, then its gammas); without it,
- * over the whole batch (e-/e+ first); hepem_step_run
+ * over the whole batch (e-/e+ first). The layout is recorded in `work`, so
+ * hepem_step_run steps the queue correctly even if the process-sort mode
+ * changed between the two calls; hepem_step_run
  * performs the physics step of every queued slot: HowFar, the continuous step
  * and the choice of the discrete interaction, then (see
  * hepem_ctx_set_process_sort) a second sort of the pending interactions by

@@ -301,6 +301,35 @@ def test_tile_local_select_covers_every_slot(tables2, dev, mix, form):
             assert same_bits(a_t[k][m], b_t[k][m]), (n, k)
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_step_prepare_run_across_a_mode_change(tables2, dev, form):
+    """hepem_step_prepare lays the queue out for the process-sort mode of its
+    call and records the layout in the work buffer; hepem_step_run reads it,
+    so a mode changed between the two calls (either way) still steps every
+    track as hepem_step does."""
+    from g4hepem_amd import HEPEM_FP32T, HEPEM_FP64, Context
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_outputs_differ, step_tracks
+    ctx = Context(tables2, HEPEM_FP64 if form == "fp64" else HEPEM_FP32T)
+    n = 70001
+    tr = step_tracks(n, 88, 4, lnemin=math.log(1e-2), lnemax=math.log(1e5))
+    tr["ptype"][:] = np.random.default_rng(88).permutation(tr["ptype"])
+    tin = TrackSoA.from_numpy(tr, "torch", dev)
+    work = ctx.work_buffer(n, dev)
+    ref_t, ref_r = TrackSoA(n, "torch", dev), StepOutSoA(n, "torch", dev)
+    ctx.step(13, tin, ref_t, ref_r, work)
+    for before, after in ((True, False), (False, True)):
+        _print(f"prepare with sort {before}, run with sort {after}, {form}")
+        t, r = TrackSoA(n, "torch", dev), StepOutSoA(n, "torch", dev)
+        ctx.set_process_sort(before)
+        ctx.step_prepare(tin, work)
+        ctx.set_process_sort(after)
+        ctx.step_run(13, tin, t, r, work)
+        torch.cuda.synchronize()
+        diff = step_outputs_differ(tin["ptype"], t, r, ref_t, ref_r)
+        assert not diff, (before, after, diff)
+    ctx.set_process_sort("auto")
+
+
 def test_step_with_tables_from_json(tables2, ctx2, dev):
     """Tables written as a G4HepEmState JSON document and read back drive the
     step bit for bit like the originals (the device image is rebuilt from
