@@ -34,9 +34,9 @@ class Context:
 
     def set_process_sort(self, on) -> None:
         """Sort of the pending interactions by process in the step and the
-        Perform calls: True (always), False (never) or "auto" (the default,
-        the faster choices measured: always, except the FP32-table form's split
-        Perform calls from 65536 slots); results are identical."""
+        Perform calls: True (always), False (never) or "auto" (the default:
+        the faster choice as measured, which is always); results are
+        identical."""
         mode = 2 if on == "auto" else (1 if on else 0)
         N.check("hepem_ctx_set_process_sort", N.lib().hepem_ctx_set_process_sort(self.h, mode))
 

@@ -600,14 +600,10 @@ extern "C" int hepem_ctx_set_process_sort(hepem_ctx* c, int mode) {
 }
 
 namespace {
-// whether a step, batch plan or split Perform (`perform`) over n slots sorts
-// its pending interactions by process; HEPEM_SORT_AUTO: the faster choice as
-// measured for each (the FP32 split Perform, over the global particle queue,
-// runs fused from HEPEM_SORT_AUTO_FP32_MAX_TRACKS slots)
-bool SortsProcesses(const hepem_ctx* c, int64_t n, bool perform = false) {
-  if (c->processSort != HEPEM_SORT_AUTO) return c->processSort == HEPEM_SORT_ON;
-  return c->precision == HEPEM_FP64 || !perform || n < HEPEM_SORT_AUTO_FP32_MAX_TRACKS;
-}
+// whether a step, batch plan or split Perform sorts its pending interactions
+// by process; HEPEM_SORT_AUTO: the faster choice as measured, which is the
+// sort at every size in both forms
+bool SortsProcesses(const hepem_ctx* c) { return c->processSort != HEPEM_SORT_OFF; }
 }  // namespace
 
 namespace {
@@ -636,7 +632,7 @@ int PrepareQueue(const hepem_ctx* c, const hepem_tracks* in, void* work, void* s
 
 extern "C" int hepem_step_prepare(const hepem_ctx* c, const hepem_tracks* in, void* work, void* stream) {
   if (!c || !TracksOk(in)) return HEPEM_ERR_INVALID;
-  return PrepareQueue(c, in, work, stream, SortsProcesses(c, in->n));
+  return PrepareQueue(c, in, work, stream, SortsProcesses(c));
 }
 
 extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tracks* in, hepem_tracks* out,
@@ -648,7 +644,7 @@ extern "C" int hepem_step_run(const hepem_ctx* c, uint64_t seed, const hepem_tra
   const WorkLayout w(work, n);
   const int grid = (n + kBlock - 1) / kBlock;
   const bool fp64 = c->precision == HEPEM_FP64;
-  if (!SortsProcesses(c, n)) {
+  if (!SortsProcesses(c)) {
     if (fp64)
       hipLaunchKernelGGL(k_step<double>, dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *in, *out, *res, w.queue, w.counters,
                          w.tinfo);
@@ -743,29 +739,34 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
   const uint32_t* queue = KIND == kMixed ? wl.queue : nullptr;
   if (!perform) {
     if (KIND == kMixed) {
-      rc = PrepareQueue(c, t, work, stream, false);
+      // the particle layout the coming Perform sorts best with, recorded in
+      // the work buffer for the Perform kernels
+      rc = PrepareQueue(c, t, work, stream, SortsProcesses(c));
       if (rc != HEPEM_OK) return rc;
     } else {
       hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, counters, 64, 0u);
     }
   }
-  const int grid = (n + kBlock - 1) / kBlock;
+  // a mixed batch's kernels cover either layout of its queue
+  const int grid = KIND == kMixed ? LocalSelectGrid(n) : (n + kBlock - 1) / kBlock;
   const bool fp64 = c->precision == HEPEM_FP64;
   if (!perform && fp64) {
-    hipLaunchKernelGGL((k_howfar<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, queue, counters);
+    hipLaunchKernelGGL((k_howfar<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, queue, counters,
+                       wl.tinfo);
   } else if (!perform) {
-    hipLaunchKernelGGL((k_howfar<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, queue, counters);
-  } else if (SortsProcesses(c, n, true)) {
+    hipLaunchKernelGGL((k_howfar<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, queue, counters,
+                       wl.tinfo);
+  } else if (SortsProcesses(c)) {
     // choice of the interaction, then the tile-local process sort and final
     // states (as hepem_step_run)
     const int tiles = (n + kClassifyTile - 1) / kClassifyTile;
     const int igrid = InteractGrid(n);
     if (fp64)
       hipLaunchKernelGGL((k_perform_select<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue,
-                         counters, wl.code, wl.st);
+                         counters, wl.tinfo, wl.code, wl.st);
     else
       hipLaunchKernelGGL((k_perform_select<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue,
-                         counters, wl.code, wl.st);
+                         counters, wl.tinfo, wl.code, wl.st);
     hipLaunchKernelGGL(k_classify_tiles, dim3(tiles), dim3(kClassifyBlock), 0, s, n, wl.code, wl.tlist, wl.tinfo);
     if (fp64)
       hipLaunchKernelGGL(k_interact_tiles<double>, dim3(igrid), dim3(kBlock), 0, s, c->t64, seed, *t, *res, wl.code, wl.st,
@@ -774,9 +775,11 @@ int SplitCall(const hepem_ctx* c, uint64_t seed, hepem_tracks* t, const hepem_ho
       hipLaunchKernelGGL(k_interact_tiles<float>, dim3(igrid), dim3(kBlock), 0, s, c->t32, seed, *t, *res, wl.code, wl.st,
                          wl.tlist, wl.tinfo);
   } else if (fp64) {
-    hipLaunchKernelGGL((k_perform<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue, counters);
+    hipLaunchKernelGGL((k_perform<double, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t64, seed, *t, *hs, *res, queue, counters,
+                       wl.tinfo);
   } else {
-    hipLaunchKernelGGL((k_perform<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue, counters);
+    hipLaunchKernelGGL((k_perform<float, KIND>), dim3(grid), dim3(kBlock), 0, s, c->t32, seed, *t, *hs, *res, queue, counters,
+                       wl.tinfo);
   }
   HIP_OK(hipGetLastError());
   return HEPEM_OK;
@@ -1077,7 +1080,7 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
   if (!p) return HEPEM_ERR_ALLOC;
   p->ctx = c;
   p->sortMode = c->processSort;
-  p->processSort = SortsProcesses(c, q);
+  p->processSort = SortsProcesses(c);
   p->nbatch = nbatch;
   p->total = q;
   p->ntiles = (uint32_t)tiles;

@@ -118,82 +118,6 @@ HD bool ResolveSlot(const hepem_tracks& t, int j, const uint32_t* queue, const u
   return KIND == kOnlyElectrons ? isE : pt == 2;
 }
 
-template <typename R, int KIND>
-__global__ void __launch_bounds__(kBlock, 4) k_howfar(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
-                                                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = t.n;
-  if (j >= n) return;
-  int i;
-  bool isE;
-  const bool kindOk = ResolveSlot<KIND>(t, j, queue, counters, i, isE);
-  if (!kindOk || !TrackValid(T, t, i)) {
-    atomicAdd(&counters[2], 1u);
-    hs.winner[i] = -99;
-    hs.geom_step[i] = 0.0;
-    return;
-  }
-  if (isE) {
-    ElState<R> el;
-    Rng rng;
-    InitElectron(t, n, i, seed, el, rng);
-    ElectronHowFar(T, el, rng);
-    StoreElectronState(t, n, i, el, rng);
-    StoreElectronHowFar(hs, n, i, el);
-  } else {
-    Trk<R> g;
-    g.isEl = false;
-    LoadTrack(t, n, i, g);
-    Rng rng;
-    rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
-    const R pe = GammaHowFar(T, g, rng);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) t.num_ia_left[k * n + i] = (double)g.nial[k];
-    StoreRng(t, i, rng);
-    hs.geom_step[i] = (double)g.gstep;
-    hs.mfp[i] = (double)g.mfp[0];
-    hs.winner[i] = g.winner;
-    hs.pe_xsec[i] = (double)pe;
-  }
-}
-
-template <typename R, int KIND>
-__global__ void __launch_bounds__(kBlock, 3) k_perform(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
-                                                    hepem_step_out res, const uint32_t* __restrict__ queue,
-                                                    uint32_t* __restrict__ counters) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = t.n;
-  if (j >= n) return;
-  int i;
-  bool isE;
-  const bool kindOk = ResolveSlot<KIND>(t, j, queue, counters, i, isE);
-  if (!kindOk || hs.winner[i] == -99) {
-    WriteRejected<R>(t, t, res, n, i);
-    return;
-  }
-  SecList<R> sl;
-  sl.ns = 0;
-  if (isE) {
-    ElState<R> el;
-    Rng rng;
-    InitElectron(t, n, i, seed, el, rng);
-    LoadElectronHowFar(hs, n, i, el);
-    ElectronPerform(T, el, sl, rng);
-    StoreElectronState(t, n, i, el, rng);
-    StoreElectronResult(t, res, n, i, el, sl);
-  } else {
-    Trk<R> g;
-    g.isEl = false;
-    LoadTrack(t, n, i, g);
-    g.gstep = (R)hs.geom_step[i];
-    g.mfp[0] = (R)hs.mfp[i];
-    Rng rng;
-    rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
-    GammaPerform(T, g, (R)hs.pe_xsec[i], sl, rng);
-    StoreGamma(t, res, n, i, g, sl, rng);
-  }
-}
-
 // ---- TrackElectron's sub-step loop driven by the caller (device/dev_tracking.hh) ----------------
 // counters[4]: tracks that take another sub-step after the last begin/along call
 
@@ -609,27 +533,6 @@ HD void GammaInteract(const DevTables<R>& T, uint64_t seed, hepem_tracks& out, c
   WriteSecondaries(res, n, i, sl);
 }
 
-// Stage 1 of the split Perform (as k_perform up to the choice of the
-// interaction).
-template <typename R, int KIND>
-__global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint64_t seed, hepem_tracks t,
-                                                           hepem_howfar_state hs, hepem_step_out res,
-                                                           const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
-                                                           uint8_t* __restrict__ code, Staged* __restrict__ st) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = t.n;
-  if (j >= n) return;
-  int i;
-  bool isE;
-  const bool kindOk = ResolveSlot<KIND>(t, j, queue, counters, i, isE);
-  if (!kindOk || hs.winner[i] == -99) {
-    WriteRejected<R>(t, t, res, n, i);
-    code[i] = kPNone;
-    return;
-  }
-  code[i] = isE ? PerformElectronSelect<R>(T, seed, t, hs, res, st, n, i) : PerformGammaSelect<R>(T, seed, t, hs, res, st, n, i);
-}
-
 // Process sort and final states, tile by tile (the single-batch two-stage
 // step and Perform).
 //
@@ -779,6 +682,126 @@ __global__ void __launch_bounds__(kBlock) k_step_select_local(DevTables<R> T, ui
   }
   CopyIdentity(in, out, i);
   code[i] = isE ? StepElectronSelect(T, seed, in, out, res, st, n, i) : StepGammaSelect(T, seed, in, out, res, st, n, i);
+}
+
+// Thread -> (slot, is-e-/e+) of the split HowFar and Perform kernels. A
+// mixed batch's queue is read in the layout its hepem_howfar prepare chose
+// (counters[kCounterLayout]): tile-local, with the blocks placed as
+// k_step_select_local places them (grid LocalSelectGrid(n)), or e-/e+ first
+// over the batch (one thread per entry). False if the thread has no slot;
+// kindOk false if the slot holds a track of the other kind.
+template <int KIND>
+__device__ inline bool ThreadSlot(const hepem_tracks& t, const uint32_t* __restrict__ queue,
+                                  const uint32_t* __restrict__ counters, const uint32_t* __restrict__ tinfo, int& i,
+                                  bool& isE, bool& kindOk) {
+  const int n = t.n;
+  if (KIND == kMixed && counters[kCounterLayout]) {
+    int tile, chunk;
+    if (!LocalSelectChunk((n + kClassifyTile - 1) / kClassifyTile, tile, chunk)) return false;
+    const int tile0 = tile * kClassifyTile, jl = chunk * kBlock + (int)threadIdx.x;
+    if (tile0 + jl >= n) return false;
+    i = (int)queue[tile0 + jl];
+    isE = jl < (int)tinfo[(size_t)tile * kTileInfo];
+    kindOk = true;
+    return true;
+  }
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return false;
+  kindOk = ResolveSlot<KIND>(t, j, queue, counters, i, isE);
+  return true;
+}
+
+template <typename R, int KIND>
+__global__ void __launch_bounds__(kBlock, 4) k_howfar(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
+                                                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ tinfo) {
+  const int n = t.n;
+  int i;
+  bool isE, kindOk;
+  if (!ThreadSlot<KIND>(t, queue, counters, tinfo, i, isE, kindOk)) return;
+  if (!kindOk || !TrackValid(T, t, i)) {
+    atomicAdd(&counters[2], 1u);
+    hs.winner[i] = -99;
+    hs.geom_step[i] = 0.0;
+    return;
+  }
+  if (isE) {
+    ElState<R> el;
+    Rng rng;
+    InitElectron(t, n, i, seed, el, rng);
+    ElectronHowFar(T, el, rng);
+    StoreElectronState(t, n, i, el, rng);
+    StoreElectronHowFar(hs, n, i, el);
+  } else {
+    Trk<R> g;
+    g.isEl = false;
+    LoadTrack(t, n, i, g);
+    Rng rng;
+    rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
+    const R pe = GammaHowFar(T, g, rng);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t.num_ia_left[k * n + i] = (double)g.nial[k];
+    StoreRng(t, i, rng);
+    hs.geom_step[i] = (double)g.gstep;
+    hs.mfp[i] = (double)g.mfp[0];
+    hs.winner[i] = g.winner;
+    hs.pe_xsec[i] = (double)pe;
+  }
+}
+
+template <typename R, int KIND>
+__global__ void __launch_bounds__(kBlock, 3) k_perform(DevTables<R> T, uint64_t seed, hepem_tracks t, hepem_howfar_state hs,
+                                                    hepem_step_out res, const uint32_t* __restrict__ queue,
+                                                    uint32_t* __restrict__ counters, const uint32_t* __restrict__ tinfo) {
+  const int n = t.n;
+  int i;
+  bool isE, kindOk;
+  if (!ThreadSlot<KIND>(t, queue, counters, tinfo, i, isE, kindOk)) return;
+  if (!kindOk || hs.winner[i] == -99) {
+    WriteRejected<R>(t, t, res, n, i);
+    return;
+  }
+  SecList<R> sl;
+  sl.ns = 0;
+  if (isE) {
+    ElState<R> el;
+    Rng rng;
+    InitElectron(t, n, i, seed, el, rng);
+    LoadElectronHowFar(hs, n, i, el);
+    ElectronPerform(T, el, sl, rng);
+    StoreElectronState(t, n, i, el, rng);
+    StoreElectronResult(t, res, n, i, el, sl);
+  } else {
+    Trk<R> g;
+    g.isEl = false;
+    LoadTrack(t, n, i, g);
+    g.gstep = (R)hs.geom_step[i];
+    g.mfp[0] = (R)hs.mfp[i];
+    Rng rng;
+    rng.init(seed, (uint32_t)t.id[i], StreamHi(t, i), t.rng_ctr[i], t.rng_has_gauss[i] != 0, t.rng_gauss[i]);
+    GammaPerform(T, g, (R)hs.pe_xsec[i], sl, rng);
+    StoreGamma(t, res, n, i, g, sl, rng);
+  }
+}
+
+// Stage 1 of the split Perform (as k_perform up to the choice of the
+// interaction).
+template <typename R, int KIND>
+__global__ void __launch_bounds__(kBlock) k_perform_select(DevTables<R> T, uint64_t seed, hepem_tracks t,
+                                                           hepem_howfar_state hs, hepem_step_out res,
+                                                           const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
+                                                           const uint32_t* __restrict__ tinfo, uint8_t* __restrict__ code,
+                                                           Staged* __restrict__ st) {
+  const int n = t.n;
+  int i;
+  bool isE, kindOk;
+  if (!ThreadSlot<KIND>(t, queue, counters, tinfo, i, isE, kindOk)) return;
+  if (!kindOk || hs.winner[i] == -99) {
+    WriteRejected<R>(t, t, res, n, i);
+    code[i] = kPNone;
+    return;
+  }
+  code[i] = isE ? PerformElectronSelect<R>(T, seed, t, hs, res, st, n, i) : PerformGammaSelect<R>(T, seed, t, hs, res, st, n, i);
 }
 
 // The fused step over either particle layout of the queue (the one

@@ -244,15 +244,12 @@ int  hepem_step_run(const hepem_ctx* ctx, uint64_t seed, const hepem_tracks* in,
 /* hepem_step_run, batch plans and the split Perform calls (hepem_perform,
  * hepem_*_perform) in one fused kernel (HEPEM_SORT_OFF), with the process sort
  * (HEPEM_SORT_ON), or with the faster of the two for the call as measured on
- * MI355X (HEPEM_SORT_AUTO, the default): steps and batch plans sort at every
- * size in both forms; the split Perform calls sort at every size in the
- * HEPEM_FP64 form and, in the HEPEM_FP32T form, below
- * HEPEM_SORT_AUTO_FP32_MAX_TRACKS slots. Results are identical; only the
- * speed differs. */
+ * MI355X (HEPEM_SORT_AUTO, the default), which is the process sort for every
+ * call at every size in both forms. Results are identical; only the speed
+ * differs. */
 #define HEPEM_SORT_OFF  0
 #define HEPEM_SORT_ON   1
 #define HEPEM_SORT_AUTO 2
-#define HEPEM_SORT_AUTO_FP32_MAX_TRACKS 65536
 int  hepem_ctx_set_process_sort(hepem_ctx* ctx, int mode);
 /* A speed switch of hepem_lookup_eloss_xsec, like the process sort above:
  * the particle's loss and cross-section tables staged in LDS (on = 1, the
