@@ -915,6 +915,8 @@ struct hepem_batch_plan {
   uint32_t* dLocal = nullptr;     // e-/e+ count per particle-classify tile of the tile-local partition
   uint32_t* dTileBatch = nullptr; // batch of every tile, one array per tile size TileSlots(t), from tbOff[t]
   uint64_t tbOff[kNumTiles] = {0, 0, 0, 0};
+  uint32_t* dChunkBatch = nullptr;  // batch of every step chunk, one array per block size ShapeBlock(s), from cbOff[s]
+  uint64_t cbOff[kNumShapes] = {0, 0, 0};
   int shape = kNumShapes - 1;     // block size ShapeBlock(shape)
   int tile = 2;                   // process-sort tile size TileSlots(tile)
   bool persistent = false;        // step kernels as one resident round of blocks taking work from counters
@@ -942,22 +944,22 @@ void LaunchStepBatched(const DevTables<R>& T, const hepem_batch_plan* p, uint64_
   uint32_t* next = p->dCounters + 4 * p->nbatch;
   if (!p->processSort) {
     if (p->persistent)
-      hipLaunchKernelGGL((k_step_batched_q<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+      hipLaunchKernelGGL((k_step_batched_q<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->dChunkBatch + p->cbOff[S],
                          p->nchunks[S], p->dQueue, p->dCounters, next);
     else
-      hipLaunchKernelGGL((k_step_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+      hipLaunchKernelGGL((k_step_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->dChunkBatch + p->cbOff[S],
                          p->nchunks[S], p->dQueue, p->dCounters);
     return;
   }
   if (p->persistent)
     hipLaunchKernelGGL((k_step_select_batched_q<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs,
-                       p->nbatch, p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged, next);
+                       p->dChunkBatch + p->cbOff[S], p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged, next);
   else if (TileLocal(p))
     hipLaunchKernelGGL((k_step_select_local_batched<R>), dim3(p->grid), dim3(kBlock), 0, s, T,
                        seed, p->dDescs, p->dTileBatch + p->tbOff[kParticleTiles], (int)p->ntiles, p->dQueue, p->dCounters,
                        p->dLocal, p->dCode, p->dStaged);
   else
-    hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->nbatch,
+    hipLaunchKernelGGL((k_step_select_batched<R, S>), dim3(p->grid), dim3(ShapeBlock(S)), 0, s, T, seed, p->dDescs, p->dChunkBatch + p->cbOff[S],
                        p->nchunks[S], p->dQueue, p->dCounters, p->dCode, p->dStaged);
   const uint32_t nt = p->nptiles[TI];
   hipLaunchKernelGGL(k_classify_tiles_batched<TI>, dim3(nt), dim3(kClassifyBlock), 0, s, p->dDescs, p->dTileBatch + p->tbOff[TI], p->dCode,
@@ -1029,6 +1031,7 @@ void FreePlan(hepem_batch_plan* p) {
   if (p->dTInfo) (void)hipFree(p->dTInfo);
   if (p->dLocal) (void)hipFree(p->dLocal);
   if (p->dTileBatch) (void)hipFree(p->dTileBatch);
+  if (p->dChunkBatch) (void)hipFree(p->dChunkBatch);
   delete p;
 }
 }  // namespace
@@ -1058,15 +1061,23 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
     for (int s = 0; s < kNumShapes; ++s) chunks[s] += (n + ShapeBlock(s) - 1) / ShapeBlock(s);
   }
   if (q >= (1ull << 31)) return HEPEM_ERR_CAPACITY;
-  // the batch of every tile, per tile size: the tile kernels look it up
-  // instead of searching the descriptors' prefix sums
-  std::vector<uint32_t> tileBatch;
-  uint64_t tbOff[kNumTiles];
+  // the batch of every tile, per tile size, and of every step chunk, per
+  // block size: the kernels look it up instead of searching the descriptors'
+  // prefix sums
+  std::vector<uint32_t> tileBatch, chunkBatch;
+  uint64_t tbOff[kNumTiles], cbOff[kNumShapes];
   for (int t = 0; t < kNumTiles; ++t) {
     tbOff[t] = tileBatch.size();
     for (int b = 0; b < nbatch; ++b) {
       const uint64_t k = (b + 1 < nbatch ? (uint64_t)h[b + 1].ptile0[t] : ptiles[t]) - h[b].ptile0[t];
       tileBatch.insert(tileBatch.end(), (size_t)k, (uint32_t)b);
+    }
+  }
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    cbOff[sh] = chunkBatch.size();
+    for (int b = 0; b < nbatch; ++b) {
+      const uint64_t k = (b + 1 < nbatch ? (uint64_t)h[b + 1].chunk0[sh] : chunks[sh]) - h[b].chunk0[sh];
+      chunkBatch.insert(chunkBatch.end(), (size_t)k, (uint32_t)b);
     }
   }
   BatchDev& last = h[nbatch];
@@ -1095,14 +1106,18 @@ extern "C" int hepem_batch_plan_create(const hepem_ctx* c, const hepem_batch_des
       hipMalloc(&p->dTList, (q > 0 ? q : 1) * sizeof(uint16_t)) != hipSuccess ||
       hipMalloc(&p->dTInfo, (size_t)kTileInfo * (ptiles[0] > 0 ? ptiles[0] : 1) * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&p->dLocal, (tiles > 0 ? tiles : 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&p->dTileBatch, std::max<size_t>(1, tileBatch.size()) * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&p->dTileBatch, std::max<size_t>(1, tileBatch.size()) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&p->dChunkBatch, std::max<size_t>(1, chunkBatch.size()) * sizeof(uint32_t)) != hipSuccess) {
     FreePlan(p);
     return HEPEM_ERR_ALLOC;
   }
   for (int t = 0; t < kNumTiles; ++t) p->tbOff[t] = tbOff[t];
+  for (int sh = 0; sh < kNumShapes; ++sh) p->cbOff[sh] = cbOff[sh];
   if (hipMemcpy(p->dDescs, h.data(), h.size() * sizeof(BatchDev), hipMemcpyHostToDevice) != hipSuccess ||
       (!tileBatch.empty() && hipMemcpy(p->dTileBatch, tileBatch.data(), tileBatch.size() * sizeof(uint32_t),
-                                       hipMemcpyHostToDevice) != hipSuccess)) {
+                                       hipMemcpyHostToDevice) != hipSuccess) ||
+      (!chunkBatch.empty() && hipMemcpy(p->dChunkBatch, chunkBatch.data(), chunkBatch.size() * sizeof(uint32_t),
+                                        hipMemcpyHostToDevice) != hipSuccess)) {
     FreePlan(p);
     return HEPEM_ERR_HIP;
   }

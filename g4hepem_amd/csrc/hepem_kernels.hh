@@ -959,16 +959,6 @@ struct BatchDev {
   uint32_t chunk0[kNumShapes];   // first step chunk for block size ShapeBlock(s)
 };
 
-// Largest b in [0, nbatch) with key(d[b]) <= x (binary search over a prefix).
-template <typename Key> HD int FindBatch(const BatchDev* __restrict__ d, int nbatch, uint32_t x, Key key) {
-  int lo = 0, hi = nbatch - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (key(d[mid]) <= x) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
 
 // bcounters[4b + 0..2]: e-/e+, gammas, rejected tracks of batch b
 __global__ void __launch_bounds__(kClassifyBlock) k_classify_batched(const BatchDev* __restrict__ d,
@@ -984,10 +974,11 @@ __global__ void __launch_bounds__(kClassifyBlock) k_classify_batched(const Batch
 // batch index is uniform over the block: it (and the descriptor loads) stay
 // scalar.
 template <typename R, int S>
-__device__ void StepChunk(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d, int nbatch, uint32_t c,
+__device__ void StepChunk(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d,
+                          const uint32_t* __restrict__ cbatch, uint32_t c,
                           const uint32_t* __restrict__ queue, uint32_t* __restrict__ bcounters) {
   constexpr int BS = ShapeBlock(S);
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, c, [](const BatchDev& x) { return x.chunk0[S]; }));
+  const int b = __builtin_amdgcn_readfirstlane((int)cbatch[c]);
   const hepem_tracks in = d[b].in;
   hepem_tracks out = d[b].out;
   const hepem_step_out res = d[b].res;
@@ -1008,11 +999,12 @@ __device__ void StepChunk(const DevTables<R>& T, uint64_t seed, const BatchDev* 
 // them tile by tile, as the single-batch step does (tiles numbered across
 // the batches).
 template <typename R, int S>
-__device__ void StepSelectChunk(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d, int nbatch, uint32_t c,
+__device__ void StepSelectChunk(const DevTables<R>& T, uint64_t seed, const BatchDev* __restrict__ d,
+                                const uint32_t* __restrict__ cbatch, uint32_t c,
                                 const uint32_t* __restrict__ queue, uint32_t* __restrict__ bcounters,
                                 uint8_t* __restrict__ code, Staged* __restrict__ st) {
   constexpr int BS = ShapeBlock(S);
-  const int b = __builtin_amdgcn_readfirstlane(FindBatch(d, nbatch, c, [](const BatchDev& x) { return x.chunk0[S]; }));
+  const int b = __builtin_amdgcn_readfirstlane((int)cbatch[c]);
   const hepem_tracks in = d[b].in;
   hepem_tracks out = d[b].out;
   const hepem_step_out res = d[b].res;
@@ -1094,9 +1086,10 @@ __device__ inline uint32_t NextItem(uint32_t* next) {
 // Dispatched form: block c takes chunk c.
 template <typename R, int S>
 __global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
-                                                                int nbatch, uint32_t nchunks, const uint32_t* __restrict__ queue,
+                                                                const uint32_t* __restrict__ cbatch, uint32_t nchunks,
+                                                                const uint32_t* __restrict__ queue,
                                                                 uint32_t* __restrict__ bcounters) {
-  if (blockIdx.x < nchunks) StepChunk<R, S>(T, seed, d, nbatch, blockIdx.x, queue, bcounters);
+  if (blockIdx.x < nchunks) StepChunk<R, S>(T, seed, d, cbatch, blockIdx.x, queue, bcounters);
 }
 
 // Persistent form: one resident round of blocks, each taking chunks from the
@@ -1104,30 +1097,30 @@ __global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched(DevTables<R> T, 
 // index is carried from one chunk to the next.
 template <typename R, int S>
 __global__ void __launch_bounds__(ShapeBlock(S)) k_step_batched_q(DevTables<R> T, uint64_t seed, const BatchDev* __restrict__ d,
-                                                                  int nbatch, uint32_t nchunks,
+                                                                  const uint32_t* __restrict__ cbatch, uint32_t nchunks,
                                                                   const uint32_t* __restrict__ queue,
                                                                   uint32_t* __restrict__ bcounters, uint32_t* __restrict__ next) {
-  for (uint32_t c = NextItem(next); c < nchunks; c = NextItem(next)) StepChunk<R, S>(T, seed, d, nbatch, c, queue, bcounters);
+  for (uint32_t c = NextItem(next); c < nchunks; c = NextItem(next)) StepChunk<R, S>(T, seed, d, cbatch, c, queue, bcounters);
 }
 
 template <typename R, int S>
 __global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched(DevTables<R> T, uint64_t seed,
-                                                                       const BatchDev* __restrict__ d, int nbatch,
+                                                                       const BatchDev* __restrict__ d, const uint32_t* __restrict__ cbatch,
                                                                        uint32_t nchunks, const uint32_t* __restrict__ queue,
                                                                        uint32_t* __restrict__ bcounters,
                                                                        uint8_t* __restrict__ code, Staged* __restrict__ st) {
-  if (blockIdx.x < nchunks) StepSelectChunk<R, S>(T, seed, d, nbatch, blockIdx.x, queue, bcounters, code, st);
+  if (blockIdx.x < nchunks) StepSelectChunk<R, S>(T, seed, d, cbatch, blockIdx.x, queue, bcounters, code, st);
 }
 
 template <typename R, int S>
 __global__ void __launch_bounds__(ShapeBlock(S)) k_step_select_batched_q(DevTables<R> T, uint64_t seed,
-                                                                         const BatchDev* __restrict__ d, int nbatch,
+                                                                         const BatchDev* __restrict__ d, const uint32_t* __restrict__ cbatch,
                                                                          uint32_t nchunks, const uint32_t* __restrict__ queue,
                                                                          uint32_t* __restrict__ bcounters,
                                                                          uint8_t* __restrict__ code, Staged* __restrict__ st,
                                                                          uint32_t* __restrict__ next) {
   for (uint32_t c = NextItem(next); c < nchunks; c = NextItem(next))
-    StepSelectChunk<R, S>(T, seed, d, nbatch, c, queue, bcounters, code, st);
+    StepSelectChunk<R, S>(T, seed, d, cbatch, c, queue, bcounters, code, st);
 }
 
 // The tile-local process sort over all batches of a plan, in tiles of
