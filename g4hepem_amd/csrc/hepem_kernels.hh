@@ -660,7 +660,10 @@ __device__ inline bool LocalSelectChunk(int ntiles, int& tile, int& chunk) {
   return true;
 }
 
-// Stage 1 of the step over the tile-local particle partition of one batch.
+// Stage 1 of the step over one batch's particle queue, its blocks placed tile by
+// tile as LocalSelectChunk places them (the tile-local partition that
+// hepem_step_prepare lays out with the process sort; a queue laid out over the
+// whole batch is read correctly too).
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_step_select_local(DevTables<R> T, uint64_t seed, hepem_tracks in, hepem_tracks out,
                                                         hepem_step_out res, const uint32_t* __restrict__ queue,
