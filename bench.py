@@ -119,6 +119,40 @@ def time_steps(ctx, tin, tout, res, work, steps, warmup, seed):
     return total_ms / steps, float(np.mean(run_ms)), wall
 
 
+def time_step_graph(ctx, tin, tout, res, work, steps, seed):
+    """One step (hepem_step_prepare + hepem_step_run) captured in a hipGraph
+    and replayed `steps` times, timed with HIP events on the replay stream.
+    The step is out of place, so every replay repeats the same step: the last
+    replay's outputs are compared with an eager step's afterwards. Returns
+    (ms per step, identical)."""
+    import torch
+    from g4hepem_amd.tracks import StepOutSoA, TrackSoA, step_outputs_differ
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            ctx.step_prepare(tin, work, stream=s)
+            ctx.step_run(seed, tin, tout, res, work, stream=s)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(s):
+        a.record(s)
+        for _ in range(steps):
+            g.replay()
+        b.record(s)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    ref_t, ref_r = TrackSoA(tin.n, "torch", tin["ekin"].device), StepOutSoA(tin.n, "torch", tin["ekin"].device)
+    ctx.step(seed, tin, ref_t, ref_r, work)
+    torch.cuda.synchronize()
+    same = not step_outputs_differ(tin["ptype"], tout, res, ref_t, ref_r)
+    return ms, same
+
+
 def time_split(ctx, tin, res, work, steps, warmup, seed):
     """Times hepem_howfar + hepem_perform on a copy of `tin` (the split calls
     step in place; the caller's geometry would run between them: here it
@@ -632,6 +666,13 @@ def main():
             ordered[form] = {"value": n / (mso * 1e-3), "unit": "track-steps/s", "ms_per_step": mso, "kernel_ms": runo}
         line["config2_type_ordered"] = ordered
         del tin_o
+        log("config-2 step replayed from a hipGraph")
+        graph = {"launch": "hepem_step_prepare + hepem_step_run captured once, replayed"}
+        for form, ctx in (("fp64", ctx64), ("fp32_table_form", ctx32)):
+            msg, same = time_step_graph(ctx, tin, tout, res, work, args.steps, args.seed)
+            graph[form] = {"value": n / (msg * 1e-3), "unit": "track-steps/s", "ms_per_step": msg,
+                           "identical_to_eager": same}
+        line["config2_graph_replay"] = graph
         log("config-2 tracks through the split HowFar and Perform calls")
         split = {"calls": "hepem_howfar, then hepem_perform (in place, no geometry between them)"}
         for form, ctx in (("fp64", ctx64), ("fp32_table_form", ctx32)):
